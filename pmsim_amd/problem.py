@@ -1,0 +1,291 @@
+"""Problem description: parameters, boundary conditions and configuration.
+
+Mirrors pmsim's builder API for the solid / von Mises path so that a driver reads like the
+reference's tests (``tests/test_spo_754_footing.rs:40-93``):
+
+* ``StressStrain.VonMises/LinearElastic`` and ``ParamSolid``  -- ``src/base/parameters.rs:6-68,276-285``
+* ``Essential.points/edges/faces(_fn)``                        -- ``src/base/essential.rs:38-135``
+* ``Natural.points/edges/faces``                               -- ``src/base/natural.rs:50-140``
+* ``Config`` (defaults and setters used on the path)           -- ``src/base/config.rs:184-231,366-664``
+* ``FemBase`` (attribute -> element map, DOF count)            -- ``src/fem/fem_base.rs:25-30``
+
+Multiplier functions ``m(t)`` are host callbacks in pmsim (``essential.rs:94-120``); here they are
+evaluated once per time station into value tables that the C ABI consumes, since no host
+callback may run on the device hot path.
+"""
+from __future__ import annotations
+
+import dataclasses
+import enum
+import math
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .mesh import Facet, Mesh
+
+
+class Dof(enum.IntEnum):
+    Ux = 0
+    Uy = 1
+    Uz = 2
+
+
+class Pbc(enum.IntEnum):
+    """Point (concentrated) loads."""
+    Fx = 0
+    Fy = 1
+    Fz = 2
+
+
+class Nbc(enum.IntEnum):
+    """Distributed boundary loads (solid subset)."""
+    Qn = 0
+    Qx = 1
+    Qy = 2
+    Qz = 3
+
+
+MODEL_LINEAR_ELASTIC = 0
+MODEL_VON_MISES = 1
+
+
+@dataclasses.dataclass(frozen=True)
+class StressStrain:
+    model: int
+    young: float
+    poisson: float
+    hh: float = 0.0
+    z_ini: float = 0.0
+
+    @staticmethod
+    def LinearElastic(young: float, poisson: float) -> "StressStrain":
+        return StressStrain(MODEL_LINEAR_ELASTIC, young, poisson)
+
+    @staticmethod
+    def VonMises(young: float, poisson: float, hh: float, z_ini: float) -> "StressStrain":
+        return StressStrain(MODEL_VON_MISES, young, poisson, hh, z_ini)
+
+
+@dataclasses.dataclass(frozen=True)
+class ParamSolid:
+    density: float
+    stress_strain: StressStrain
+    ngauss: Optional[int] = None
+
+
+class Elem:
+    @staticmethod
+    def Solid(p: ParamSolid) -> ParamSolid:
+        return p
+
+
+class Essential:
+    """Prescribed values ``value = c * m(t)``; a later call overrides an earlier one for the same (point, dof)."""
+
+    def __init__(self):
+        self.all: Dict[Tuple[int, int], Tuple[float, Optional[int]]] = {}
+        self.functions: List[Callable[[float], float]] = []
+
+    def _put(self, pts, dof, c, f_index):
+        for p in pts:
+            self.all[(int(p), int(dof))] = (float(c), f_index)
+        return self
+
+    def point(self, p, dof, value):
+        return self._put([p], dof, value, None)
+
+    def points(self, pts, dof, value):
+        return self._put(pts, dof, value, None)
+
+    def edges(self, facets: Sequence[Facet], dof, value):
+        return self._put([p for f in facets for p in f.points], dof, value, None)
+
+    faces = edges
+
+    def _fn(self, pts, dof, c, m):
+        self.functions.append(m)
+        return self._put(pts, dof, c, len(self.functions) - 1)
+
+    def points_fn(self, pts, dof, c, m):
+        return self._fn(pts, dof, c, m)
+
+    def edges_fn(self, facets, dof, c, m):
+        return self._fn([p for f in facets for p in f.points], dof, c, m)
+
+    faces_fn = edges_fn
+
+    def size(self):
+        return len(self.all)
+
+    def has_non_zero(self):
+        return any(c != 0.0 for (c, _) in self.all.values())
+
+    def tables(self, times: np.ndarray):
+        """Sorted (point, dof) keys and the (n × nstation) value table at the given times."""
+        keys = sorted(self.all.keys())
+        tab = np.zeros((len(keys), len(times)))
+        for i, k in enumerate(keys):
+            c, fi = self.all[k]
+            if fi is None:
+                tab[i, :] = c
+            else:
+                f = self.functions[fi]
+                tab[i, :] = [c * f(float(t)) for t in times]
+        pts = np.asarray([k[0] for k in keys], dtype=np.int32)
+        dofs = np.asarray([k[1] for k in keys], dtype=np.int32)
+        return pts, dofs, tab
+
+
+class Natural:
+    """Concentrated point loads and normal distributed loads (Qn) on boundary facets."""
+
+    def __init__(self):
+        self.at_points: List[Tuple[int, int, float]] = []
+        self.on_facets: List[Tuple[Facet, int, float]] = []
+
+    def point(self, p, pbc, value):
+        self.at_points.append((int(p), int(pbc), float(value)))
+        return self
+
+    def points(self, pts, pbc, value):
+        for p in pts:
+            self.point(p, pbc, value)
+        return self
+
+    def edge(self, f: Facet, nbc, value):
+        self.on_facets.append((f, int(nbc), float(value)))
+        return self
+
+    def edges(self, facets, nbc, value):
+        for f in facets:
+            self.edge(f, nbc, value)
+        return self
+
+    face = edge
+    faces = edges
+
+
+class Config:
+    """Solver configuration (defaults of ``src/base/config.rs:184-231``)."""
+
+    def __init__(self, mesh: Mesh):
+        self.ndim = mesh.ndim
+        self.linear_problem = False
+        self.constant_tangent = False
+        self.allowed_step_n_failure = 100
+        self.lagrange_mult_method = False
+        self.alt_bb_matrix_method = False
+        self.symmetry_check_tolerance: Optional[float] = 1e-10
+        self.axisymmetric = False
+        self.plane_stress = False
+        self.thickness = 1.0
+        self.t_ini = 0.0
+        self.t_fin = 1.0
+        self.ddt: Callable[[float], float] = lambda t: 1.0
+        self.ddt_out: Callable[[float], float] = lambda t: 1.0
+        self.ddt_min = 1e-10
+        self.n_max_time_steps = 1000
+        self.n_max_iterations = 10
+        self.tol_rr_abs = 1e-10
+        self.tol_mdu_rel = 1e-8
+        # MI355X path selection (not in pmsim): linear solver and tangent precision
+        self.lin_sol = "pcg"          # "pcg" (device FP64 block-Jacobi PCG) or "host" (host direct)
+        self.tangent_precision = "f64"  # "f64", "f32" or "bf16" (inexact-Newton tangent)
+        self.pcg_tol = 1e-12
+        self.pcg_max_iter = 20000
+        self.scaled_rr = 0.0           # > 0 enables ||R||inf < scaled_rr * ||F_int||inf as an extra test
+
+    # setters (config.rs:366-664)
+    def set_linear_problem(self, v=True): self.linear_problem = bool(v); return self
+    def set_constant_tangent(self, v=True): self.constant_tangent = bool(v); return self
+    def set_allowed_step_n_failure(self, n): self.allowed_step_n_failure = int(n); return self
+    def set_lagrange_mult_method(self, v=True): self.lagrange_mult_method = bool(v); return self
+    def set_alt_bb_matrix_method(self, v=True): self.alt_bb_matrix_method = bool(v); return self
+    def set_symmetry_check_tolerance(self, tol): self.symmetry_check_tolerance = tol; return self
+    def set_axisymmetric(self):
+        if self.ndim != 2:
+            raise ValueError("axisymmetric requires ndim = 2")
+        self.axisymmetric = True; self.plane_stress = False; return self
+    def set_plane_stress(self, thickness):
+        if self.ndim != 2:
+            raise ValueError("plane-stress requires ndim = 2")
+        self.plane_stress = True; self.axisymmetric = False; self.thickness = float(thickness); return self
+    def set_t_ini(self, t): self.t_ini = float(t); return self
+    def set_t_fin(self, t): self.t_fin = float(t); return self
+    def set_dt(self, f): self.ddt = f; return self
+    def set_dt_out(self, f): self.ddt_out = f; return self
+    def set_dt_min(self, v): self.ddt_min = float(v); return self
+    def set_n_max_time_steps(self, n): self.n_max_time_steps = int(n); return self
+    def set_n_max_iterations(self, n): self.n_max_iterations = int(n); return self
+    def set_tol_rr_abs(self, v): self.tol_rr_abs = float(v); return self
+    def set_tol_mdu_rel(self, v): self.tol_mdu_rel = float(v); return self
+
+    def set_incremental(self, n_station: int):
+        """``config.rs:538-546``: t_ini = 0, Δt = 1, t_fin = n_station - 1."""
+        self.set_t_ini(0.0).set_dt(lambda t: 1.0).set_dt_out(lambda t: 1.0)
+        return self.set_t_fin(float(n_station - 1) if n_station > 1 else 0.0)
+
+    # MI355X-specific knobs
+    def set_lin_sol(self, kind: str): self.lin_sol = kind; return self
+    def set_tangent_precision(self, p: str): self.tangent_precision = p; return self
+    def set_pcg(self, tol=1e-12, max_iter=20000): self.pcg_tol = float(tol); self.pcg_max_iter = int(max_iter); return self
+    def set_scaled_rr(self, v): self.scaled_rr = float(v); return self
+
+    def stations(self) -> Tuple[float, np.ndarray]:
+        """Constant Δt (evaluated at t_ini) and the time stations t_k = t_ini + k Δt covering [t_ini, t_fin]."""
+        dt = float(self.ddt(self.t_ini))
+        if dt < self.ddt_min:
+            raise ValueError("Δt is smaller than the allowed minimum")
+        n = int(math.floor((self.t_fin - self.t_ini) / dt + 1e-9)) + 1
+        return dt, self.t_ini + dt * np.arange(max(n, 1))
+
+
+class FemBase:
+    """Attribute -> ParamSolid map and DOF numbering (pure solid: eq = ndim*p + d)."""
+
+    def __init__(self, mesh: Mesh, amap: Sequence[Tuple[int, ParamSolid]]):
+        self.mesh = mesh
+        self.amap: Dict[int, ParamSolid] = {int(a): p for (a, p) in amap}
+        for a in np.unique(mesh.attrs):
+            if int(a) not in self.amap:
+                raise ValueError("cannot find attribute in the element map")
+        used = np.zeros(mesh.npoint, dtype=bool)
+        used[mesh.cell_pts] = True
+        if not used.all():
+            raise ValueError("all points must belong to at least one cell")
+        self.ndof = mesh.ndim * mesh.npoint
+
+    def eq(self, point: int, dof: int) -> int:
+        if dof >= self.mesh.ndim:
+            raise ValueError("cannot find (PointId, Dof) in the DOF map")
+        return self.mesh.ndim * int(point) + int(dof)
+
+
+def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, natural: Natural) -> dict:
+    """Flat description of a problem (plain arrays) consumed by the C ABI and by the oracle."""
+    dt, times = config.stations()
+    pts, dofs, tab = essential.tables(times)
+    params = {}
+    for a, p in base.amap.items():
+        ss = p.stress_strain
+        params[a] = (ss.model, ss.young, ss.poisson, ss.hh, ss.z_ini, int(p.ngauss or 0))
+    pl = [(p, d, v) for (p, d, v) in natural.at_points]
+    facets = []
+    for (f, nbc, v) in natural.on_facets:
+        if nbc != Nbc.Qn:
+            raise ValueError("only Qn distributed loads are supported on the solid path")
+        facets.append((f.kind, tuple(f.points), f.cell, v))
+    return dict(
+        ndim=mesh.ndim, coords=mesh.coords, kinds=mesh.kinds, attrs=mesh.attrs,
+        cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts, params=params,
+        axisym=config.axisymmetric, plane_stress=config.plane_stress, thickness=config.thickness,
+        lmm=config.lagrange_mult_method, linear_problem=config.linear_problem,
+        t_ini=config.t_ini, t_fin=config.t_fin, dt=dt, n_max_iterations=config.n_max_iterations,
+        tol_rr_abs=config.tol_rr_abs, tol_mdu_rel=config.tol_mdu_rel,
+        ess_points=pts, ess_dofs=dofs, ess_table=tab,
+        pl_points=np.asarray([x[0] for x in pl], dtype=np.int32),
+        pl_dofs=np.asarray([x[1] for x in pl], dtype=np.int32),
+        pl_values=np.asarray([x[2] for x in pl], dtype=np.float64),
+        facets=facets,
+    )

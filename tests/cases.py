@@ -1,0 +1,219 @@
+"""Problem builders shared by the tests: each mirrors one of the reference's integration tests
+or one of the benchmark configurations (mesh, parameters, BCs, config)."""
+import gzip
+import json
+import math
+import os
+
+import numpy as np
+
+from pmsim_amd import (At, Config, Dof, Elem, Essential, Features, FemBase, Mesh, Natural, Nbc, ParamSolid, Pbc,
+                       StressStrain, any_x, describe)
+from pmsim_amd import generators
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_spo(name):
+    with gzip.open(os.path.join(GOLDEN, name + ".json.gz"), "rt") as f:
+        return json.load(f)["all"]
+
+
+def smith_mesh(name):
+    d = json.load(open(os.path.join(GOLDEN, "smith_meshes.json")))[name]
+    return Mesh(d["ndim"], d["coords"], [tuple(c) for c in d["cells"]])
+
+
+def smith_u(name):
+    return np.asarray(json.load(open(os.path.join(GOLDEN, "smith_u.json")))[name])
+
+
+def one_qua4():
+    return Mesh(2, [[0, 0], [1, 0], [1, 1], [0, 1]], [(1, "qua4", [0, 1, 2, 3])])
+
+
+def block_2d_four_qua8():
+    X = [[0, 0], [1, 0], [1, 1], [0, 1], [0.5, 0], [1, 0.5], [0.5, 1], [0, 0.5], [2, 0], [2, 1], [1.5, 0], [2, 0.5],
+         [1.5, 1], [1, 2], [0, 2], [1, 1.5], [0.5, 2], [0, 1.5], [2, 2], [2, 1.5], [1.5, 2]]
+    cells = [(1, "qua8", [0, 1, 2, 3, 4, 5, 6, 7]), (1, "qua8", [1, 8, 9, 2, 10, 11, 12, 5]),
+             (1, "qua8", [3, 2, 13, 14, 6, 15, 16, 17]), (1, "qua8", [2, 9, 18, 13, 12, 19, 20, 15])]
+    return Mesh(2, X, cells)
+
+
+def von_mises_square(mesh, top_y, ngauss):
+    """tests/test_von_mises_single_element_2d.rs:61-113 and tests/test_von_mises_2x2_elements_2d.rs:75-127."""
+    young, poisson, z_ini = 1500.0, 0.25, 9.0
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0), any_x)
+    bottom = feat.search_edges(At.Y(0.0), any_x)
+    top = feat.search_edges(At.Y(top_y), any_x)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(young, poisson, 800.0, z_ini), ngauss)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    nu2 = poisson * poisson
+    delta_y = z_ini * (1.0 - nu2) / (young * math.sqrt(1.0 - poisson + nu2))
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0).edges_fn(top, Dof.Uy, 1.0, lambda t: -delta_y * t)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_dt(lambda t: 1.0).set_dt_out(lambda t: 1.0).set_t_fin(5.0).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def single_element():
+    return von_mises_square(one_qua4(), 1.0, 1)
+
+
+def four_qua8():
+    return von_mises_square(block_2d_four_qua8(), 2.0, 4)
+
+
+def spo_754_footing():
+    """tests/test_spo_754_footing.rs:17-104."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "spo_754_footing.msh"))
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0), any_x)
+    right = feat.search_edges(At.X(500.0), any_x)
+    bottom = feat.search_edges(At.Y(0.0), any_x)
+    footing = feat.search_edges(At.Y(500.0), lambda x: x[0] <= 50.0)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(1e7, 0.48, 0.0, 848.7), 4)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    UY = [0.0, -0.01, -0.015, -0.02, -0.025, -0.035, -0.045, -0.055, -0.065, -0.075, -0.08, -0.09, -0.11, -0.14, -0.2]
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(right, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    ess.edges_fn(footing, Dof.Uy, 1.0, lambda t: UY[int(t)])
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(len(UY)).set_symmetry_check_tolerance(1e-5).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def spo_755_tensile():
+    """tests/test_spo_755_tensile.rs:19-85."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "spo_755_tensile.msh"))
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0), any_x)
+    bottom = feat.search_edges(At.Y(0.0), lambda x: x[0] < 0.50001)
+    top = feat.search_edges(At.Y(15.0), any_x)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(206.9, 0.29, 0.0, 0.45), 4)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    UY = [0.0, 0.005, 0.01, 0.015, 0.02, 0.03, 0.04, 0.05, 0.07, 0.09, 0.11, 0.115, 0.12, 0.13, 0.15, 0.17]
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0).edges_fn(top, Dof.Uy, 1.0, lambda t: UY[int(t)])
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(len(UY)).set_symmetry_check_tolerance(1e-5).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def smith(name):
+    """tests/test_solid_smith_*.rs setups (linear elastic)."""
+    nat = Natural()
+    ess = Essential()
+    if name == "5d2_tri3":
+        mesh = smith_mesh("smith_example_5d2_tri3"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(1e6, 0.3)))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-1.0)), Dof.Uy, 0.0)
+        nat.edges(f.search_edges(At.Y(0.0)), Nbc.Qn, -1.0)
+        cfg = Config(mesh)
+    elif name == "5d7_tri15":
+        mesh = smith_mesh("smith_example_5d7_tri15"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(1e5, 0.2), 12))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(6.0)), Dof.Ux, 0.0)
+        ess.edges(f.search_edges(At.Y(-2.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-2.0)), Dof.Uy, 0.0)
+        nat.points([0, 20], Pbc.Fy, -0.0778).points([5, 15], Pbc.Fy, -0.3556).points([10], Pbc.Fy, -0.1333)
+        cfg = Config(mesh)
+    elif name == "5d11_qua4":
+        mesh = smith_mesh("smith_example_5d11_qua4"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(1e6, 0.3)))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(30.0)), Dof.Ux, 0.0)
+        ess.edges(f.search_edges(At.Y(-10.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-10.0)), Dof.Uy, 0.0)
+        ess.edges(f.search_edges(At.Y(0.0), lambda x: x[0] <= 10.0), Dof.Uy, -1e-5)
+        cfg = Config(mesh); cfg.set_lagrange_mult_method(True)
+    elif name == "5d15_qua8":
+        mesh = smith_mesh("smith_example_5d15_qua8"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(1e6, 0.3), 4))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(6.0)), Dof.Ux, 0.0)
+        ess.edges(f.search_edges(At.Y(-9.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-9.0)), Dof.Uy, 0.0)
+        nat.edges(f.search_edges(At.Y(0.0), lambda x: x[0] <= 3.0), Nbc.Qn, -1.0)
+        cfg = Config(mesh); cfg.set_alt_bb_matrix_method(True).set_symmetry_check_tolerance(1e-9)
+    elif name == "5d17_qua4_axisym":
+        mesh = smith_mesh("smith_example_5d17_qua4"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(100.0, 0.3), 9)),
+                              (2, ParamSolid(1.0, StressStrain.LinearElastic(1000.0, 0.45), 9))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(30.0)), Dof.Ux, 0.0)
+        ess.edges(f.search_edges(At.Y(-10.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-10.0)), Dof.Uy, 0.0)
+        nat.points([0], Pbc.Fy, -2.6667).points([3], Pbc.Fy, -23.3333).points([6], Pbc.Fy, -24.0)
+        cfg = Config(mesh); cfg.set_alt_bb_matrix_method(True).set_axisymmetric()
+    elif name == "5d24_hex20":
+        mesh = smith_mesh("smith_example_5d24_hex20"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(100.0, 0.3), 8)),
+                              (2, ParamSolid(1.0, StressStrain.LinearElastic(50.0, 0.3), 8))])
+        bottom = f.search_faces(At.Z(-2.0))
+        ess.faces(f.search_faces(At.X(0.0)), Dof.Ux, 0.0).faces(f.search_faces(At.Y(0.0)), Dof.Uy, 0.0)
+        ess.faces(bottom, Dof.Ux, 0.0).faces(bottom, Dof.Uy, 0.0).faces(bottom, Dof.Uz, 0.0)
+        nat.faces(f.search_faces(At.Z(0.0), lambda x: x[1] <= 1.0), Nbc.Qn, -1.0)
+        cfg = Config(mesh)
+    elif name == "5d27_qua9":
+        mesh = smith_mesh("smith_example_5d27_qua9"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(1e6, 0.3)))])
+        ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(6.0)), Dof.Ux, 0.0)
+        ess.edges(f.search_edges(At.Y(-9.0)), Dof.Ux, 0.0).edges(f.search_edges(At.Y(-9.0)), Dof.Uy, 0.0)
+        nat.edges(f.search_edges(At.Y(0.0), lambda x: x[0] <= 3.0), Nbc.Qn, -1.0)
+        cfg = Config(mesh)
+    elif name == "5d30_tet4":
+        mesh = smith_mesh("smith_example_5d30_tet4"); f = Features(mesh)
+        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.LinearElastic(100.0, 0.3)))])
+        ess.faces(f.search_faces(At.X(0.0)), Dof.Ux, 0.0).faces(f.search_faces(At.Y(0.0)), Dof.Uy, 0.0)
+        ess.faces(f.search_faces(At.Z(-1.0)), Dof.Uz, 0.0)
+        nat.points([0, 5], Pbc.Fz, -0.1667).points([1, 4], Pbc.Fz, -0.3333)
+        cfg = Config(mesh)
+    else:
+        raise KeyError(name)
+    return mesh, base, cfg, ess, nat
+
+
+SMITH = {  # name -> (fixture key, tolerance of the reference test)
+    "5d2_tri3": ("test_solid_smith_5d2_tri3_plane_strain", 1e-15),
+    "5d7_tri15": ("test_solid_smith_5d7_tri15_plane_strain", 1e-11),
+    "5d11_qua4": ("test_solid_smith_5d11_qua4_plane_strain_uy", 1e-13),
+    "5d15_qua8": ("test_solid_smith_5d15_qua8_plane_strain", 1e-12),
+    "5d17_qua4_axisym": ("test_solid_smith_5d17_qua4_axisym", 1e-9),
+    "5d24_hex20": ("test_solid_smith_5d24_hex20_3d", 2e-9),
+    "5d27_qua9": ("test_solid_smith_5d27_qua9_plane_strain", 3e-13),
+    "5d30_tet4": ("test_solid_smith_5d30_tet4_3d", 1e-15),
+}
+
+
+def footing_c1(nx=16, ny=8, ngauss=9, n_inc=20):
+    """Plane-strain strip footing: 10 m x 5 m, structured nx x ny Qua8, von Mises E=1e4 nu=0.3 z0=200 H=0;
+    bottom fixed, lateral ux=0, uy ramps to -0.05 over the top-left 1 m in n_inc increments."""
+    mesh = generators.qua8_block(nx, ny, 10.0, 5.0)
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0)); right = feat.search_edges(At.X(10.0)); bottom = feat.search_edges(At.Y(0.0))
+    foot = feat.search_edges(At.Y(5.0), lambda x: x[0] <= 1.0)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(1e4, 0.3, 0.0, 200.0), ngauss)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(right, Dof.Ux, 0.0).edges(bottom, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    ess.edges_fn(foot, Dof.Uy, -0.05, lambda t: t / n_inc)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def hex20_cube(nx, ny, nz, n_inc=10, seed=20261203, jitter=0.05, uz=-0.01, ngauss=27):
+    """Unit cube of nx x ny x nz Hex20 (seeded corner jitter), von Mises E=2e5 nu=0.3 z0=250 H=1e3;
+    bottom fixed, top uz ramps to `uz` in n_inc increments."""
+    mesh = generators.hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=seed, jitter=jitter)
+    feat = Features(mesh)
+    bottom = feat.search_faces(At.Z(0.0)); top = feat.search_faces(At.Z(1.0))
+    p1 = ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), ngauss)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.faces(bottom, Dof.Ux, 0.0).faces(bottom, Dof.Uy, 0.0).faces(bottom, Dof.Uz, 0.0)
+    ess.faces_fn(top, Dof.Uz, uz, lambda t: t / n_inc)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def flat(case):
+    mesh, base, cfg, ess, nat = case
+    return describe(mesh, base, cfg, ess, nat)

@@ -1,0 +1,80 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own data and test vectors.
+
+Run in the build container only (it reads /root/reference, which is absent on GPU hosts):
+
+    python tests/golden/make_fixtures.py
+
+Outputs (data only: inputs and expected outputs):
+* spo_*.json.gz   -- HYPLAS per-step displacements, IP stresses and plastic flags/epbar
+                     (data/spo/*_ref.json, loaded by src/util/reference_data_spo.rs:38-151)
+* *.msh           -- meshes shipped with the reference (data/spo/*.msh)
+* sgm_5d17_ref.json -- Smith-Griffiths-Margetts Example 5.17 reference data (data/sgm/)
+* smith_meshes.json -- point coordinates and cell connectivity of the Smith sample meshes
+                     (src/base/sample_meshes.rs:261-735)
+* smith_u.json    -- expected displacement vectors of tests/test_solid_smith_*.rs
+"""
+import gzip
+import json
+import os
+import re
+import shutil
+
+REF = "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def spo(name):
+    with open(os.path.join(REF, "data/spo", name + ".json")) as f:
+        d = json.load(f)
+    steps = []
+    for s in d["all"]:
+        steps.append(dict(load_factor=s["load_factor"], n_iterations=len(s["iterations"]),
+                          displacement=s["displacement"], stresses=s["stresses"],
+                          plast_apex_epbar=s["plast_apex_epbar"]))
+    with gzip.open(os.path.join(OUT, name + ".json.gz"), "wt") as f:
+        json.dump(dict(all=steps), f)
+
+
+def meshes():
+    src = open(os.path.join(REF, "src/base/sample_meshes.rs")).read()
+    out = {}
+    for m in re.finditer(r"pub fn (smith_example_\w+)\(\) -> Mesh \{(.*?)\n    \}\n", src, re.S):
+        name, body = m.group(1), m.group(2)
+        ndim = int(re.search(r"ndim:\s*(\d+)", body).group(1))
+        pts = {}
+        for p in re.finditer(r"Point \{ id:\s*(\d+),\s*marker:\s*-?\d+,\s*coords: vec!\[([^\]]*)\]", body):
+            pts[int(p.group(1))] = [float(v) for v in p.group(2).split(",") if v.strip()]
+        cells = []
+        for c in re.finditer(r"Cell \{ id:\s*(\d+),\s*attribute:\s*(\d+),\s*kind: GeoKind::(\w+),\s*points: vec!\[([^\]]*)\]", body):
+            cells.append([int(c.group(2)), c.group(3).lower(), [int(v) for v in c.group(4).split(",") if v.strip()]])
+        coords = [pts[i] for i in range(len(pts))]
+        out[name] = dict(ndim=ndim, coords=coords, cells=cells)
+    with open(os.path.join(OUT, "smith_meshes.json"), "w") as f:
+        json.dump(out, f, indent=0)
+
+
+def smith_vectors():
+    out = {}
+    for fn in sorted(os.listdir(os.path.join(REF, "tests"))):
+        if not fn.startswith("test_solid_smith_"):
+            continue
+        src = open(os.path.join(REF, "tests", fn)).read()
+        m = re.search(r"let uu_correct = &\[(.*?)\];", src, re.S)
+        if not m:
+            continue
+        vals = [float(v) for v in re.findall(r"[-+]?\d+\.\d+e[-+]\d+", m.group(1))]
+        out[fn[:-3]] = vals
+    with open(os.path.join(OUT, "smith_u.json"), "w") as f:
+        json.dump(out, f)
+
+
+if __name__ == "__main__":
+    for n in ["test_von_mises_single_element_2d_ref", "test_von_mises_2x2_elements_2d_ref",
+              "spo_754_footing_ref", "spo_755_tensile_perf_plast_ref"]:
+        spo(n)
+    for n in ["spo_754_footing.msh", "spo_755_tensile.msh"]:
+        shutil.copy(os.path.join(REF, "data/spo", n), os.path.join(OUT, n))
+    shutil.copy(os.path.join(REF, "data/sgm/sgm_5d17_ref.json"), os.path.join(OUT, "sgm_5d17_ref.json"))
+    meshes()
+    smith_vectors()
+    print("fixtures written to", OUT)
