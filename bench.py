@@ -1,0 +1,121 @@
+"""Benchmark: Hex20 IP-updates/s of the implicit element pass (K + R + return map) on one MI355X.
+
+Workload: unit cube, structured 64x64x32 Hex20 (131,072 elements, 27 IPs each = 3,538,944 IPs,
+549,185 points, 1,647,555 DOFs), corner jitter ±5% of h (seed 20261203), von Mises E=2e5 nu=0.3
+z0=250 H=1e3.  A step is one element pass over device-resident data: gather of coordinates and
+displacement increments, strain increment, radial return from the backup state (with per-IP
+sigma, z, lambda, loading flag, epbar), B^T sigma into the element forces and their deterministic
+gather into R, B^T D B node blocks and their deterministic gather into the node-block CSR tangent.
+The increment is a seeded field large enough that part of the IPs return plastically.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 64 --ny 64 --nz 32] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
+
+
+def build_problem(nx, ny, nz, precision="f64"):
+    from pmsim_amd import Config, Essential, FemBase, Natural, ParamSolid, StressStrain, describe
+    from pmsim_amd import generators as G
+    mesh = G.hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=20261203, jitter=0.05)
+    base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
+    cfg = Config(mesh).set_tangent_precision(precision)
+    return mesh, base, cfg, describe(mesh, base, cfg, Essential(), Natural())
+
+
+def increment(mesh, seed=7):
+    """Compression of 0.3% in z plus shear and seeded noise: drives part of the IPs plastic."""
+    rng = np.random.default_rng(seed)
+    X = mesh.coords
+    u = np.zeros_like(X)
+    u[:, 2] = -3e-3 * X[:, 2]
+    u[:, 0] = 1.5e-3 * X[:, 2] * X[:, 1]
+    u += rng.uniform(-2e-6, 2e-6, X.shape)
+    return u.ravel()
+
+
+def cpu_baseline(seconds=12.0):
+    """Oracle element pass (update + f_int + K per element) on one host core over a bounded sample."""
+    from oracle import oracle as orc
+    mesh, base, cfg, d = build_problem(8, 8, 4)
+    o = orc.Oracle(d)
+    o.init_state()
+    ddu = increment(mesh)
+    t1 = o.time_element_pass(ddu, reps=1, nthreads=1)
+    reps = max(1, int(seconds / max(t1, 1e-6)))
+    t = o.time_element_pass(ddu, reps=reps, nthreads=1)
+    nip = mesh.ncell * 27
+    return dict(value=nip * reps / t, unit="IP-updates/s", cores=1, kind="port",
+                sample=f"8x8x4 Hex20 (256 elements, {nip} IPs) x {reps} passes, serial C++ oracle (update+f_int+K)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=32)
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the element pass shards by mesh "
+              f"partition; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+    from pmsim_amd.solver import Fem
+    t0 = time.time()
+    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision)
+    f = Fem(d, cfg, keep_history=False)
+    setup_s = time.time() - t0
+    print(f"setup: {mesh.ncell} cells, {f.nip} IPs, {f.ndof} DOFs, {f.nnzb} blocks, {setup_s:.1f} s", file=sys.stderr, flush=True)
+    f.set_ddu(increment(mesh))
+    for _ in range(a.warmup):
+        f.element_pass()
+    ms = f.time_element_pass(a.steps)
+    st = f.get_ip_state()
+    plastic = float(np.mean(st[:, f.ns + 2] == 0.0))
+    nip = f.nip
+    value = nip / (ms["total"] * 1e-3)
+    tangent_flops = FLOP_PER_IP * nip
+    achieved_tf = tangent_flops / (ms["tangent"] * 1e-3) / 1e12
+    out = {
+        "metric": "Hex20 IP-updates/s (K+R+return map)",
+        "value": value,
+        "unit": "IP-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms["total"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "f64" else "f32-tangent/f64",
+        "data": "synthetic (structured jittered Hex20 cube, seeded increment)",
+        "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, one element pass",
+                   "elements": int(mesh.ncell), "ips": int(nip), "dofs": int(f.ndof), "plastic_fraction": plastic},
+        "kernels_ms": {k: ms[k] for k in ("update", "residual", "tangent", "scatter")},
+        "roofline": {"bound": "mfma", "kernel": "k_tangent", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None},
+    }
+    if not a.no_cpu:
+        cb = cpu_baseline()
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

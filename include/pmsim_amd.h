@@ -1,0 +1,137 @@
+/*
+ * pmsim_amd.h -- C ABI of the MI355X-native implicit solid element loop.
+ *
+ * One opaque handle (pm_fem) holds what pmsim spreads over FemBase + FemState +
+ * SolverImplicit for the solid / von Mises path: the mesh grouped into per-kind element
+ * batches, the device-resident integration-point state, the CSR (node-block) tangent and
+ * the Newton driver.  Every function returns 0 on success; on failure it returns a
+ * non-zero status and pm_last_error() gives the message, using pmsim's own error texts
+ * where pmsim has one (src/lib.rs:2 StrError).
+ *
+ * Threading: a handle is not thread-safe; it owns one HIP stream (default priority) and
+ * every call is ordered on it.  Different handles may be used from different threads.
+ *
+ * Kind codes: Tri3=0 Tri6=1 Tri10=2 Tri15=3 Qua4=4 Qua8=5 Qua9=6 Tet4=7 Tet10=8 Hex8=9
+ * Hex20=10; boundary facets add Lin2=100 Lin3=101 Lin4=102 Lin5=103 (2D edges) and use
+ * Tri3/Tri6/Qua4/Qua8 codes for 3D faces.  Local node orders follow gemlab's GeoKind.
+ * DOF numbering is pmsim's closed form for solids: eq = ndim*point + dof
+ * (src/base/all_dofs.rs:87-117, element_dofs.rs:123-131).
+ */
+#ifndef PMSIM_AMD_H
+#define PMSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_fem pm_fem;
+
+/* Solver knobs.  Field meaning and defaults follow pmsim's Config (src/base/config.rs:184-231);
+ * fields after `symmetry_check_tolerance` select the MI355X linear solver and tangent precision. */
+typedef struct pm_config {
+  int32_t lagrange_mult_method;     /* config.rs:426  (default 0) */
+  int32_t linear_problem;           /* config.rs:371  (default 0) */
+  int32_t constant_tangent;         /* config.rs:389  (default 0) */
+  int32_t n_max_time_steps;         /* default 1000 */
+  int32_t n_max_iterations;         /* default 10 */
+  int32_t allowed_step_n_failure;   /* default 100 */
+  double t_ini, t_fin, dt, dt_out;  /* defaults 0, 1, 1, 1 (constant Δt) */
+  double tol_rr_abs;                /* default 1e-10 */
+  double tol_mdu_rel;               /* default 1e-8 */
+  double symmetry_check_tolerance;  /* < 0 disables (default 1e-10) */
+  int32_t lin_sol;                  /* 0: device FP64 block-Jacobi PCG, 1: host direct (Cholesky) */
+  int32_t tangent_precision;        /* 0: FP64, 1: FP32, 2: BF16 inputs with FP32 accumulation */
+  double pcg_rtol;                  /* PCG stop: ||r|| <= pcg_rtol * ||b||  (default 1e-12) */
+  int32_t pcg_max_iter;             /* default 20000 */
+  int32_t keep_history;             /* store u and IP state of every output step on the host */
+  double tol_rr_scaled;             /* > 0 adds ||R||inf <= tol_rr_scaled * ||F_int||inf as a convergence test */
+} pm_config;
+
+/* Statistics of the last pm_solve / element pass. */
+typedef struct pm_stats {
+  int32_t n_steps;                  /* converged time steps */
+  int32_t n_iterations;             /* Newton iterations (all steps) */
+  int64_t n_pcg_iterations;         /* PCG iterations (all solves) */
+  int32_t n_host_fallbacks;         /* linear solves that fell back to the host direct solver */
+  double ms_update, ms_residual, ms_tangent, ms_scatter, ms_linear, ms_total;
+  int64_t n_ip;                     /* integration points */
+  int64_t n_block_nnz;              /* node-block nonzeros of K */
+  int64_t device_bytes;             /* device memory held by the handle */
+} pm_stats;
+
+/* ---- errors ---------------------------------------------------------------------------- */
+const char* pm_last_error(const pm_fem* fem);      /* message of the last failure on `fem` */
+const char* pm_global_last_error(void);            /* message of the last pm_fem_create failure */
+void pm_default_config(pm_config* cfg);            /* fills pmsim's defaults (config.rs:184-231) */
+
+/* ---- problem definition (replaces gemlab Mesh + FemBase::new, src/fem/fem_base.rs:25-30) -- */
+int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell,
+                  const int32_t* kinds, const int32_t* attributes, const int64_t* cell_ptr,
+                  const int32_t* cell_points);
+void pm_fem_destroy(pm_fem* fem);
+
+/* ParamSolid + StressStrain (src/base/parameters.rs:6-68,276-285); model 0 LinearElastic, 1 VonMises;
+ * ngauss <= 0 selects the kind's default rule. */
+int pm_set_param_solid(pm_fem* fem, int32_t attribute, int32_t model, double young, double poisson,
+                       double hh, double z_ini, int32_t ngauss);
+/* Idealization (src/base/idealization.rs): 2D plane strain unless axisymmetric or plane stress. */
+int pm_set_idealization(pm_fem* fem, int32_t axisymmetric, int32_t plane_stress, double thickness);
+int pm_set_config(pm_fem* fem, const pm_config* cfg);
+/* Essential (src/base/essential.rs:38-135): n prescribed (point, dof) with values at the time
+ * stations t_k = t_ini + k*dt, table is n x nstation row-major (value = c * m(t_k)). */
+int pm_set_essential(pm_fem* fem, int64_t n, const int32_t* points, const int32_t* dofs,
+                     const double* table, int32_t nstation);
+/* Natural point loads (src/fem/bc_concentrated.rs:46-55). */
+int pm_set_point_loads(pm_fem* fem, int64_t n, const int32_t* points, const int32_t* dofs, const double* values);
+/* Natural normal distributed load Qn on one boundary facet (src/fem/bc_distributed.rs:166-174). */
+int pm_add_facet_qn(pm_fem* fem, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell,
+                    double value);
+
+/* Builds element batches, the node-block CSR pattern, the deterministic scatter maps and all
+ * device buffers (replaces SolverImplicit::new / SolverData::new / LinearSystem::new,
+ * src/fem/solver_implicit.rs:27-51, solver_data.rs:37-82, linear_system.rs:89-181). */
+int pm_initialize(pm_fem* fem, int32_t device);
+
+/* ---- batched ElementTrait over device-resident state (src/fem/element_trait.rs:1-38) ------ */
+int pm_initialize_internal_values(pm_fem* fem);  /* elements.rs -> von_mises.rs:107-116 */
+int pm_reset_algorithmic_variables(pm_fem* fem); /* element_solid.rs:241-246 */
+int pm_backup_secondary_values(pm_fem* fem);     /* element_solid.rs:227-231 */
+int pm_restore_secondary_values(pm_fem* fem);    /* element_solid.rs:234-238 */
+/* element_solid.rs:190-224: per IP strain increment from the device ddu, then the stress update
+ * starting from the backup state (the driver's backup/restore protocol, solver_implicit.rs:254-263). */
+int pm_update_secondary_values(pm_fem* fem);
+int pm_assemble_f_int(pm_fem* fem);              /* element_solid.rs:124-138 + assembly.rs:38-46 */
+int pm_assemble_kk(pm_fem* fem);                 /* element_solid.rs:168-185 + assembly.rs:70-139 */
+/* One element pass = update + f_int + K (the benchmarked unit). */
+int pm_element_pass(pm_fem* fem);
+
+/* ---- Newton driver (SolverImplicit::solve, src/fem/solver_implicit.rs:56-272) ---------- */
+int pm_solve(pm_fem* fem, int32_t* n_converged);
+int pm_get_stats(const pm_fem* fem, pm_stats* stats);
+
+/* ---- data movement -------------------------------------------------------------------- */
+int pm_sizes(const pm_fem* fem, int64_t* out /* [ndof, n_prescribed, nip, nsig, n_block_nnz, ndim] */);
+int pm_set_ddu(pm_fem* fem, const double* ddu /* ndof */);
+int pm_get_u(const pm_fem* fem, double* u /* ndof */, double* lambda /* n_prescribed or NULL */);
+int pm_get_f_int(const pm_fem* fem, double* f /* ndof */);
+/* IP state in cell order: per IP [sigma (nsig Mandel), z, lambda, elastic(0/1), epbar]. */
+int pm_get_ip_state(const pm_fem* fem, double* v);
+int pm_set_ip_state(pm_fem* fem, const double* v);
+/* Global K (ndof x ndof) in scalar CSR: rowptr (ndof+1), cols (nnz), vals (nnz); vals in FP64. */
+int pm_get_csr(const pm_fem* fem, int64_t* rowptr, int32_t* cols, double* vals);
+/* History of output steps kept with keep_history: count, then u (n x ndof) and IP states. */
+int pm_history_count(const pm_fem* fem, int32_t* n);
+int pm_get_history(const pm_fem* fem, int32_t index, double* u, double* lambda, double* ip_state, double* t);
+/* Times `reps` element passes with HIP events on the handle's stream; per kernel class in ms. */
+int pm_time_element_pass(pm_fem* fem, int32_t reps, double* ms /* [update, residual, tangent, scatter, total] */);
+/* Linear solve of the current K with a given right-hand side on the free DOFs (testing and
+ * benchmarking the PCG): x = K_ff^-1 b_f, prescribed entries of x are zero. */
+int pm_linear_solve(pm_fem* fem, const double* b, double* x, int32_t* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMSIM_AMD_H */

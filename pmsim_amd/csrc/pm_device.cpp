@@ -1,0 +1,118 @@
+// Device buffers of a pm_fem handle: allocation, upload of the element batches, patterns and
+// gather maps, and release.
+#include <algorithm>
+#include <cstring>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+template <class T>
+static int dalloc(pm_fem* f, T** p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    f->err = std::string("device allocation failed (") + hipGetErrorString(e) + ")";
+    return 1;
+  }
+  f->device_bytes += (int64_t)sizeof(T) * n;
+  return 0;
+}
+
+template <class T>
+static int upload(pm_fem* f, T* d, const T* h, int64_t n) {
+  if (n <= 0) return 0;
+  PM_HIP(hipMemcpyAsync(d, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, f->stream));
+  return 0;
+}
+
+int device_alloc(pm_fem* f) {
+  PM_HIP(hipSetDevice(f->device));
+  PM_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+  const int nd = f->ndim, ns = f->nsig;
+  const int64_t b2 = (int64_t)nd * nd;
+  if (dalloc(f, &f->d_X, f->npoint * nd) || upload(f, f->d_X, f->X.data(), f->npoint * nd)) return 1;
+  // element batches
+  for (auto& g : f->groups) {
+    std::vector<int32_t> conn((size_t)g.nel * g.nnode);
+    for (int64_t el = 0; el < g.nel; el++) {
+      int64_t e = g.cells[el];
+      std::copy(f->cpts.begin() + f->cptr[e], f->cpts.begin() + f->cptr[e + 1], conn.begin() + el * g.nnode);
+    }
+    Rule R;
+    gauss_rule(g.kind, g.nip, R);
+    std::vector<double> tab((size_t)g.nip * (1 + g.nnode + g.nnode * nd));
+    double* w = tab.data();
+    double* N = w + g.nip;
+    double* dN = N + (size_t)g.nip * g.nnode;
+    for (int q = 0; q < g.nip; q++) {
+      w[q] = R.w[q];
+      shape(g.kind, &R.x[3 * q], N + (size_t)q * g.nnode, dN + (size_t)q * g.nnode * nd);
+    }
+    const int64_t nipg = g.nel * g.nip;
+    if (dalloc(f, &g.d_conn, (int64_t)conn.size()) || upload(f, g.d_conn, conn.data(), (int64_t)conn.size())) return 1;
+    if (dalloc(f, &g.d_tab, (int64_t)tab.size()) || upload(f, g.d_tab, tab.data(), (int64_t)tab.size())) return 1;
+    if (dalloc(f, &g.d_state, nipg * (ns + 4)) || dalloc(f, &g.d_backup, nipg * (ns + 4))) return 1;
+    PM_HIP(hipStreamSynchronize(f->stream));  // host vectors go out of scope
+  }
+  // patterns and maps
+  if (dalloc(f, &f->d_browptr, f->npoint + 1) || upload(f, f->d_browptr, f->h_browptr.data(), f->npoint + 1)) return 1;
+  if (dalloc(f, &f->d_bcols, f->nnzb) || upload(f, f->d_bcols, f->h_bcols.data(), f->nnzb)) return 1;
+  if (dalloc(f, &f->d_kmap_ptr, f->nnzb + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzb + 1)) return 1;
+  if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
+  if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
+  if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
+  if (dalloc(f, &f->d_kval, f->nnzb * b2)) return 1;
+  if (f->cfg.tangent_precision == 0) {
+    if (dalloc(f, &f->d_ke, f->n_ke_blocks * b2)) return 1;
+  } else {
+    if (dalloc(f, &f->d_ke32, f->n_ke_blocks * b2) || dalloc(f, &f->d_kval32, f->nnzb * b2)) return 1;
+  }
+  if (dalloc(f, &f->d_fe, f->n_fe)) return 1;
+  // vectors
+  const int64_t n = f->ndof, npr = std::max<int64_t>(f->nprescribed, 1);
+  if (dalloc(f, &f->d_u, n) || dalloc(f, &f->d_ddu, n) || dalloc(f, &f->d_fint, n) || dalloc(f, &f->d_fext, n) ||
+      dalloc(f, &f->d_rr, n) || dalloc(f, &f->d_mdu, n) || dalloc(f, &f->d_mdu0, n + npr) || dalloc(f, &f->d_lam, npr) ||
+      dalloc(f, &f->d_y, npr) || dalloc(f, &f->d_cval, npr) || dalloc(f, &f->d_peq, npr) || dalloc(f, &f->d_free, n) ||
+      dalloc(f, &f->d_pcg, 8 * n + 2 * npr) || dalloc(f, &f->d_binv, f->npoint * b2))
+    return 1;
+  const int64_t gn = (f->npoint + 255) / 256;
+  if (dalloc(f, &f->d_red, std::max<int64_t>(2 * gn, 1024) + 64) || dalloc(f, &f->d_scal, 16) || dalloc(f, &f->d_err, 4)) return 1;
+  std::vector<double> freev(n, 1.0);
+  for (auto eq : f->prescribed_eq) freev[eq] = 0.0;
+  if (upload(f, f->d_free, freev.data(), n)) return 1;
+  if (upload(f, f->d_fext, f->f_ext.data(), n)) return 1;
+  if (f->nprescribed && upload(f, f->d_peq, f->prescribed_eq.data(), f->nprescribed)) return 1;
+  PM_HIP(hipMemsetAsync(f->d_u, 0, sizeof(double) * n, f->stream));
+  PM_HIP(hipMemsetAsync(f->d_ddu, 0, sizeof(double) * n, f->stream));
+  PM_HIP(hipMemsetAsync(f->d_lam, 0, sizeof(double) * npr, f->stream));
+  PM_HIP(hipMemsetAsync(f->d_err, 0, sizeof(int32_t) * 4, f->stream));
+  PM_HIP(hipMemsetAsync(f->d_scal, 0, sizeof(double) * 16, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  for (auto& g : f->groups) { launch_init_state(f, g); launch_copy(f, g.d_backup, g.d_state, g.nel * g.nip * (ns + 4)); }
+  PM_HIP(hipStreamSynchronize(f->stream));
+  // host copies of the maps are no longer needed
+  std::vector<int32_t>().swap(f->h_kmap);
+  std::vector<int32_t>().swap(f->h_kmap_ptr);
+  std::vector<int32_t>().swap(f->h_fmap);
+  std::vector<int32_t>().swap(f->h_fmap_ptr);
+  return 0;
+}
+
+void device_free(pm_fem* f) {
+  auto fr = [](void* p) { if (p) (void)hipFree(p); };
+  for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
+  fr(f->d_X); fr(f->d_browptr); fr(f->d_bcols); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
+  fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
+  fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
+  fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+  f->stream = nullptr;
+  f->d_X = nullptr; f->d_browptr = f->d_bcols = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;
+  f->d_kmap_ptr = f->d_kmap = nullptr; f->d_fe = nullptr; f->d_fmap_ptr = f->d_fmap = nullptr;
+  f->d_u = f->d_ddu = f->d_fint = f->d_fext = f->d_rr = f->d_mdu = f->d_mdu0 = f->d_lam = nullptr;
+  f->d_y = f->d_cval = nullptr; f->d_peq = nullptr; f->d_free = f->d_pcg = f->d_binv = f->d_red = f->d_scal = nullptr; f->d_err = nullptr;
+}
+
+}  // namespace pm

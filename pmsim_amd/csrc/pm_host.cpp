@@ -1,0 +1,340 @@
+// Host-side setup: element batches, node-block CSR pattern, deterministic scatter maps,
+// external forces, and the host direct solver used as fallback and judge.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+// Element batches: one per (kind, attribute); cells keep ascending global order.
+int build_groups(pm_fem* f) {
+  f->nsig = f->ndim == 2 ? 4 : 6;
+  f->ndof = (int64_t)f->ndim * f->npoint;
+  std::map<std::pair<int, int>, int> key2g;
+  f->cell_group.assign(f->ncell, -1);
+  f->cell_local.assign(f->ncell, 0);
+  for (int64_t e = 0; e < f->ncell; e++) {
+    int k = f->kind[e];
+    if (k < 0 || k >= NKIND) { f->err = "invalid cell kind"; return 1; }
+    if (kind_ndim(k) != f->ndim) { f->err = "cell kind does not match ndim"; return 1; }
+    if (f->cptr[e + 1] - f->cptr[e] != kind_nnode(k)) { f->err = "wrong number of points for cell kind"; return 1; }
+    auto key = std::make_pair(k, (int)f->attr[e]);
+    auto it = key2g.find(key);
+    if (it == key2g.end()) {
+      auto pit = f->params.find(f->attr[e]);
+      if (pit == f->params.end()) { f->err = "cannot find attribute in the element map"; return 1; }
+      Group g;
+      g.kind = k; g.ndim = f->ndim; g.nnode = kind_nnode(k); g.attribute = f->attr[e]; g.param = pit->second;
+      int ng = g.param.ngauss > 0 ? g.param.ngauss : kind_default_ngauss(k);
+      Rule r;
+      if (!gauss_rule(k, ng, r)) { f->err = "requested number of integration points is not available"; return 1; }
+      g.nip = r.n();
+      g.nblk = g.nnode * (g.nnode + 1) / 2;
+      const ParamSolid& p = g.param;
+      if (p.model == 1) {
+        if (f->plane_stress) { f->err = "von Mises model does not work in plane-stress"; return 1; }
+        if (p.z_ini <= 1e-6) { f->err = "von Mises initial size of the yield surface must > 1e-6"; return 1; }
+      }
+      g.K = p.young / (3.0 * (1.0 - 2.0 * p.poisson));
+      g.G = p.young / (2.0 * (1.0 + p.poisson));
+      g.plane_stress = f->plane_stress;
+      it = key2g.emplace(key, (int)f->groups.size()).first;
+      f->groups.push_back(g);
+    }
+    Group& g = f->groups[it->second];
+    f->cell_group[e] = it->second;
+    f->cell_local[e] = (int64_t)g.cells.size();
+    g.cells.push_back(e);
+  }
+  // IP numbering in cell order
+  f->cell_ip.assign(f->ncell + 1, 0);
+  for (int64_t e = 0; e < f->ncell; e++) f->cell_ip[e + 1] = f->cell_ip[e] + f->groups[f->cell_group[e]].nip;
+  f->nip = f->cell_ip[f->ncell];
+  int64_t fe = 0, ke = 0;
+  for (auto& g : f->groups) {
+    g.nel = (int64_t)g.cells.size();
+    g.fe_base = fe; fe += g.nel * g.nnode * g.ndim;
+    g.ke_base = ke; ke += g.nel * g.nblk;
+  }
+  f->n_fe = fe;
+  f->n_ke_blocks = ke;
+  // every point must belong to a cell (pure solid DOF map)
+  std::vector<uint8_t> used(f->npoint, 0);
+  for (auto p : f->cpts) {
+    if (p < 0 || p >= f->npoint) { f->err = "cell point id out of range"; return 1; }
+    used[p] = 1;
+  }
+  for (auto u : used) if (!u) { f->err = "all points must belong to at least one cell"; return 1; }
+  // prescribed equations
+  f->nprescribed = (int64_t)f->ess_point.size();
+  f->prescribed_eq.resize(f->nprescribed);
+  f->is_prescribed.assign(f->ndof, 0);
+  for (int64_t i = 0; i < f->nprescribed; i++) {
+    if (f->ess_point[i] < 0 || f->ess_point[i] >= f->npoint || f->ess_dof[i] < 0 || f->ess_dof[i] >= f->ndim) {
+      f->err = "cannot find (PointId, Dof) in the DOF map"; return 1;
+    }
+    int64_t eq = (int64_t)f->ndim * f->ess_point[i] + f->ess_dof[i];
+    f->prescribed_eq[i] = eq;
+    f->is_prescribed[eq] = 1;
+  }
+  return 0;
+}
+
+// Node-block CSR pattern of K and the two deterministic gather maps:
+//  * K: for each block (a,b), the element blocks that contribute, in ascending global cell order
+//    (pmsim's COO insertion order, src/base/assembly.rs:124-135; duplicates summed in that order);
+//  * R: for each node, the element force vectors that contribute, in ascending global cell order.
+int build_patterns(pm_fem* f) {
+  const int64_t np = f->npoint;
+  f->bs = f->ndim;
+  // node -> cells (ascending)
+  std::vector<int64_t> nptr(np + 1, 0);
+  for (auto p : f->cpts) nptr[p + 1]++;
+  for (int64_t i = 0; i < np; i++) nptr[i + 1] += nptr[i];
+  std::vector<int64_t> ncells(nptr[np]);
+  std::vector<int32_t> nloc(nptr[np]);
+  {
+    std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t e = 0; e < f->ncell; e++)
+      for (int64_t k = f->cptr[e]; k < f->cptr[e + 1]; k++) {
+        int32_t p = f->cpts[k];
+        ncells[fill[p]] = e;
+        nloc[fill[p]] = (int32_t)(k - f->cptr[e]);
+        fill[p]++;
+      }
+  }
+  // block rows
+  f->h_browptr.assign(np + 1, 0);
+  std::vector<std::vector<int32_t>> rows(np);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t a = 0; a < np; a++) {
+    std::vector<int32_t>& v = rows[a];
+    for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) {
+      int64_t e = ncells[k];
+      for (int64_t q = f->cptr[e]; q < f->cptr[e + 1]; q++) v.push_back(f->cpts[q]);
+    }
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  int64_t nnzb = 0;
+  for (int64_t a = 0; a < np; a++) { nnzb += (int64_t)rows[a].size(); }
+  if (nnzb >= (int64_t)INT32_MAX) { f->err = "too many nonzero blocks for int32 indices"; return 1; }
+  f->nnzb = nnzb;
+  f->h_bcols.resize(nnzb);
+  for (int64_t a = 0; a < np; a++) {
+    f->h_browptr[a + 1] = f->h_browptr[a] + (int32_t)rows[a].size();
+    std::copy(rows[a].begin(), rows[a].end(), f->h_bcols.begin() + f->h_browptr[a]);
+    std::vector<int32_t>().swap(rows[a]);
+  }
+  // K gather map: count contributions per block
+  std::vector<int32_t> kptr(nnzb + 1, 0);
+  auto block_index = [&](int32_t a, int32_t b) -> int64_t {
+    const int32_t* lo = f->h_bcols.data() + f->h_browptr[a];
+    const int32_t* hi = f->h_bcols.data() + f->h_browptr[a + 1];
+    return (int64_t)(std::lower_bound(lo, hi, b) - f->h_bcols.data());
+  };
+  // per element, block index of every (la, lb) local pair
+  int64_t total = 0;
+  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; total += n * n; }
+  if (2 * f->n_ke_blocks >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
+  std::vector<int32_t> pairblk(total);
+  std::vector<int64_t> pbase(f->ncell + 1, 0);
+  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; pbase[e + 1] = pbase[e] + n * n; }
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < f->ncell; e++) {
+    int64_t n = f->cptr[e + 1] - f->cptr[e];
+    const int32_t* pts = &f->cpts[f->cptr[e]];
+    for (int64_t la = 0; la < n; la++)
+      for (int64_t lb = 0; lb < n; lb++) pairblk[pbase[e] + la * n + lb] = (int32_t)block_index(pts[la], pts[lb]);
+  }
+  for (int64_t i = 0; i < total; i++) kptr[pairblk[i] + 1]++;
+  for (int64_t i = 0; i < nnzb; i++) kptr[i + 1] += kptr[i];
+  std::vector<int32_t> kmap(total);
+  {
+    std::vector<int32_t> fill(kptr.begin(), kptr.end() - 1);
+    for (int64_t e = 0; e < f->ncell; e++) {  // ascending cells => ascending contributions
+      const Group& g = f->groups[f->cell_group[e]];
+      int64_t n = g.nnode, el = f->cell_local[e];
+      for (int64_t la = 0; la < n; la++)
+        for (int64_t lb = 0; lb < n; lb++) {
+          int64_t lo = std::min(la, lb), hi = std::max(la, lb);
+          // upper-triangular local block index (row-major over lo <= hi)
+          int64_t ub = lo * n - lo * (lo - 1) / 2 + (hi - lo);
+          int64_t blk = g.ke_base + el * g.nblk + ub;
+          int32_t ent = (int32_t)((blk << 1) | (la > lb ? 1 : 0));
+          int32_t pb = pairblk[pbase[e] + la * n + lb];
+          kmap[fill[pb]++] = ent;
+        }
+    }
+  }
+  // upload happens in device_alloc; keep host copies in temporaries stored on the handle
+  f->h_kmap_ptr.swap(kptr);
+  f->h_kmap.swap(kmap);
+  // R gather map
+  std::vector<int32_t> fptr(np + 1, 0), fmap(nptr[np]);
+  for (int64_t a = 0; a < np; a++) {
+    fptr[a + 1] = (int32_t)nptr[a + 1];
+    for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) {
+      int64_t e = ncells[k];
+      const Group& g = f->groups[f->cell_group[e]];
+      int64_t off = g.fe_base / g.ndim + f->cell_local[e] * g.nnode + nloc[k];  // in ndim-vectors
+      fmap[k] = (int32_t)off;
+    }
+  }
+  f->h_fmap_ptr.swap(fptr);
+  f->h_fmap.swap(fmap);
+  return 0;
+}
+
+// External forces: concentrated loads (bc_concentrated.rs:46-55) and normal distributed loads
+// (bc_distributed.rs:166-174: f = int N qn n dGamma with n the outward normal).
+void assemble_f_ext(pm_fem* f) {
+  f->f_ext.assign(f->ndof, 0.0);
+  for (size_t i = 0; i < f->pl_point.size(); i++) f->f_ext[(int64_t)f->ndim * f->pl_point[i] + f->pl_dof[i]] += f->pl_value[i];
+  const int nd = f->ndim;
+  for (const Facet& F : f->facets) {
+    int nn = (int)F.pts.size();
+    Rule R;
+    bool tri = (F.kind == TRI3 || F.kind == TRI6);
+    if (F.kind >= LIN2) gauss_rule(F.kind, std::max(3, std::min(4, nn)), R);
+    else if (tri) gauss_rule(F.kind, 6, R);
+    else gauss_rule(QUA9, 9, R);
+    // owner cell centroid and facet centroid (outward orientation)
+    double cc[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+    int64_t cn = f->cptr[F.cell + 1] - f->cptr[F.cell];
+    for (int64_t k = 0; k < cn; k++) for (int i = 0; i < nd; i++) cc[i] += f->X[(int64_t)f->cpts[f->cptr[F.cell] + k] * nd + i] / cn;
+    for (int k = 0; k < nn; k++) for (int i = 0; i < nd; i++) fc[i] += f->X[(int64_t)F.pts[k] * nd + i] / nn;
+    for (int q = 0; q < R.n(); q++) {
+      double N[20], dN[40];
+      shape(F.kind, &R.x[3 * q], N, dN);
+      int fd = kind_ndim(F.kind);
+      double n[3] = {0, 0, 0};
+      if (nd == 2) {
+        double dx = 0, dy = 0;
+        for (int m = 0; m < nn; m++) { dx += f->X[(int64_t)F.pts[m] * 2] * dN[m]; dy += f->X[(int64_t)F.pts[m] * 2 + 1] * dN[m]; }
+        n[0] = dy; n[1] = -dx;
+      } else {
+        double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        for (int m = 0; m < nn; m++) for (int i = 0; i < 3; i++) {
+          a[i] += f->X[(int64_t)F.pts[m] * 3 + i] * dN[m * fd];
+          b[i] += f->X[(int64_t)F.pts[m] * 3 + i] * dN[m * fd + 1];
+        }
+        n[0] = a[1] * b[2] - a[2] * b[1]; n[1] = a[2] * b[0] - a[0] * b[2]; n[2] = a[0] * b[1] - a[1] * b[0];
+      }
+      double dot = 0;
+      for (int i = 0; i < nd; i++) dot += n[i] * (fc[i] - cc[i]);
+      double w = R.w[q] * (dot < 0 ? -1.0 : 1.0) * (nd == 2 ? f->thickness : 1.0);
+      if (nd == 2 && f->axisym) {
+        double r = 0;
+        for (int m = 0; m < nn; m++) r += N[m] * f->X[(int64_t)F.pts[m] * 2];
+        w *= r;
+      }
+      for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) f->f_ext[(int64_t)F.pts[m] * nd + i] += N[m] * F.value * n[i] * w;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+// host direct solver: envelope (skyline) Cholesky of K_ff after a reverse Cuthill-McKee ordering
+// ----------------------------------------------------------------------------------------------
+int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x) {
+  const int nd = f->ndim;
+  const int64_t ndof = f->ndof;
+  // K values to host (FP64 copy of whatever precision the tangent used)
+  std::vector<double> kv((size_t)f->nnzb * nd * nd);
+  if (hipMemcpyAsync(kv.data(), f->d_kval, kv.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
+      hipStreamSynchronize(f->stream) != hipSuccess) {
+    f->err = "HIP error while copying K to the host"; return 1;
+  }
+  // free DOF numbering in RCM node order
+  const int64_t np = f->npoint;
+  std::vector<int32_t> order;
+  order.reserve(np);
+  {
+    std::vector<char> seen(np, 0);
+    auto deg = [&](int64_t a) { return f->h_browptr[a + 1] - f->h_browptr[a]; };
+    while ((int64_t)order.size() < np) {
+      int64_t start = -1;
+      for (int64_t i = 0; i < np; i++) if (!seen[i] && (start < 0 || deg(i) < deg(start))) start = i;
+      size_t head = order.size();
+      order.push_back((int32_t)start); seen[start] = 1;
+      while (head < order.size()) {
+        int32_t v = order[head++];
+        std::vector<int32_t> nb;
+        for (int32_t k = f->h_browptr[v]; k < f->h_browptr[v + 1]; k++) {
+          int32_t w = f->h_bcols[k];
+          if (!seen[w]) { seen[w] = 1; nb.push_back(w); }
+        }
+        std::sort(nb.begin(), nb.end(), [&](int32_t p, int32_t q) { return deg(p) < deg(q) || (deg(p) == deg(q) && p < q); });
+        for (auto w : nb) order.push_back(w);
+      }
+    }
+    std::reverse(order.begin(), order.end());
+  }
+  std::vector<int64_t> pos(ndof, -1);
+  std::vector<int64_t> eqs;
+  for (auto a : order) for (int d = 0; d < nd; d++) {
+    int64_t eq = (int64_t)nd * a + d;
+    if (!f->is_prescribed[eq]) { pos[eq] = (int64_t)eqs.size(); eqs.push_back(eq); }
+  }
+  const int64_t n = (int64_t)eqs.size();
+  // envelope: first column of each row
+  std::vector<int64_t> first(n);
+  for (int64_t i = 0; i < n; i++) first[i] = i;
+  for (int64_t a = 0; a < np; a++)
+    for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++) {
+      int32_t bcol = f->h_bcols[k];
+      for (int i = 0; i < nd; i++) for (int j = 0; j < nd; j++) {
+        int64_t r = pos[(int64_t)nd * a + i], c = pos[(int64_t)nd * bcol + j];
+        if (r < 0 || c < 0) continue;
+        if (c < first[r]) first[r] = c;
+      }
+    }
+  std::vector<int64_t> rptr(n + 1, 0);
+  for (int64_t i = 0; i < n; i++) rptr[i + 1] = rptr[i] + (i - first[i] + 1);
+  if (rptr[n] > (int64_t)4e9) { f->err = "host direct solver: envelope too large"; return 1; }
+  std::vector<double> L(rptr[n], 0.0);
+  auto at = [&](int64_t r, int64_t c) -> double& { return L[rptr[r] + (c - first[r])]; };
+  for (int64_t a = 0; a < np; a++)
+    for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++) {
+      int32_t bcol = f->h_bcols[k];
+      for (int i = 0; i < nd; i++) for (int j = 0; j < nd; j++) {
+        int64_t r = pos[(int64_t)nd * a + i], c = pos[(int64_t)nd * bcol + j];
+        if (r < 0 || c < 0 || c > r) continue;
+        at(r, c) = kv[(size_t)k * nd * nd + i * nd + j];
+      }
+    }
+  // Cholesky in envelope storage (row-oriented)
+  for (int64_t i = 0; i < n; i++) {
+    for (int64_t j = first[i]; j <= i; j++) {
+      int64_t k0 = std::max(first[i], first[j]);
+      double s = at(i, j);
+      const double* li = &L[rptr[i] + (k0 - first[i])];
+      const double* lj = &L[rptr[j] + (k0 - first[j])];
+      for (int64_t k = 0; k < j - k0; k++) s -= li[k] * lj[k];
+      if (j == i) {
+        if (s <= 0.0) { f->err = "host direct solver: matrix is not positive definite"; return 1; }
+        at(i, i) = std::sqrt(s);
+      } else {
+        at(i, j) = s / at(j, j);
+      }
+    }
+  }
+  std::vector<double> y(n);
+  for (int64_t i = 0; i < n; i++) {
+    double s = b[eqs[i]];
+    for (int64_t k = first[i]; k < i; k++) s -= at(i, k) * y[k];
+    y[i] = s / at(i, i);
+  }
+  for (int64_t i = n - 1; i >= 0; i--) {
+    y[i] /= at(i, i);
+    for (int64_t k = first[i]; k < i; k++) y[k] -= at(i, k) * y[i];
+  }
+  x.assign(ndof, 0.0);
+  for (int64_t i = 0; i < n; i++) x[eqs[i]] = y[i];
+  return 0;
+}
+
+}  // namespace pm

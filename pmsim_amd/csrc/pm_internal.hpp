@@ -1,0 +1,200 @@
+// Internal structures of the MI355X solid element loop library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pmsim_amd.h"
+
+namespace pm {
+
+enum Kind { TRI3 = 0, TRI6, TRI10, TRI15, QUA4, QUA8, QUA9, TET4, TET10, HEX8, HEX20, NKIND };
+enum FacetKind { LIN2 = 100, LIN3, LIN4, LIN5 };
+
+int kind_nnode(int kind);
+int kind_ndim(int kind);
+int kind_default_ngauss(int kind);
+
+// Integration rule: points (r,s,t) and weights
+struct Rule {
+  std::vector<double> x;  // npoint*3
+  std::vector<double> w;  // npoint
+  int n() const { return (int)w.size(); }
+};
+bool gauss_rule(int kind, int ngauss, Rule& r);
+
+// Shape functions N and derivatives dN/dxi (nnode*ndim) at a reference point (closed forms).
+void shape(int kind, const double* x, double* N, double* dN);
+
+struct ParamSolid {
+  int model = 0;  // 0 linear elastic, 1 von Mises
+  double young = 0, poisson = 0, hh = 0, z_ini = 0;
+  int ngauss = 0;
+  bool set = false;
+};
+
+// Per-kind element batch on the device.
+struct Group {
+  int kind = 0, ndim = 2, nnode = 0, nip = 0, attribute = 0;
+  ParamSolid param;
+  // material constants
+  double K = 0, G = 0;
+  // plane-stress elastic (LinearElastic only)
+  bool plane_stress = false;
+  std::vector<int64_t> cells;      // global cell ids (ascending)
+  int64_t nel = 0;
+  int64_t ip_base = 0;             // first global IP index (cell order) of this group's first cell
+  // device arrays
+  int32_t* d_conn = nullptr;       // nel*nnode
+  double* d_tab = nullptr;         // [w (nip) | N (nip*nnode) | dN (nip*nnode*ndim)]
+  double* d_state = nullptr;       // SoA: [nsig+4][nel*nip]  (sigma..., z, lambda, elastic, epbar)
+  double* d_backup = nullptr;
+  int64_t fe_base = 0;             // offset (in doubles) of this group's f_e in the element force buffer
+  int64_t ke_base = 0;             // offset (in blocks) of this group's K_e blocks in the block buffer
+  int nblk = 0;                    // nnode*(nnode+1)/2 stored (upper) blocks per element
+};
+
+struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; double value; };
+
+struct Timers { double update = 0, residual = 0, tangent = 0, scatter = 0, linear = 0, total = 0; };
+
+}  // namespace pm
+
+struct pm_fem {
+  // mesh
+  int ndim = 2;
+  int64_t npoint = 0, ncell = 0;
+  std::vector<double> X;
+  std::vector<int32_t> kind, attr;
+  std::vector<int64_t> cptr;
+  std::vector<int32_t> cpts;
+  std::map<int, pm::ParamSolid> params;
+  bool axisym = false, plane_stress = false;
+  double thickness = 1.0;
+  pm_config cfg;
+  // boundary conditions
+  std::vector<int32_t> ess_point, ess_dof;
+  std::vector<double> ess_table;
+  int32_t nstation = 0;
+  std::vector<int32_t> pl_point, pl_dof;
+  std::vector<double> pl_value;
+  std::vector<pm::Facet> facets;
+  // derived sizes
+  int nsig = 4;
+  int64_t ndof = 0, nprescribed = 0, nip = 0;
+  std::vector<int64_t> cell_ip;     // first IP of each cell (cell order)
+  std::vector<int32_t> cell_group;  // group of each cell
+  std::vector<int64_t> cell_local;  // index inside its group
+  std::vector<pm::Group> groups;
+  std::vector<int64_t> prescribed_eq;
+  std::vector<uint8_t> is_prescribed;  // per DOF
+  std::vector<double> f_ext;           // host-assembled external forces (constant in time)
+  // node-block CSR of K
+  int bs = 2;                          // block size = ndim
+  std::vector<int32_t> h_browptr;      // npoint+1
+  std::vector<int32_t> h_bcols;        // nnzb
+  int64_t nnzb = 0;
+  std::vector<int32_t> h_kmap_ptr, h_kmap;   // K gather map (host copy until upload)
+  std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
+  // device
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool initialized = false;
+  double* d_X = nullptr;               // point coordinates
+  int32_t* d_browptr = nullptr;
+  int32_t* d_bcols = nullptr;
+  double* d_kval = nullptr;            // nnzb*bs*bs (FP64)
+  float* d_kval32 = nullptr;           // FP32 copy when tangent precision < FP64
+  double* d_ke = nullptr;              // element K blocks (upper) FP64: nblocks_total*bs*bs
+  float* d_ke32 = nullptr;             // FP32 element blocks
+  int64_t n_ke_blocks = 0;
+  int32_t* d_kmap_ptr = nullptr;       // nnzb+1
+  int32_t* d_kmap = nullptr;           // contributions: (ke block << 1) | transposed
+  double* d_fe = nullptr;              // element forces
+  int64_t n_fe = 0;
+  int32_t* d_fmap_ptr = nullptr;       // npoint+1
+  int32_t* d_fmap = nullptr;           // fe offsets (in units of ndim-vectors)
+  double* d_u = nullptr;
+  double* d_ddu = nullptr;
+  double* d_fint = nullptr;
+  double* d_fext = nullptr;
+  double* d_rr = nullptr;              // residual (ndof)
+  double* d_mdu = nullptr;             // correction (ndof)
+  double* d_mdu0 = nullptr;            // first correction of the step (ndof + nprescribed)
+  double* d_lam = nullptr;             // Lagrange multipliers (nprescribed)
+  double* d_y = nullptr;               // multiplier corrections (nprescribed)
+  double* d_cval = nullptr;            // prescribed values at the current time (nprescribed)
+  int64_t* d_peq = nullptr;            // prescribed equations
+  double* d_free = nullptr;            // 1 for free DOFs, 0 for prescribed (ndof)
+  double* d_pcg = nullptr;             // PCG work vectors (6*ndof)
+  double* d_binv = nullptr;            // block-Jacobi inverses (npoint*bs*bs)
+  double* d_red = nullptr;             // reduction partials
+  double* d_scal = nullptr;            // device scalars
+  int32_t* d_err = nullptr;            // device error flags
+  int64_t device_bytes = 0;
+  // state
+  double t = 0;
+  std::vector<double> h_lam;           // host mirror of multipliers
+  // history
+  std::vector<std::vector<double>> hist_u, hist_lam, hist_ip;
+  std::vector<double> hist_t;
+  // stats
+  pm_stats stats{};
+  pm::Timers timers;
+  std::string err;
+};
+
+namespace pm {
+
+// host setup (pm_host.cpp)
+int build_groups(pm_fem* f);
+int build_patterns(pm_fem* f);
+void assemble_f_ext(pm_fem* f);
+// device (pm_kernels.hip)
+int device_alloc(pm_fem* f);
+void device_free(pm_fem* f);
+void launch_update(pm_fem* f, Group& g, bool do_update);
+void launch_residual_gather(pm_fem* f);
+void launch_tangent(pm_fem* f, Group& g);
+void launch_scatter(pm_fem* f);
+void launch_init_state(pm_fem* f, Group& g);
+void launch_reset_lambda(pm_fem* f, Group& g);
+void launch_copy(pm_fem* f, double* dst, const double* src, int64_t n);
+int check_device_error(pm_fem* f);
+size_t update_shared_bytes(const Group& g);
+size_t tangent_shared_bytes(const Group& g);
+// vector kernels (pm_kernels.hip)
+double dev_max_abs(pm_fem* f, const double* x, int64_t n, const double* mask);
+double dev_max_rel(pm_fem* f, const double* x, const double* x0, int64_t n);
+double dev_dot(pm_fem* f, const double* a, const double* b, int64_t n);
+void dev_residual(pm_fem* f, double* g);
+void dev_rhs(pm_fem* f, const double* g, double* t, double* kt, double* b);
+void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y);
+void dev_update_primary(pm_fem* f, const double* x, const double* y);
+void dev_fill(pm_fem* f, double* x, double v, int64_t n);
+// linear solvers
+int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters);
+int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x);
+void build_block_jacobi(pm_fem* f);
+void dev_spmv(pm_fem* f, const double* x, double* y, bool masked);
+// Newton driver (pm_solver.cpp)
+int newton_solve(pm_fem* f, int32_t* n_converged);
+int element_pass(pm_fem* f, bool update, bool residual, bool tangent);
+int linear_solve_free(pm_fem* f, const double* d_b, double* d_x, int* iters);
+int copy_ip_state_to_host(const pm_fem* f, double* v);
+int copy_ip_state_to_device(pm_fem* f, const double* v);
+
+#define PM_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      f->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call; \
+      return 1;                                                             \
+    }                                                                       \
+  } while (0)
+
+}  // namespace pm

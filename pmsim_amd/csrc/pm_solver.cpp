@@ -1,0 +1,198 @@
+// Newton driver over device-resident state (SolverImplicit::solve / iterate,
+// src/fem/solver_implicit.rs:56-272), with the Lagrange-multiplier system reproduced by
+// condensation of the prescribed DOFs so that the free block can be solved by PCG:
+//
+//   [K  A^T] [x]   [r]      x_p = g = u_p - c,   K_ff x_f = r_f - K_fp g,   y = r_p - (K x)_p,
+//   [A   0 ] [y] = [g]      u -= x, lambda -= y  (bc_prescribed.rs:83-134, solver_data.rs:135-152).
+//
+// The convergence norms are taken over the same augmented vectors as pmsim's, so iteration
+// counts follow the reference's (convergence_control.rs:52-127).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
+  for (auto& g : f->groups) launch_update(f, g, update);
+  if (residual) launch_residual_gather(f);
+  if (tangent) {
+    for (auto& g : f->groups) launch_tangent(f, g);
+    launch_scatter(f);
+  }
+  if (hipGetLastError() != hipSuccess) { f->err = "kernel launch failed"; return 1; }
+  return 0;
+}
+
+int copy_ip_state_to_host(const pm_fem* f, double* v) {
+  const int nv = f->nsig + 4;
+  for (const auto& g : f->groups) {
+    const int64_t nipg = g.nel * g.nip;
+    std::vector<double> h((size_t)nipg * nv);
+    if (hipMemcpyAsync(h.data(), g.d_state, h.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
+    if (hipStreamSynchronize(f->stream) != hipSuccess) return 1;
+    for (int64_t el = 0; el < g.nel; el++) {
+      const int64_t ip0 = f->cell_ip[g.cells[el]];
+      for (int q = 0; q < g.nip; q++)
+        for (int c = 0; c < nv; c++) v[(ip0 + q) * nv + c] = h[(size_t)c * nipg + el * g.nip + q];
+    }
+  }
+  return 0;
+}
+
+int copy_ip_state_to_device(pm_fem* f, const double* v) {
+  const int nv = f->nsig + 4;
+  for (auto& g : f->groups) {
+    const int64_t nipg = g.nel * g.nip;
+    std::vector<double> h((size_t)nipg * nv);
+    for (int64_t el = 0; el < g.nel; el++) {
+      const int64_t ip0 = f->cell_ip[g.cells[el]];
+      for (int q = 0; q < g.nip; q++)
+        for (int c = 0; c < nv; c++) h[(size_t)c * nipg + el * g.nip + q] = v[(ip0 + q) * nv + c];
+    }
+    PM_HIP(hipMemcpyAsync(g.d_state, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    PM_HIP(hipMemcpyAsync(g.d_backup, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    PM_HIP(hipStreamSynchronize(f->stream));
+  }
+  return 0;
+}
+
+// x = K_ff^-1 b_f (prescribed entries of x are zero); PCG with host direct fallback
+int linear_solve_free(pm_fem* f, const double* d_b, double* d_x, int* iters) {
+  *iters = 0;
+  if (f->cfg.lin_sol == 0) {
+    int rc = pcg_solve(f, d_b, d_x, iters);
+    if (rc == 0) return 0;
+    if (rc == 1) return 1;  // NaN/Inf: no fallback
+    f->stats.n_host_fallbacks++;
+  }
+  std::vector<double> b(f->ndof), x;
+  PM_HIP(hipMemcpyAsync(b.data(), d_b, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  for (int64_t i = 0; i < f->ndof; i++) if (f->is_prescribed[i]) b[i] = 0.0;
+  if (host_direct_solve(f, b, x)) return 1;
+  PM_HIP(hipMemcpyAsync(d_x, x.data(), sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+static int save_output(pm_fem* f) {
+  if (!f->cfg.keep_history) return 0;
+  std::vector<double> u(f->ndof), lam(f->nprescribed), ip((size_t)f->nip * (f->nsig + 4));
+  PM_HIP(hipMemcpyAsync(u.data(), f->d_u, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream));
+  if (f->nprescribed) PM_HIP(hipMemcpyAsync(lam.data(), f->d_lam, sizeof(double) * f->nprescribed, hipMemcpyDeviceToHost, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  if (copy_ip_state_to_host(f, ip.data())) { f->err = "HIP error while copying the state"; return 1; }
+  f->hist_u.push_back(std::move(u));
+  f->hist_lam.push_back(std::move(lam));
+  f->hist_ip.push_back(std::move(ip));
+  f->hist_t.push_back(f->t);
+  return 0;
+}
+
+static void set_prescribed_values(pm_fem* f) {
+  if (!f->nprescribed) return;
+  std::vector<double> c(f->nprescribed);
+  int64_t k = (int64_t)std::llround((f->t - f->cfg.t_ini) / f->cfg.dt);
+  if (k < 0) k = 0;
+  if (k >= f->nstation) k = f->nstation - 1;
+  for (int64_t p = 0; p < f->nprescribed; p++) c[p] = f->ess_table[(size_t)p * f->nstation + k];
+  (void)hipMemcpyAsync(f->d_cval, c.data(), sizeof(double) * f->nprescribed, hipMemcpyHostToDevice, f->stream);
+  (void)hipStreamSynchronize(f->stream);
+}
+
+int newton_solve(pm_fem* f, int32_t* n_converged) {
+  auto t_start = std::chrono::steady_clock::now();
+  const pm_config& cfg = f->cfg;
+  *n_converged = 0;
+  f->stats.n_steps = 0; f->stats.n_iterations = 0; f->stats.n_pcg_iterations = 0; f->stats.n_host_fallbacks = 0;
+  f->hist_u.clear(); f->hist_lam.clear(); f->hist_ip.clear(); f->hist_t.clear();
+  if (!cfg.lagrange_mult_method)
+    for (double v : f->ess_table) if (v != 0.0) { f->err = "the Lagrange multiplier method is required for non-zero prescribed values"; return 1; }
+  const int64_t n = f->ndof, npr = f->nprescribed;
+  const bool lmm = cfg.lagrange_mult_method != 0;
+  // time_control.initialize
+  f->t = cfg.t_ini;
+  if (cfg.dt < 1e-10) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
+  // initialize_internal_values and fresh primary variables
+  for (auto& g : f->groups) launch_init_state(f, g);
+  dev_fill(f, f->d_u, 0.0, n);
+  dev_fill(f, f->d_ddu, 0.0, n);
+  if (npr) dev_fill(f, f->d_lam, 0.0, npr);
+  if (element_pass(f, false, false, false)) return 1;  // f_e of the initial state
+  if (save_output(f)) return 1;
+  double t_out = f->t + cfg.dt_out;
+  // work vectors
+  double* g = f->d_pcg + 4 * n;      // prescribed residual rows (npr)
+  double* tv = g + std::max<int64_t>(npr, 1);
+  double* kt = tv + n;
+  double* b = kt + n;
+  double* kx = b + n;                // reuses b region only after b is consumed
+  int step_n_failure = 0;
+  for (int timestep = 0; timestep < cfg.n_max_time_steps; timestep++) {
+    if (f->t + cfg.dt > cfg.t_fin) { if (save_output(f)) return 1; break; }
+    f->t += cfg.dt;
+    set_prescribed_values(f);
+    dev_fill(f, f->d_ddu, 0.0, n);
+    if (!cfg.linear_problem) for (auto& gr : f->groups) launch_reset_lambda(f, gr);
+    bool converged = false;
+    for (int it = 0; it < cfg.n_max_iterations; it++) {
+      f->stats.n_iterations++;
+      launch_residual_gather(f);
+      dev_residual(f, g);
+      double norm_rr = dev_max_abs(f, f->d_rr, n, nullptr);
+      if (lmm && npr) norm_rr = std::max(norm_rr, dev_max_abs(f, g, npr, nullptr));
+      if (!std::isfinite(norm_rr)) { f->err = "Found NaN or Inf"; return 1; }
+      bool conv_rr = norm_rr < cfg.tol_rr_abs;
+      if (!conv_rr && cfg.tol_rr_scaled > 0.0) conv_rr = norm_rr <= cfg.tol_rr_scaled * dev_max_abs(f, f->d_fint, n, nullptr);
+      if (conv_rr) { converged = true; break; }
+      if (it == 0 || !cfg.constant_tangent) {
+        for (auto& gr : f->groups) launch_tangent(f, gr);
+        launch_scatter(f);
+        if (check_device_error(f)) return 1;
+        if (cfg.lin_sol == 0) build_block_jacobi(f);
+      }
+      dev_rhs(f, g, tv, kt, b);
+      int pits = 0;
+      if (linear_solve_free(f, b, f->d_mdu, &pits)) return 1;
+      if (lmm) dev_complete_correction(f, g, f->d_mdu, kx, f->d_y);
+      // analyze_mdu over the augmented correction [x; y]
+      bool conv_mdu = false;
+      if (it == 0) {
+        double nm = dev_max_abs(f, f->d_mdu, n, nullptr);
+        if (lmm && npr) nm = std::max(nm, dev_max_abs(f, f->d_y, npr, nullptr));
+        if (!std::isfinite(nm)) { f->err = "Found NaN or Inf in mdu"; return 1; }
+        launch_copy(f, f->d_mdu0, f->d_mdu, n);
+        if (lmm && npr) launch_copy(f, f->d_mdu0 + n, f->d_y, npr);
+      } else {
+        double rel = dev_max_rel(f, f->d_mdu, f->d_mdu0, n);
+        if (lmm && npr) rel = std::max(rel, dev_max_rel(f, f->d_y, f->d_mdu0 + n, npr));
+        if (!std::isfinite(rel)) { f->err = "Found NaN or Inf in mdu"; return 1; }
+        conv_mdu = rel < cfg.tol_mdu_rel;
+      }
+      if (conv_mdu) { converged = true; break; }
+      dev_update_primary(f, f->d_mdu, f->d_y);
+      // backup at the first iteration; later iterations update from the backup (= restore)
+      if (it == 0 || cfg.linear_problem) for (auto& gr : f->groups) launch_copy(f, gr.d_backup, gr.d_state, gr.nel * gr.nip * (f->nsig + 4));
+      if (element_pass(f, true, false, false)) return 1;
+      if (check_device_error(f)) return 1;
+      if (cfg.linear_problem) { converged = true; break; }
+      if (it == cfg.n_max_iterations - 1) { f->err = "Newton-Raphson did not converge"; return 1; }
+    }
+    if (!converged) { f->err = "Newton-Raphson did not converge"; return 1; }
+    (*n_converged)++;
+    f->stats.n_steps++;
+    bool last = timestep == cfg.n_max_time_steps - 1;
+    if ((converged && f->t >= t_out) || last) { if (save_output(f)) return 1; t_out += cfg.dt_out; }
+    if (++step_n_failure > cfg.allowed_step_n_failure) { f->err = "too many step failures"; return 1; }
+    if (f->t >= cfg.t_fin) break;
+  }
+  if (hipStreamSynchronize(f->stream) != hipSuccess) { f->err = "HIP error at the end of the solve"; return 1; }
+  f->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return 0;
+}
+
+}  // namespace pm

@@ -1,0 +1,208 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the reference's fixtures.
+
+Tolerances: element-level quantities agree with the oracle to 1e-12 relative (FP64, different
+operation order); Newton solutions to the fixtures' own tolerances and to 1e-10 relative against
+the oracle; integer data (CSR pattern, DOF numbering, plastic flags) bit-exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+from gpu_util import csr_dense, pair, rel_err
+from oracle import oracle as orc
+from pmsim_amd import Config, Elem, Essential, FemBase, Natural, ParamSolid, StressStrain, describe
+from pmsim_amd import generators as G
+from pmsim_amd import mesh as M
+from pmsim_amd.solver import Fem
+
+pytestmark = pytest.mark.gpu
+SQ2 = math.sqrt(2.0)
+KINDS = [M.TRI3, M.TRI6, M.TRI10, M.TRI15, M.QUA4, M.QUA8, M.QUA9, M.TET4, M.TET10, M.HEX8, M.HEX20]
+
+
+def _block_problem(kind, n, model=1, ngauss=None, jitter=0.1, axisym=False):
+    nd = 2 if kind <= M.QUA9 else 3
+    mesh = G.block(kind, n, (1.0,) * nd, seed=11, jitter=jitter)
+    if axisym:
+        mesh.coords[:, 0] += 0.5
+    ss = StressStrain.VonMises(1000.0, 0.3, 60.0, 1.0) if model == 1 else StressStrain.LinearElastic(1000.0, 0.3)
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, ss, ngauss)))])
+    cfg = Config(mesh)
+    if axisym:
+        cfg.set_axisymmetric()
+    return mesh, describe(mesh, base, cfg, Essential(), Natural())
+
+
+def _ddu(mesh, scale, seed):
+    rng = np.random.default_rng(seed)
+    nd = mesh.ndim
+    A = rng.uniform(-scale, scale, (nd, nd))
+    return ((mesh.coords @ A.T) + rng.uniform(-scale / 5, scale / 5, mesh.coords.shape)).ravel()
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_element_pass_parity(kind):
+    """update (strain, return map, epbar) -> f_int -> K against the oracle, twice (elastic, then from a
+    partly plastic state so the consistent-tangent branch runs)."""
+    n = (3, 2) if kind <= M.QUA9 else (2, 2, 1)
+    mesh, d = _block_problem(kind, n)
+    o, f = pair(d)
+    o.init_state()
+    for it, scale in enumerate([4e-4, 3e-3]):
+        ddu = _ddu(mesh, scale, 100 + it)
+        base_state = o.get_ip_state()
+        f.set_ip_state(base_state)
+        o.update_secondary(ddu)
+        f.set_ddu(ddu)
+        f.update_secondary_values()
+        so, sf = o.get_ip_state(), f.get_ip_state()
+        assert np.array_equal(so[:, o.ns + 2], sf[:, o.ns + 2]), "plastic flags differ"
+        for c in range(o.ns + 4):
+            assert rel_err(sf[:, c], so[:, c]) < 1e-12 or np.abs(sf[:, c] - so[:, c]).max() < 1e-14
+        f.assemble_f_int()
+        assert rel_err(f.get_f_int(), o.f_int()) < 1e-12
+        f.assemble_kk()
+        rp_o, c_o, v_o = o.kk_csr()
+        rp_f, c_f, v_f = f.get_csr()
+        assert np.array_equal(rp_o, rp_f) and np.array_equal(c_o, c_f), "CSR pattern differs"
+        assert rel_err(v_f, v_o) < 1e-12
+    if it == 1:
+        assert so[:, o.ns + 2].min() == 0.0  # some IPs plastic in the second pass
+
+
+@pytest.mark.parametrize("kind", [M.QUA4, M.QUA8, M.TRI6])
+def test_element_pass_axisymmetric(kind):
+    mesh, d = _block_problem(kind, (2, 2), model=1, axisym=True)
+    o, f = pair(d)
+    o.init_state()
+    ddu = _ddu(mesh, 2e-3, 7)
+    f.set_ddu(ddu)
+    o.update_secondary(ddu)
+    f.update_secondary_values()
+    assert rel_err(f.get_ip_state()[:, :o.ns], o.get_ip_state()[:, :o.ns]) < 1e-12
+    f.assemble_f_int()
+    assert rel_err(f.get_f_int(), o.f_int()) < 1e-12
+    f.assemble_kk()
+    assert rel_err(f.get_csr()[2], o.kk_csr()[2]) < 1e-12
+
+
+def _solve_pair(case, lin_sol="host", prec="f64"):
+    mesh, base, cfg, ess, nat = case
+    cfg.set_lin_sol(lin_sol).set_tangent_precision(prec)
+    d = describe(mesh, base, cfg, ess, nat)
+    f = Fem(d, cfg)
+    f.solve()
+    return d, f, f.history()
+
+
+def _check_spo(case, fixture, tol_u, tol_s, lin_sol):
+    d, f, hist = _solve_pair(case, lin_sol)
+    ref = cases.load_spo(fixture)
+    nd = d["ndim"]
+    assert len(hist) == len(ref) + 1
+    for k, s in enumerate(ref):
+        t, u, lam, ip = hist[k + 1]
+        assert np.abs(u.reshape(-1, nd) - np.asarray(s["displacement"])).max() <= tol_u
+        sig = np.asarray(s["stresses"], dtype=float).reshape(-1, f.ns)
+        sig[:, 3:] *= SQ2
+        assert np.abs(ip[:, : f.ns] - sig).max() <= tol_s
+    # and against the oracle, step by step
+    o = orc.Oracle(d)
+    uo, ipo, _ = o.solve()
+    for k in range(len(hist)):
+        assert rel_err(hist[k][1], uo[k][: f.ndof]) < 1e-9
+    return f
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+def test_spo_single_element(lin_sol):
+    _check_spo(cases.single_element(), "test_von_mises_single_element_2d_ref", 1e-13, 1e-10, lin_sol)
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+def test_spo_four_qua8(lin_sol):
+    _check_spo(cases.four_qua8(), "test_von_mises_2x2_elements_2d_ref", 1e-12, 1e-9, lin_sol)
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+def test_spo_754_footing(lin_sol):
+    _check_spo(cases.spo_754_footing(), "spo_754_footing_ref", 1e-10, 5e-5, lin_sol)
+
+
+def test_spo_755_tensile():
+    _check_spo(cases.spo_755_tensile(), "spo_755_tensile_perf_plast_ref", 1e-7, 1.2e-5, "pcg")
+
+
+@pytest.mark.parametrize("name", list(cases.SMITH.keys()))
+def test_smith_displacements(name):
+    key, tol = cases.SMITH[name]
+    d, f, hist = _solve_pair(cases.smith(name), "host")
+    ref = cases.smith_u(key)
+    u = hist[-1][1]
+    assert np.abs(u[: len(ref)] - ref).max() <= tol
+
+
+def test_footing_c1_vs_oracle():
+    """Plane-strain strip footing, 16x8 Qua8 3x3 IP, 20 increments: displacements, stresses and plastic
+    state against the oracle at 1e-10 relative."""
+    d, f, hist = _solve_pair(cases.footing_c1(), "pcg")
+    o = orc.Oracle(d)
+    uo, ipo, its = o.solve()
+    assert len(hist) == len(uo)
+    t, u, lam, ip = hist[-1]
+    assert rel_err(u, uo[-1][: f.ndof]) < 1e-10
+    assert rel_err(ip[:, : f.ns], ipo[-1][:, : f.ns]) < 1e-9
+    assert np.array_equal(ip[:, f.ns + 2], ipo[-1][:, f.ns + 2])
+    assert rel_err(ip[:, f.ns + 3], ipo[-1][:, f.ns + 3]) < 1e-8
+    assert ip[:, f.ns + 2].min() == 0.0  # plasticity developed
+
+
+def test_hex20_cube_vs_oracle():
+    case = cases.hex20_cube(3, 3, 2, n_inc=4, uz=-0.004)
+    d, f, hist = _solve_pair(case, "pcg")
+    o = orc.Oracle(d)
+    uo, ipo, its = o.solve()
+    assert rel_err(hist[-1][1], uo[-1][: f.ndof]) < 1e-10
+    assert rel_err(hist[-1][3][:, : f.ns], ipo[-1][:, : f.ns]) < 1e-9
+    assert hist[-1][3][:, f.ns + 2].min() == 0.0
+
+
+def test_pcg_matches_host_direct():
+    mesh, base, cfg, ess, nat = cases.footing_c1()
+    d = describe(mesh, base, cfg, ess, nat)
+    f = Fem(d, cfg)
+    f.assemble_kk()
+    rng = np.random.default_rng(0)
+    b = rng.normal(size=f.ndof)
+    x_pcg, its = f.linear_solve(b)
+    cfg2 = Config(mesh); cfg2.set_lin_sol("host")
+    f2 = Fem(d, cfg2)
+    f2.assemble_kk()
+    x_dir, _ = f2.linear_solve(b)
+    assert its > 0 and rel_err(x_pcg, x_dir) < 1e-8
+
+
+def test_hex20_full_size_properties():
+    """64x64x32 Hex20 (131,072 elements, 3,538,944 IPs): for an elastic increment from zero stress,
+    f_int = K ddu (linearity), K is symmetric, and two passes are bitwise identical."""
+    mesh = G.hex20_block(64, 64, 32, seed=20261203, jitter=0.05)
+    base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
+    d = describe(mesh, base, Config(mesh), Essential(), Natural())
+    f = Fem(d, keep_history=False)
+    rng = np.random.default_rng(1)
+    ddu = rng.uniform(-1e-9, 1e-9, f.ndof)
+    f.set_ddu(ddu)
+    f.element_pass()
+    st = f.get_ip_state()
+    assert st[:, f.ns + 2].min() == 1.0  # elastic everywhere
+    fint = f.get_f_int()
+    rp, cols, vals = f.get_csr()
+    import scipy.sparse as sp
+    K = sp.csr_matrix((vals, cols, rp), shape=(f.ndof, f.ndof))
+    assert rel_err(K @ ddu, fint) < 1e-11
+    v, w = rng.normal(size=f.ndof), rng.normal(size=f.ndof)
+    assert abs(v @ (K @ w) - w @ (K @ v)) <= 1e-11 * abs(v @ (K @ w))
+    f.element_pass()
+    assert np.array_equal(f.get_csr()[2], vals) and np.array_equal(f.get_f_int(), fint)
