@@ -51,7 +51,14 @@ int device_alloc(pm_fem* f) {
       shape(g.kind, &R.x[3 * q], N + (size_t)q * g.nnode, dN + (size_t)q * g.nnode * nd);
     }
     const int64_t nipg = g.nel * g.nip;
+    std::vector<uint8_t> rank((size_t)g.nel * g.nnode);
+    for (int64_t el = 0; el < g.nel; el++) {
+      int rk[32];
+      cell_ranks(f, g.cells[el], rk);
+      for (int m = 0; m < g.nnode; m++) rank[(size_t)el * g.nnode + m] = (uint8_t)rk[m];
+    }
     if (dalloc(f, &g.d_conn, (int64_t)conn.size()) || upload(f, g.d_conn, conn.data(), (int64_t)conn.size())) return 1;
+    if (dalloc(f, &g.d_rank, (int64_t)rank.size()) || upload(f, g.d_rank, rank.data(), (int64_t)rank.size())) return 1;
     if (dalloc(f, &g.d_tab, (int64_t)tab.size()) || upload(f, g.d_tab, tab.data(), (int64_t)tab.size())) return 1;
     if (dalloc(f, &g.d_state, nipg * (ns + 4)) || dalloc(f, &g.d_backup, nipg * (ns + 4))) return 1;
     PM_HIP(hipStreamSynchronize(f->stream));  // host vectors go out of scope
@@ -102,7 +109,7 @@ int device_alloc(pm_fem* f) {
 
 void device_free(pm_fem* f) {
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
-  for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
+  for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
   fr(f->d_X); fr(f->d_browptr); fr(f->d_bcols); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);

@@ -31,7 +31,7 @@ int build_groups(pm_fem* f) {
       Rule r;
       if (!gauss_rule(k, ng, r)) { f->err = "requested number of integration points is not available"; return 1; }
       g.nip = r.n();
-      g.nblk = g.nnode * (g.nnode + 1) / 2;
+      g.nblk = g.nnode * g.nnode;
       const ParamSolid& p = g.param;
       if (p.model == 1) {
         if (f->plane_stress) { f->err = "von Mises model does not work in plane-stress"; return 1; }
@@ -80,6 +80,17 @@ int build_groups(pm_fem* f) {
     f->is_prescribed[eq] = 1;
   }
   return 0;
+}
+
+// rank of each local node of cell e among the cell's global point ids (ascending)
+void cell_ranks(const pm_fem* f, int64_t e, int* rk) {
+  const int32_t* pts = &f->cpts[f->cptr[e]];
+  const int n = (int)(f->cptr[e + 1] - f->cptr[e]);
+  for (int a = 0; a < n; a++) {
+    int r = 0;
+    for (int b = 0; b < n; b++) r += (pts[b] < pts[a]) ? 1 : 0;
+    rk[a] = r;
+  }
 }
 
 // Node-block CSR pattern of K and the two deterministic gather maps:
@@ -138,7 +149,7 @@ int build_patterns(pm_fem* f) {
   // per element, block index of every (la, lb) local pair
   int64_t total = 0;
   for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; total += n * n; }
-  if (2 * f->n_ke_blocks >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
+  if (f->n_ke_blocks >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
   std::vector<int32_t> pairblk(total);
   std::vector<int64_t> pbase(f->ncell + 1, 0);
   for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; pbase[e + 1] = pbase[e] + n * n; }
@@ -157,15 +168,14 @@ int build_patterns(pm_fem* f) {
     for (int64_t e = 0; e < f->ncell; e++) {  // ascending cells => ascending contributions
       const Group& g = f->groups[f->cell_group[e]];
       int64_t n = g.nnode, el = f->cell_local[e];
+      int rk[32];
+      cell_ranks(f, e, rk);
       for (int64_t la = 0; la < n; la++)
         for (int64_t lb = 0; lb < n; lb++) {
-          int64_t lo = std::min(la, lb), hi = std::max(la, lb);
-          // upper-triangular local block index (row-major over lo <= hi)
-          int64_t ub = lo * n - lo * (lo - 1) / 2 + (hi - lo);
-          int64_t blk = g.ke_base + el * g.nblk + ub;
-          int32_t ent = (int32_t)((blk << 1) | (la > lb ? 1 : 0));
+          // the element stores node block (la, lb) at rank position (rank[la], rank[lb])
+          int64_t blk = g.ke_base + el * g.nblk + (int64_t)rk[la] * n + rk[lb];
           int32_t pb = pairblk[pbase[e] + la * n + lb];
-          kmap[fill[pb]++] = ent;
+          kmap[fill[pb]++] = (int32_t)blk;
         }
     }
   }

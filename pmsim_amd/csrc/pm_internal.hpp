@@ -50,12 +50,14 @@ struct Group {
   int64_t ip_base = 0;             // first global IP index (cell order) of this group's first cell
   // device arrays
   int32_t* d_conn = nullptr;       // nel*nnode
+  uint8_t* d_rank = nullptr;       // nel*nnode: rank of each local node among the cell's global ids
   double* d_tab = nullptr;         // [w (nip) | N (nip*nnode) | dN (nip*nnode*ndim)]
   double* d_state = nullptr;       // SoA: [nsig+4][nel*nip]  (sigma..., z, lambda, elastic, epbar)
   double* d_backup = nullptr;
   int64_t fe_base = 0;             // offset (in doubles) of this group's f_e in the element force buffer
   int64_t ke_base = 0;             // offset (in blocks) of this group's K_e blocks in the block buffer
-  int nblk = 0;                    // nnode*(nnode+1)/2 stored (upper) blocks per element
+  int nblk = 0;                    // nnode*nnode stored node blocks per element (both triangles, rank order)
+  int tune_epb = 0, tune_nqg = 0;  // tangent work split chosen by the tuner (0: heuristic)
 };
 
 struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; double value; };
@@ -109,11 +111,11 @@ struct pm_fem {
   int32_t* d_bcols = nullptr;
   double* d_kval = nullptr;            // nnzb*bs*bs (FP64)
   float* d_kval32 = nullptr;           // FP32 copy when tangent precision < FP64
-  double* d_ke = nullptr;              // element K blocks (upper) FP64: nblocks_total*bs*bs
+  double* d_ke = nullptr;              // element K blocks (full, rank order) FP64: nblocks_total*bs*bs
   float* d_ke32 = nullptr;             // FP32 element blocks
   int64_t n_ke_blocks = 0;
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
-  int32_t* d_kmap = nullptr;           // contributions: (ke block << 1) | transposed
+  int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
   double* d_fe = nullptr;              // element forces
   int64_t n_fe = 0;
   int32_t* d_fmap_ptr = nullptr;       // npoint+1
@@ -153,6 +155,7 @@ namespace pm {
 // host setup (pm_host.cpp)
 int build_groups(pm_fem* f);
 int build_patterns(pm_fem* f);
+void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 void assemble_f_ext(pm_fem* f);
 // device (pm_kernels.hip)
 int device_alloc(pm_fem* f);

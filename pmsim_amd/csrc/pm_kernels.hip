@@ -105,9 +105,18 @@ __device__ __forceinline__ void stress_update(const Mat& mp, double* sig, double
   epbar += l;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2); remap so each XCD
+// walks a contiguous range of logical blocks and neighbouring elements / rows share one L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t per = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // ------------------------------------------------------------------------------------------------
 // update kernel: gather (coords, ddu) -> strain increment -> stress update from the backup state ->
 // element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
+// Shape tables are read through the vector L1 (they are a few KB and shared by all workgroups).
 // ------------------------------------------------------------------------------------------------
 template <int ND, int NN>
 __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, const int32_t* __restrict__ conn,
@@ -118,16 +127,14 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int SW = NS + ND * ND + 1;  // per IP: c*sigma, J^-1, radius
   extern __shared__ double sh[];
-  const int ntab = nip * (1 + NN + NN * ND);
-  double* sw = sh;
-  double* sN = sw + nip;
-  double* sdN = sN + nip * NN;
-  double* sX = sh + ntab;
+  const double* __restrict__ tw = tab;
+  const double* __restrict__ tN = tab + nip;
+  const double* __restrict__ tdN = tN + nip * NN;
+  double* sX = sh;
   double* sU = sX + epb * NN * ND;
   double* sS = sU + epb * NN * ND;
   const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * epb;
-  for (int i = tid; i < ntab; i += blockDim.x) sh[i] = tab[i];
+  const int64_t e0 = xcd_block() * epb;
   for (int i = tid; i < epb * NN; i += blockDim.x) {
     int64_t e = e0 + i / NN;
     if (e < nel) {
@@ -142,14 +149,14 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
     const int el = w / nip, q = w - el * nip;
     const int64_t e = e0 + el;
     if (e >= nel) continue;
-    const double* dN = sdN + q * NN * ND;
+    const double* dN = tdN + q * NN * ND;
     const double* Xe = sX + el * NN * ND;
     const double* Ue = sU + el * NN * ND;
     double Ji[ND][ND], det;
     jacobian<ND, NN>(Xe, dN, Ji, det);
     if (!(det > 0.0)) *err = ERR_DETJ;
     double r = 0.0;
-    if (axisym) for (int m = 0; m < NN; m++) r += sN[q * NN + m] * Xe[m * ND];
+    if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * Xe[m * ND];
     // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
     double exx = 0, eyy = 0, ezz = 0, exy = 0, eyz = 0, exz = 0;
 #pragma unroll 4
@@ -171,7 +178,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
         exy += (ux * G[1] + uy * G[0]) * 0.5; eyz += (uy * G[2] + uz * G[1]) * 0.5; exz += (ux * G[2] + uz * G[0]) * 0.5;
       }
     }
-    if (axisym) for (int m = 0; m < NN; m++) ezz += Ue[m * ND] * sN[q * NN + m] / r;
+    if (axisym) for (int m = 0; m < NN; m++) ezz += Ue[m * ND] * tN[q * NN + m] / r;
     double de[NS];
     de[0] = exx; de[1] = eyy; de[2] = ezz; de[3] = exy * SQ2;
     if constexpr (NS == 6) { de[4] = eyz * SQ2; de[5] = exz * SQ2; }
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
       st_out[(NS + 2) * nipg + ip] = el_;
       st_out[(NS + 3) * nipg + ip] = ep;
     }
-    double coef = det * sw[q] * thickness * (axisym ? r : 1.0);
+    double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
     double* S = sS + (el * nip + q) * SW;
 #pragma unroll
     for (int c = 0; c < NS; c++) S[c] = coef * sig[c];
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
     for (int d = 0; d < ND; d++) fm[d] = 0.0;
     for (int q = 0; q < nip; q++) {
       const double* S = sS + (el * nip + q) * SW;
-      const double* dN = sdN + (q * NN + m) * ND;
+      const double* dN = tdN + (q * NN + m) * ND;
       double G[ND];
 #pragma unroll
       for (int j = 0; j < ND; j++) {
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
       if constexpr (ND == 2) {
         double sxy = S[3] * ISQ2;
         double fx = S[0] * G[0] + sxy * G[1];
-        if (axisym) fx += S[2] * sN[q * NN + m] / S[NS + 4];
+        if (axisym) fx += S[2] * tN[q * NN + m] / S[NS + 4];
         fm[0] += fx;
         fm[1] += sxy * G[0] + S[1] * G[1];
       } else {
@@ -238,64 +245,94 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
 }
 
 // ------------------------------------------------------------------------------------------------
-// tangent kernel: K_e node blocks (a <= b) of  sum_q c_q B_a^T D_q B_b  (element_solid.rs:168-185).
+// tangent kernel: K_e node blocks of  sum_q c_q B_a^T D_q B_b  (element_solid.rs:168-185).
 // For the isotropic forms used here D = alpha*I + beta*I(x)I + gamma*s(x)s (Mandel), so
 //   K_ab = sum_q alpha/2 (Ga.Gb) 1 + alpha/2 Gb Ga^T + alpha h_a h_b e_x e_x^T + beta tIa tIb^T + gamma tsa tsb^T
-// with tI = B^T I, ts = B^T s, h = N/r (axisymmetric hoop term).
+// with tI = B^T I, ts = B^T s, h = N/r (axisymmetric hoop term): three rank-1 terms and a diagonal per
+// node block (33 FMA per block and IP instead of the dense 6x6 triple product).
+//
+// Work split per workgroup: epb elements x nqg IP groups x NT upper 2x2 tiles of node blocks.  The node
+// vectors (G, h, ts) of every IP are staged in LDS first; each thread accumulates its tile over its IP
+// group in registers; the IP groups are summed by a halving tree in LDS; the element matrix is then
+// staged in LDS in rank order (nodes sorted by global id, both triangles) and written contiguously.
 // ------------------------------------------------------------------------------------------------
 template <int ND>
-struct TanIP {  // per IP data derived from the state
-  double alpha, beta, gamma;
+struct NodeVec { static constexpr int W = 2 * ND + 1; };  // G(ND), h, ts(ND): odd stride in dwords/2 -> no LDS bank conflicts
+
+template <int ND, int NN>
+struct TanShape {
+  static constexpr int NH = (NN + 1) / 2;
+  static constexpr int NT = NH * (NH + 1) / 2;
+  static constexpr int NNP = 2 * NH;
+  static constexpr int PW = NodeVec<ND>::W;
+  static constexpr int QW = 3 + ND * ND + 6 + 1;  // alpha/2, beta, gamma (times c), J^-1 (J first), s, r
+  static constexpr int B2 = ND * ND;
 };
 
 template <int ND, int NN, typename KT>
-__global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, const int32_t* __restrict__ conn,
-                                                 const double* __restrict__ X, const double* __restrict__ tab,
-                                                 const double* __restrict__ st, KT* __restrict__ ke, Mat mp, int axisym,
-                                                 double thickness, int32_t* err) {
+__global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, int nqg, const int32_t* __restrict__ conn,
+                                                 const uint8_t* __restrict__ rank, const double* __restrict__ X,
+                                                 const double* __restrict__ tab, const double* __restrict__ st,
+                                                 KT* __restrict__ ke, Mat mp, int axisym, double thickness, int32_t* err) {
+  using TS = TanShape<ND, NN>;
   constexpr int NS = ND == 2 ? 4 : 6;
-  constexpr int NBLK = NN * (NN + 1) / 2;
-  constexpr int PW = 2 * ND + 1;         // per (IP, node): G, ts, h
-  constexpr int QW = 3 + ND * ND + 6 + 1;  // per IP: alpha, beta, gamma, J^-1, s (up to 6), r
+  constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH;
+  constexpr int DS = NN * ND + 1;  // padded stride of one IP's dN rows in LDS
   extern __shared__ double sh[];
-  const int ntab = nip * (1 + NN + NN * ND);
-  double* sw = sh;
-  double* sN = sw + nip;
-  double* sdN = sN + nip * NN;
-  double* sX = sh + ntab;
+  double* sw = sh;                       // nip weights
+  double* sdN = sw + nip;                // nip * DS
+  double* sN = sdN + nip * DS;           // nip * NN (axisymmetric only)
+  double* sX = sN + (axisym ? nip * NN : 0);
   double* sQ = sX + epb * NN * ND;
-  double* sP = sQ + epb * nip * QW;
-  __shared__ unsigned char pa[NBLK], pb[NBLK];
+  double* sP = sQ + epb * nip * QW;  // node vectors; later the reduction buffer and the output staging
+  __shared__ uint8_t srank[2 * TPB];  // epb * NN <= TPB * NN / NT <= 4/3 TPB
   const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * epb;
-  for (int i = tid; i < ntab; i += blockDim.x) sh[i] = tab[i];
-  for (int i = tid; i < NBLK; i += blockDim.x) {
-    int a = 0, rem = i;
-    while (rem >= NN - a) { rem -= NN - a; a++; }
-    pa[i] = (unsigned char)a; pb[i] = (unsigned char)(a + rem);
-  }
+  const int64_t e0 = xcd_block() * epb;
+  for (int i = tid; i < nip; i += blockDim.x) sw[i] = tab[i];
+  for (int i = tid; i < nip * NN * ND; i += blockDim.x) { const int q = i / (NN * ND); sdN[q * DS + (i - q * NN * ND)] = tab[nip + nip * NN + i]; }
+  if (axisym) for (int i = tid; i < nip * NN; i += blockDim.x) sN[i] = tab[nip + i];
+  const double* tw = sw;
+  const double* tN = sN;
   for (int i = tid; i < epb * NN; i += blockDim.x) {
     int64_t e = e0 + i / NN;
     if (e < nel) {
       int64_t p = conn[e0 * NN + i];
+      srank[i] = rank[e0 * NN + i];
 #pragma unroll
       for (int d = 0; d < ND; d++) sX[i * ND + d] = X[p * ND + d];
     }
   }
   __syncthreads();
+  // J entries in parallel: J[i][j] = sum_m X_m[i] dN_m[j]
+  for (int w = tid; w < epb * nip * B2; w += blockDim.x) {
+    const int el = w / (nip * B2), rem = w - el * nip * B2, q = rem / B2, ij = rem - q * B2;
+    if (e0 + el >= nel) continue;
+    const int i = ij / ND, j = ij - i * ND;
+    const double* Xe = sX + el * NN * ND;
+    const double* dN = sdN + q * DS;
+    double s = 0.0;
+#pragma unroll 4
+    for (int m = 0; m < NN; m++) s += Xe[m * ND + i] * dN[m * ND + j];
+    sQ[(el * nip + q) * QW + 3 + ij] = s;
+  }
+  __syncthreads();
   const int64_t nipg = nel * nip;
-  // per IP: J^-1, coefficients of D (von_mises.rs:124-167)
+  // per IP: J^-1 in place, D coefficients (von_mises.rs:124-167), deviator
   for (int w = tid; w < epb * nip; w += blockDim.x) {
     const int el = w / nip, q = w - el * nip;
     const int64_t e = e0 + el;
     if (e >= nel) continue;
-    const double* Xe = sX + el * NN * ND;
-    double Ji[ND][ND], det;
-    jacobian<ND, NN>(Xe, sdN + q * NN * ND, Ji, det);
+    double* Q = sQ + (el * nip + q) * QW;
+    double J[ND][ND], Ji[ND][ND], det;
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) J[i][j] = Q[3 + i * ND + j];
+    if constexpr (ND == 2) inv_jac2(J, Ji, det); else inv_jac3(J, Ji, det);
     if (!(det > 0.0)) *err = ERR_DETJ;
     double r = 0.0;
-    if (axisym) for (int m = 0; m < NN; m++) r += sN[q * NN + m] * Xe[m * ND];
-    double coef = det * sw[q] * thickness * (axisym ? r : 1.0);
+    if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * sX[(el * NN + m) * ND];
+    double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
     const int64_t ip = e * nip + q;
     double alpha, beta, gamma = 0.0;
     double s[6] = {0, 0, 0, 0, 0, 0};
@@ -322,8 +359,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
       gamma = 6.0 * mp.G * mp.G * (lam / sdtr - 1.0 / d) / (norm_s * norm_s);
       beta = mp.K - alpha / 3.0;
     }
-    double* Q = sQ + (el * nip + q) * QW;
-    Q[0] = coef * alpha * 0.5;  // alpha/2 (c folded in)
+    Q[0] = coef * alpha * 0.5;
     Q[1] = coef * beta;
     Q[2] = coef * gamma;
 #pragma unroll
@@ -335,107 +371,193 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     Q[3 + ND * ND + 6] = r;
   }
   __syncthreads();
-  // per (IP, node): G = dN J^-1, ts = B^T s, h = N / r
-  for (int w = tid; w < epb * nip * NN; w += blockDim.x) {
-    const int el = w / (nip * NN), rem = w - el * nip * NN, q = rem / NN, m = rem - q * NN;
-    if (e0 + el >= nel) continue;
+  // node vectors of every (element, IP, node); the padding node of odd NN stays zero
+  for (int w = tid; w < epb * nip * NNP; w += blockDim.x) {
+    const int el = w / (nip * NNP), rem = w - el * nip * NNP, q = rem / NNP, m = rem - q * NNP;
+    double* P = sP + w * PW;
+    if (e0 + el >= nel || m >= NN) {
+#pragma unroll
+      for (int k = 0; k < PW; k++) P[k] = 0.0;
+      continue;
+    }
     const double* Q = sQ + (el * nip + q) * QW;
-    const double* dN = sdN + (q * NN + m) * ND;
+    const double* dN = sdN + q * DS + m * ND;
     double G[ND];
 #pragma unroll
     for (int j = 0; j < ND; j++) {
-      double s = 0.0;
+      double sum = 0.0;
 #pragma unroll
-      for (int k = 0; k < ND; k++) s += dN[k] * Q[3 + k * ND + j];
-      G[j] = s;
+      for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
+      G[j] = sum;
     }
-    const double* s = Q + 3 + ND * ND;
-    double h = axisym ? sN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
-    double* P = sP + ((el * nip + q) * NN + m) * PW;
+    const double* sv = Q + 3 + ND * ND;
+    double h = axisym ? tN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
     if constexpr (ND == 2) {
-      double sxy = s[3] * ISQ2;
-      P[0] = G[0]; P[1] = G[1];
-      P[2] = s[0] * G[0] + sxy * G[1] + h * s[2];
-      P[3] = sxy * G[0] + s[1] * G[1];
+      double sxy = sv[3] * ISQ2;
+      P[0] = G[0]; P[1] = G[1]; P[2] = h;
+      P[3] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
+      P[4] = sxy * G[0] + sv[1] * G[1];
     } else {
-      double sxy = s[3] * ISQ2, syz = s[4] * ISQ2, sxz = s[5] * ISQ2;
-      P[0] = G[0]; P[1] = G[1]; P[2] = G[2];
-      P[3] = s[0] * G[0] + sxy * G[1] + sxz * G[2];
-      P[4] = sxy * G[0] + s[1] * G[1] + syz * G[2];
-      P[5] = sxz * G[0] + syz * G[1] + s[2] * G[2];
+      double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
+      P[0] = G[0]; P[1] = G[1]; P[2] = G[2]; P[3] = h;
+      P[4] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
+      P[5] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
+      P[6] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
     }
-    P[2 * ND] = h;
   }
   __syncthreads();
-  for (int w = tid; w < epb * NBLK; w += blockDim.x) {
-    const int el = w / NBLK, bk = w - el * NBLK;
-    const int64_t e = e0 + el;
-    if (e >= nel) continue;
-    const int a = pa[bk], b = pb[bk];
-    double Kb[ND][ND];
+  // this thread: element el, IP group qg, tile (I <= J) -> blocks (2I+u, 2J+v)
+  const int per_el = nqg * NT;
+  const int el = tid / per_el, rem0 = tid - el * per_el, qg = rem0 / NT, tile = rem0 - qg * NT;
+  const bool active = el < epb && (e0 + el) < nel;
+  int I = 0, J = 0;
+  {
+    int rem = tile;
+    while (rem >= NH - I) { rem -= NH - I; I++; }
+    J = I + rem;
+  }
+  double K[2][2][ND][ND];
 #pragma unroll
-    for (int i = 0; i < ND; i++)
+  for (int u = 0; u < 2; u++)
 #pragma unroll
-      for (int j = 0; j < ND; j++) Kb[i][j] = 0.0;
-    for (int q = 0; q < nip; q++) {
+    for (int v = 0; v < 2; v++)
+#pragma unroll
+      for (int i = 0; i < ND; i++)
+#pragma unroll
+        for (int j = 0; j < ND; j++) K[u][v][i][j] = 0.0;
+  if (active) {
+    for (int q = qg; q < nip; q += nqg) {
       const double* Q = sQ + (el * nip + q) * QW;
-      const double* Pa = sP + ((el * nip + q) * NN + a) * PW;
-      const double* Pb = sP + ((el * nip + q) * NN + b) * PW;
       const double ha = Q[0], be = Q[1], ga = Q[2];
-      double Ga[ND], Gb[ND], ta[ND], tb[ND];
+      const double* Pq = sP + (el * nip + q) * NNP * PW;
+      double Ga[2][ND], ta[2][ND], hA[2];
 #pragma unroll
-      for (int i = 0; i < ND; i++) { Ga[i] = Pa[i]; Gb[i] = Pb[i]; ta[i] = Pa[ND + i]; tb[i] = Pb[ND + i]; }
-      const double hha = Pa[2 * ND], hhb = Pb[2 * ND];
-      double dot = 0.0;
+      for (int u = 0; u < 2; u++) {
+        const double* Pa = Pq + (2 * I + u) * PW;
 #pragma unroll
-      for (int i = 0; i < ND; i++) dot += Ga[i] * Gb[i];
-      double tIa0 = Ga[0] + hha, tIb0 = Gb[0] + hhb;
+        for (int i = 0; i < ND; i++) { Ga[u][i] = Pa[i]; ta[u][i] = Pa[ND + 1 + i]; }
+        hA[u] = Pa[ND];
+      }
 #pragma unroll
-      for (int i = 0; i < ND; i++) {
-        double tIai = i == 0 ? tIa0 : Ga[i];
+      for (int v = 0; v < 2; v++) {
+        const double* Pb = Pq + (2 * J + v) * PW;
+        double Gb[ND], ub[ND], vb[ND], wb[ND];
 #pragma unroll
-        for (int j = 0; j < ND; j++) {
-          double tIbj = j == 0 ? tIb0 : Gb[j];
-          double v = ha * Gb[i] * Ga[j] + be * tIai * tIbj + ga * ta[i] * tb[j];
-          if (i == j) v += ha * dot;
-          Kb[i][j] += v;
+        for (int i = 0; i < ND; i++) Gb[i] = Pb[i];
+        const double hb = Pb[ND];
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+          ub[i] = ha * Gb[i];
+          vb[i] = be * (Gb[i] + (i == 0 ? hb : 0.0));
+          wb[i] = ga * Pb[ND + 1 + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          double dot = 0.0;
+#pragma unroll
+          for (int i = 0; i < ND; i++) dot += Ga[u][i] * Gb[i];
+          const double hd = ha * dot;
+#pragma unroll
+          for (int i = 0; i < ND; i++) {
+            const double tIa = Ga[u][i] + (i == 0 ? hA[u] : 0.0);
+#pragma unroll
+            for (int j = 0; j < ND; j++) K[u][v][i][j] += ub[i] * Ga[u][j] + tIa * vb[j] + ta[u][i] * wb[j];
+            K[u][v][i][i] += hd;
+          }
+          K[u][v][0][0] += 2.0 * ha * hA[u] * hb;
         }
       }
-      Kb[0][0] += 2.0 * ha * hha * hhb;
     }
-    KT* out = ke + ((int64_t)e * NBLK + bk) * ND * ND;
-#pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) out[i * ND + j] = (KT)Kb[i][j];
   }
+  // sum the IP groups: halving tree through LDS (the node vectors are no longer needed)
+  constexpr int TW = 4 * B2 + 1;  // doubles per tile, padded to an odd stride
+  for (int n = nqg; n > 1;) {
+    const int h = (n + 1) / 2;
+    __syncthreads();
+    if (active && qg >= h && qg < n) {
+      double* buf = sP + (((qg - h) * epb + el) * NT + tile) * TW;
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++)
+#pragma unroll
+          for (int i = 0; i < ND; i++)
+#pragma unroll
+            for (int j = 0; j < ND; j++) buf[((u * 2 + v) * ND + i) * ND + j] = K[u][v][i][j];
+    }
+    __syncthreads();
+    if (active && qg < n - h) {
+      const double* buf = sP + ((qg * epb + el) * NT + tile) * TW;
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++)
+#pragma unroll
+          for (int i = 0; i < ND; i++)
+#pragma unroll
+            for (int j = 0; j < ND; j++) K[u][v][i][j] += buf[((u * 2 + v) * ND + i) * ND + j];
+    }
+    n = h;
+  }
+  // stage the full element matrix in rank order: block (ra, rb) at (ra*NN + rb)*B2
+  __syncthreads();
+  KT* stage = reinterpret_cast<KT*>(sP);
+  constexpr int EW = NN * NN * B2;  // values per element
+  if (active && qg == 0) {
+    const uint8_t* rk = srank + el * NN;
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        const int a = 2 * I + u, b = 2 * J + v;
+        if (a >= NN || b >= NN || a > b) continue;
+        const int ra = rk[a], rb = rk[b];
+        KT* o1 = stage + el * EW + (ra * NN + rb) * B2;
+        KT* o2 = stage + el * EW + (rb * NN + ra) * B2;
+#pragma unroll
+        for (int i = 0; i < ND; i++)
+#pragma unroll
+          for (int j = 0; j < ND; j++) { o1[i * ND + j] = (KT)K[u][v][i][j]; o2[j * ND + i] = (KT)K[u][v][i][j]; }
+      }
+  }
+  __syncthreads();
+  const int64_t nv = (int64_t)min((int64_t)epb, nel - e0) * EW;
+  KT* dst = ke + e0 * EW;
+  for (int64_t i = tid; i < nv; i += blockDim.x) dst[i] = stage[i];
 }
 
-// deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135)
+// deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135): one thread
+// per CSR block sums its contributing element blocks in ascending cell order
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ kptr, const int32_t* __restrict__ kmap,
                                                  const KT* __restrict__ ke, double* __restrict__ kval, float* __restrict__ kval32) {
   constexpr int B2 = BS * BS;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nnzb * B2) return;
-  const int64_t blk = t / B2;
-  const int c = (int)(t - blk * B2);
-  const int ct = (c % BS) * BS + c / BS;
-  double s = 0.0;
-  for (int32_t k = kptr[blk]; k < kptr[blk + 1]; k++) {
-    const int32_t ent = kmap[k];
-    const int64_t b = (int64_t)(ent >> 1);
-    s += (double)ke[b * B2 + ((ent & 1) ? ct : c)];
+  const int64_t blk = xcd_block() * blockDim.x + threadIdx.x;
+  if (blk >= nnzb) return;
+  double acc[B2];
+#pragma unroll
+  for (int c = 0; c < B2; c++) acc[c] = 0.0;
+  const int32_t k1 = kptr[blk + 1];
+  for (int32_t k = kptr[blk]; k < k1; k++) {
+    const KT* src = ke + (int64_t)kmap[k] * B2;
+#pragma unroll
+    for (int c = 0; c < B2; c++) acc[c] += (double)src[c];
   }
-  if (kval) kval[t] = s;
-  if (kval32) kval32[t] = (float)s;
+  if (kval) {
+#pragma unroll
+    for (int c = 0; c < B2; c++) kval[blk * B2 + c] = acc[c];
+  }
+  if (kval32) {
+#pragma unroll
+    for (int c = 0; c < B2; c++) kval32[blk * B2 + c] = (float)acc[c];
+  }
 }
 
 // deterministic gather of element forces into the global internal force vector (assembly.rs:38-46)
 template <int ND>
 __global__ __launch_bounds__(TPB) void k_rgather(int64_t np, const int32_t* __restrict__ fptr, const int32_t* __restrict__ fmap,
                                                  const double* __restrict__ fe, double* __restrict__ fint) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= np * ND) return;
   const int64_t a = t / ND;
   const int d = (int)(t - a * ND);
@@ -481,21 +603,30 @@ constexpr size_t SHM_CAP = 60 * 1024;  // dynamic LDS budget per workgroup (stat
 
 static size_t shm_update(int nd, int nn, int nip, int epb) {
   int ns = nd == 2 ? 4 : 6;
-  return sizeof(double) * ((size_t)nip * (1 + nn + nn * nd) + 2 * (size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
+  return sizeof(double) * (2 * (size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
 }
 static int epb_update(int nd, int nn, int nip) {
   int epb = std::max(1, TPB / nip);
   while (epb > 1 && shm_update(nd, nn, nip, epb) > SHM_CAP) epb--;
   return epb;
 }
-static size_t shm_tangent(int nd, int nn, int nip, int epb) {
-  return sizeof(double) * ((size_t)nip * (1 + nn + nn * nd) + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) +
-                           (size_t)epb * nip * nn * (2 * nd + 1));
+static size_t shm_tangent(int nd, int nn, int nip, int epb, int nqg, bool axisym) {
+  int nh = (nn + 1) / 2, nt = nh * (nh + 1) / 2, pw = 2 * nd + 1, b2 = nd * nd;
+  size_t table = (size_t)nip * (1 + nn * nd + 1) + (axisym ? (size_t)nip * nn : 0);
+  size_t pblk = (size_t)epb * nip * 2 * nh * pw;                 // node vectors
+  size_t red = (size_t)(nqg / 2) * epb * nt * (4 * b2 + 1);      // largest halving-tree level
+  size_t stg = (size_t)epb * nn * nn * b2;                        // output staging (FP64 bound)
+  size_t uni = std::max(pblk, std::max(red, stg));
+  return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + uni);
 }
-static int epb_tangent(int nd, int nn, int nip) {
-  int epb = std::max(1, TPB / (nn * (nn + 1) / 2));
-  while (epb > 1 && shm_tangent(nd, nn, nip, epb) > SHM_CAP) epb--;
-  return epb;
+// work split of the tangent kernel: elements per workgroup and IP groups per element
+static void split_tangent(int nd, int nn, int nip, bool axisym, int& epb, int& nqg) {
+  int nh = (nn + 1) / 2, nt = nh * (nh + 1) / 2;
+  nqg = std::max(1, std::min(nip, TPB / nt));
+  if (nqg > 4) nqg = 4;
+  epb = std::max(1, TPB / (nt * nqg));
+  while (epb > 1 && shm_tangent(nd, nn, nip, epb, nqg, axisym) > SHM_CAP) epb--;
+  while (nqg > 1 && shm_tangent(nd, nn, nip, epb, nqg, axisym) > SHM_CAP) nqg--;
 }
 
 #define PM_DISPATCH_KIND(KIND, ND, NN, ...)          \
@@ -527,26 +658,27 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
 
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
-  const int epb = epb_tangent(g.ndim, g.nnode, g.nip);
+  int epb, nqg;
+  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, nqg);
+  if (g.tune_epb > 0) { epb = g.tune_epb; nqg = g.tune_nqg; }
   const int64_t grid = (g.nel + epb - 1) / epb;
-  const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb);
+  const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, nqg, f->axisym);
   const Mat mp = mat_of(f, g);
-  const size_t b2 = (size_t)g.ndim * g.ndim;
   if (f->cfg.tangent_precision == 0) {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, g.d_conn, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base * b2, mp,
+                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base * (size_t)(g.ndim * g.ndim), mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   } else {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, g.d_conn, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * b2, mp,
+                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * (size_t)(g.ndim * g.ndim), mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   }
 }
 
 void launch_scatter(pm_fem* f) {
-  const int64_t n = f->nnzb * f->bs * f->bs;
+  const int64_t n = f->nnzb;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;
   if (f->bs == 2) {
@@ -605,6 +737,6 @@ int check_device_error(pm_fem* f) {
 }
 
 size_t update_shared_bytes(const Group& g) { return shm_update(g.ndim, g.nnode, g.nip, epb_update(g.ndim, g.nnode, g.nip)); }
-size_t tangent_shared_bytes(const Group& g) { return shm_tangent(g.ndim, g.nnode, g.nip, epb_tangent(g.ndim, g.nnode, g.nip)); }
+size_t tangent_shared_bytes(const Group& g) { int e, q; split_tangent(g.ndim, g.nnode, g.nip, false, e, q); return shm_tangent(g.ndim, g.nnode, g.nip, e, q, false); }
 
 }  // namespace pm
