@@ -85,7 +85,7 @@ int device_alloc(pm_fem* f) {
       dalloc(f, &f->d_pcg, 8 * n + 2 * npr) || dalloc(f, &f->d_binv, f->npoint * b2))
     return 1;
   const int64_t gn = (f->npoint + 255) / 256;
-  if (dalloc(f, &f->d_red, std::max<int64_t>(2 * gn, 1024) + 64) || dalloc(f, &f->d_scal, 16) || dalloc(f, &f->d_err, 4)) return 1;
+  if (dalloc(f, &f->d_red, std::max<int64_t>(4096 + 2 * std::min<int64_t>(gn, 2048), 1024) + 64) || dalloc(f, &f->d_scal, 32) || dalloc(f, &f->d_err, 4)) return 1;
   std::vector<double> freev(n, 1.0);
   for (auto eq : f->prescribed_eq) freev[eq] = 0.0;
   if (upload(f, f->d_free, freev.data(), n)) return 1;
@@ -95,7 +95,7 @@ int device_alloc(pm_fem* f) {
   PM_HIP(hipMemsetAsync(f->d_ddu, 0, sizeof(double) * n, f->stream));
   PM_HIP(hipMemsetAsync(f->d_lam, 0, sizeof(double) * npr, f->stream));
   PM_HIP(hipMemsetAsync(f->d_err, 0, sizeof(int32_t) * 4, f->stream));
-  PM_HIP(hipMemsetAsync(f->d_scal, 0, sizeof(double) * 16, f->stream));
+  PM_HIP(hipMemsetAsync(f->d_scal, 0, sizeof(double) * 32, f->stream));
   PM_HIP(hipStreamSynchronize(f->stream));
   for (auto& g : f->groups) { launch_init_state(f, g); launch_copy(f, g.d_backup, g.d_state, g.nel * g.nip * (ns + 4)); }
   PM_HIP(hipStreamSynchronize(f->stream));

@@ -137,6 +137,7 @@ struct pm_fem {
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags
+  int64_t pcg_small_max = 16384;       // systems up to this many DOFs use the single-workgroup PCG
   int64_t device_bytes = 0;
   // state
   double t = 0;

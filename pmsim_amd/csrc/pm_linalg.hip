@@ -235,6 +235,207 @@ __global__ __launch_bounds__(TPB) void k_axpy(int64_t n, double a, const double*
   if (i < n) y[i] += a * x[i];
 }
 
+// ---- fused PCG iteration (3 kernels, fixed grid; every block reduces the previous kernel's partials
+//      in the same order, so the scalars are identical in all blocks and across runs) ----------------
+constexpr int PCG_TPB = 256;
+
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int n, double* sh) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+  return block_sum(acc, sh);
+}
+
+template <int BS>
+__device__ __forceinline__ void bjacobi_apply(int64_t a, const double* __restrict__ binv, const double* __restrict__ free_,
+                                              const double* rv, double* zv) {
+#pragma unroll
+  for (int i = 0; i < BS; i++) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < BS; j++) s += binv[a * BS * BS + i * BS + j] * rv[j];
+    zv[i] = s * free_[a * BS + i];
+  }
+}
+
+// K1: q = (A p) * free ; partial p.q
+template <int BS, typename VT>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_q(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                   const VT* __restrict__ vals, const double* __restrict__ free_,
+                                                   const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
+  __shared__ double sh[PCG_TPB];
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * BS; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = t / BS;
+    const int i = (int)(t - a * BS);
+    double s = 0.0;
+    for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
+      const int64_t b = cols[k];
+      const VT* v = vals + (int64_t)k * BS * BS + i * BS;
+#pragma unroll
+      for (int j = 0; j < BS; j++) s += (double)v[j] * p[b * BS + j];
+    }
+    s *= free_[t];
+    q[t] = s;
+    acc += p[t] * s;
+  }
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// K2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = M^-1 r ; partials r.z, r.r
+template <int BS>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_xrz(int64_t np, int npart, const double* __restrict__ part_pq, const double* __restrict__ rz_in,
+                                                     const double* __restrict__ p, const double* __restrict__ q,
+                                                     const double* __restrict__ free_, const double* __restrict__ binv,
+                                                     double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                     double* __restrict__ part_rz, double* __restrict__ part_rr) {
+  __shared__ double sh[PCG_TPB];
+  const double pq = sum_partials(part_pq, npart, sh);
+  const double alpha = pq != 0.0 ? (*rz_in) / pq : 0.0;
+  double rz = 0.0, rr = 0.0;
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < np; a += (int64_t)gridDim.x * blockDim.x) {
+    double rv[BS], zv[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) {
+      const int64_t k = a * BS + i;
+      x[k] += alpha * p[k];
+      rv[i] = r[k] - alpha * q[k];
+      r[k] = rv[i];
+      rr += rv[i] * rv[i];
+    }
+    bjacobi_apply<BS>(a, binv, free_, rv, zv);
+#pragma unroll
+    for (int i = 0; i < BS; i++) { z[a * BS + i] = zv[i]; rz += rv[i] * zv[i]; }
+  }
+  const double s1 = block_sum(rz, sh), s2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) { part_rz[blockIdx.x] = s1; part_rr[blockIdx.x] = s2; }
+}
+
+// K3: rz' = sum ; beta = rz'/rz ; p = z + beta p ; block 0 stores rz' (other slot) and r.r
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_p2(int64_t n, int npart, const double* __restrict__ part_rz, const double* __restrict__ part_rr,
+                                                    const double* __restrict__ rz_old, double* __restrict__ rz_new_out,
+                                                    double* __restrict__ rr_out, const double* __restrict__ z, double* __restrict__ p) {
+  __shared__ double sh[PCG_TPB];
+  const double rzn = sum_partials(part_rz, npart, sh);
+  const double beta = (*rz_old) != 0.0 ? rzn / (*rz_old) : 0.0;
+  if (blockIdx.x == 0) {
+    const double rr = sum_partials(part_rr, npart, sh);
+    if (threadIdx.x == 0) { *rz_new_out = rzn; *rr_out = rr; }
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+// start: r = b*free, x = 0, z = M^-1 r, p = z ; partials r.z and b.b
+template <int BS>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_init(int64_t np, const double* __restrict__ b, const double* __restrict__ free_,
+                                                      const double* __restrict__ binv, double* __restrict__ x, double* __restrict__ r,
+                                                      double* __restrict__ z, double* __restrict__ p, double* __restrict__ part_rz,
+                                                      double* __restrict__ part_bb) {
+  __shared__ double sh[PCG_TPB];
+  double rz = 0.0, bb = 0.0;
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < np; a += (int64_t)gridDim.x * blockDim.x) {
+    double rv[BS], zv[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) { rv[i] = b[a * BS + i] * free_[a * BS + i]; r[a * BS + i] = rv[i]; x[a * BS + i] = 0.0; bb += rv[i] * rv[i]; }
+    bjacobi_apply<BS>(a, binv, free_, rv, zv);
+#pragma unroll
+    for (int i = 0; i < BS; i++) { z[a * BS + i] = zv[i]; p[a * BS + i] = zv[i]; rz += rv[i] * zv[i]; }
+  }
+  const double s1 = block_sum(rz, sh), s2 = block_sum(bb, sh);
+  if (threadIdx.x == 0) { part_rz[blockIdx.x] = s1; part_bb[blockIdx.x] = s2; }
+}
+
+__global__ void k_pcg_init_fin(int npart, const double* __restrict__ part_rz, const double* __restrict__ part_bb, double* rz_out, double* bb_out) {
+  __shared__ double sh[PCG_TPB];
+  const double a = sum_partials(part_rz, npart, sh);
+  const double b = sum_partials(part_bb, npart, sh);
+  if (threadIdx.x == 0) { *rz_out = a; *bb_out = b; }
+}
+
+// ---- single-workgroup persistent PCG for small systems: the whole loop runs in one launch ----------
+constexpr int SMALL_TPB = 1024;
+
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = a; sh[SMALL_TPB + t] = b;
+  __syncthreads();
+  for (int s = SMALL_TPB / 2; s > 0; s >>= 1) {
+    if (t < s) { sh[t] += sh[t + s]; sh[SMALL_TPB + t] += sh[SMALL_TPB + t + s]; }
+    __syncthreads();
+  }
+  a = sh[0]; b = sh[SMALL_TPB];
+  __syncthreads();
+}
+
+template <int BS, typename VT>
+__global__ __launch_bounds__(SMALL_TPB) void k_pcg_small(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                         const VT* __restrict__ vals, const double* __restrict__ free_,
+                                                         const double* __restrict__ binv, const double* __restrict__ b,
+                                                         double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                         double* __restrict__ p, double* __restrict__ q, double rtol2, int maxit,
+                                                         double* __restrict__ out /* [iterations, rr, bb] */) {
+  __shared__ double sh[2 * SMALL_TPB];
+  const int64_t n = np * BS;
+  double rz = 0.0, bb = 0.0;
+  for (int64_t a = threadIdx.x; a < np; a += SMALL_TPB) {
+    double rv[BS], zv[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) { rv[i] = b[a * BS + i] * free_[a * BS + i]; r[a * BS + i] = rv[i]; x[a * BS + i] = 0.0; bb += rv[i] * rv[i]; }
+    bjacobi_apply<BS>(a, binv, free_, rv, zv);
+#pragma unroll
+    for (int i = 0; i < BS; i++) { z[a * BS + i] = zv[i]; p[a * BS + i] = zv[i]; rz += rv[i] * zv[i]; }
+  }
+  block_sum2(rz, bb, sh);
+  const double tol2 = rtol2 * bb;
+  int it = 0;
+  double rr = bb;
+  if (bb > 0.0) {
+    while (it < maxit) {
+      __syncthreads();  // p complete
+      double pq = 0.0, dummy = 0.0;
+      for (int64_t t = threadIdx.x; t < n; t += SMALL_TPB) {
+        const int64_t a = t / BS;
+        const int i = (int)(t - a * BS);
+        double s = 0.0;
+        for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
+          const int64_t c = cols[k];
+          const VT* v = vals + (int64_t)k * BS * BS + i * BS;
+#pragma unroll
+          for (int j = 0; j < BS; j++) s += (double)v[j] * p[c * BS + j];
+        }
+        s *= free_[t];
+        q[t] = s;
+        pq += p[t] * s;
+      }
+      block_sum2(pq, dummy, sh);
+      const double alpha = pq != 0.0 ? rz / pq : 0.0;
+      double rzn = 0.0;
+      rr = 0.0;
+      for (int64_t a = threadIdx.x; a < np; a += SMALL_TPB) {
+        double rv[BS], zv[BS];
+#pragma unroll
+        for (int i = 0; i < BS; i++) {
+          const int64_t k = a * BS + i;
+          x[k] += alpha * p[k];
+          rv[i] = r[k] - alpha * q[k];
+          r[k] = rv[i];
+          rr += rv[i] * rv[i];
+        }
+        bjacobi_apply<BS>(a, binv, free_, rv, zv);
+#pragma unroll
+        for (int i = 0; i < BS; i++) { z[a * BS + i] = zv[i]; rzn += rv[i] * zv[i]; }
+      }
+      block_sum2(rzn, rr, sh);
+      it++;
+      if (rr <= tol2 || !(rr == rr)) break;
+      const double beta = rz != 0.0 ? rzn / rz : 0.0;
+      rz = rzn;
+      for (int64_t i = threadIdx.x; i < n; i += SMALL_TPB) p[i] = z[i] + beta * p[i];
+    }
+  }
+  if (threadIdx.x == 0) { out[0] = (double)it; out[1] = rr; out[2] = bb; }
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
 
 }  // namespace
@@ -291,41 +492,72 @@ void build_block_jacobi(pm_fem* f) {
 }
 
 // Preconditioned CG on the free DOFs of K (prescribed entries of b ignored, of x zero).
+// Small systems run the whole loop in one single-workgroup launch; larger ones run three fused kernels
+// per iteration on a fixed grid and check convergence on the host every few iterations.
 int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   const int64_t np = f->npoint, n = np * f->bs;
   double* r = f->d_pcg;
   double* z = r + n;
   double* p = z + n;
   double* q = p + n;
-  const unsigned gn = nblk(np);
-  double* part_a = f->d_red;
-  double* part_b = f->d_red + gn;
-  if (f->bs == 2) hipLaunchKernelGGL((k_pcg_start<2>), dim3(gn), dim3(TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, part_a, part_b);
-  else hipLaunchKernelGGL((k_pcg_start<3>), dim3(gn), dim3(TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, part_a, part_b);
-  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(TPB), 0, f->stream, 0, (int)gn, part_a, f->d_scal, (int)S_RZNEW);
-  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(TPB), 0, f->stream, 0, (int)gn, part_b, f->d_scal, (int)S_BB);
-  hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, f->stream, 2, f->d_scal);
-  double bb = read_scal(f, S_BB);
+  const bool lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
+  const double rtol2 = f->cfg.pcg_rtol * f->cfg.pcg_rtol;
   *iters = 0;
+  if (n <= f->pcg_small_max) {
+    double* out = f->d_scal + 20;
+#define PM_SMALL(BS, VT, VALS) hipLaunchKernelGGL((k_pcg_small<BS, VT>), dim3(1), dim3(SMALL_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, VALS, \
+                                                  f->d_free, f->d_binv, d_b, d_x, r, z, p, q, rtol2, f->cfg.pcg_max_iter, out)
+    if (f->bs == 2) { if (lowp) PM_SMALL(2, float, f->d_kval32); else PM_SMALL(2, double, f->d_kval); }
+    else { if (lowp) PM_SMALL(3, float, f->d_kval32); else PM_SMALL(3, double, f->d_kval); }
+#undef PM_SMALL
+    double h[3];
+    PM_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, f->stream));
+    PM_HIP(hipStreamSynchronize(f->stream));
+    *iters = (int)h[0];
+    f->stats.n_pcg_iterations += *iters;
+    if (!std::isfinite(h[1]) || !std::isfinite(h[2])) { f->err = "Found NaN or Inf in mdu"; return 1; }
+    if (h[2] == 0.0 || h[1] <= rtol2 * h[2]) return 0;
+    f->err = "PCG did not converge";
+    return 2;
+  }
+  // grids: rows for the SpMV (rbq), nodes for the vector updates (rb); partial arrays sized accordingly
+  const int rb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + PCG_TPB - 1) / PCG_TPB));
+  const int rbq = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + PCG_TPB - 1) / PCG_TPB));
+  double* pa = f->d_red;            // partials: pa (rbq), pb (rb), pc (rb)
+  double* pb = pa + rbq;
+  double* pc = pb + rb;
+  double* rzs = f->d_scal + 16;      // rz double buffer [16], [17]; rr at [18]; bb at [19]
+  double* rr_s = f->d_scal + 18;
+  double* bb_s = f->d_scal + 19;
+  if (f->bs == 2) hipLaunchKernelGGL((k_pcg_init<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, pa, pb);
+  else hipLaunchKernelGGL((k_pcg_init<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, pa, pb);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TPB), 0, f->stream, rb, pa, pb, rzs, bb_s);
+  double bb = 0.0;
+  PM_HIP(hipMemcpyAsync(&bb, bb_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
   if (bb == 0.0) return 0;
   if (!std::isfinite(bb)) { f->err = "Found NaN or Inf"; return 1; }
-  const double tol2 = f->cfg.pcg_rtol * f->cfg.pcg_rtol * bb;
-  const int check_every = 8;
+  const double tol2 = rtol2 * bb;
+  const int check_every = 16;
   int it = 0;
   double rr = bb;
   while (it < f->cfg.pcg_max_iter) {
-    dev_spmv(f, p, q, true);
-    reduce(f, 0, n, p, q, nullptr, S_PQ);
-    hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, f->stream, 0, f->d_scal);
-    if (f->bs == 2) hipLaunchKernelGGL((k_pcg_xr<2>), dim3(gn), dim3(TPB), 0, f->stream, np, f->d_scal, p, q, f->d_free, f->d_binv, d_x, r, z, part_a, part_b);
-    else hipLaunchKernelGGL((k_pcg_xr<3>), dim3(gn), dim3(TPB), 0, f->stream, np, f->d_scal, p, q, f->d_free, f->d_binv, d_x, r, z, part_a, part_b);
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(TPB), 0, f->stream, 0, (int)gn, part_a, f->d_scal, (int)S_RZNEW);
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(TPB), 0, f->stream, 0, (int)gn, part_b, f->d_scal, (int)S_RR);
-    hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, f->stream, 1, f->d_scal);
-    hipLaunchKernelGGL(k_pcg_p, dim3(nblk(n)), dim3(TPB), 0, f->stream, n, f->d_scal, z, p);
+    const double* rz_old = rzs + (it & 1);
+    double* rz_new = rzs + ((it + 1) & 1);
+    if (f->bs == 2) {
+      if (lowp) hipLaunchKernelGGL((k_pcg_q<2, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, p, q, pa);
+      else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, p, q, pa);
+      hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
+    } else {
+      if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, p, q, pa);
+      else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, p, q, pa);
+      hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
+    }
+    hipLaunchKernelGGL(k_pcg_p2, dim3(rb), dim3(PCG_TPB), 0, f->stream, n, rb, pb, pc, rz_old, rz_new, rr_s, z, p);
     it++;
     if (it % check_every == 0 || it == f->cfg.pcg_max_iter) {
-      rr = read_scal(f, S_RR);
+      PM_HIP(hipMemcpyAsync(&rr, rr_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
+      PM_HIP(hipStreamSynchronize(f->stream));
       if (!std::isfinite(rr)) { f->err = "Found NaN or Inf in mdu"; *iters = it; return 1; }
       if (rr <= tol2) break;
     }

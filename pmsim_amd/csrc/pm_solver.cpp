@@ -16,6 +16,8 @@
 
 namespace pm {
 
+constexpr int64_t kHostDirectMaxDof = 300000;  // host envelope Cholesky is the fallback below this size
+
 int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
   for (auto& g : f->groups) launch_update(f, g, update);
   if (residual) launch_residual_gather(f);
@@ -63,10 +65,16 @@ int copy_ip_state_to_device(pm_fem* f, const double* v) {
 // x = K_ff^-1 b_f (prescribed entries of x are zero); PCG with host direct fallback
 int linear_solve_free(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   *iters = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  struct Tm {
+    pm_fem* f; std::chrono::steady_clock::time_point t0;
+    ~Tm() { f->stats.ms_linear += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+  } tm{f, t0};
   if (f->cfg.lin_sol == 0) {
     int rc = pcg_solve(f, d_b, d_x, iters);
     if (rc == 0) return 0;
     if (rc == 1) return 1;  // NaN/Inf: no fallback
+    if (f->ndof > kHostDirectMaxDof) return 1;  // too large for the host envelope solver: report the PCG failure
     f->stats.n_host_fallbacks++;
   }
   std::vector<double> b(f->ndof), x;
@@ -109,6 +117,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
   const pm_config& cfg = f->cfg;
   *n_converged = 0;
   f->stats.n_steps = 0; f->stats.n_iterations = 0; f->stats.n_pcg_iterations = 0; f->stats.n_host_fallbacks = 0;
+  f->stats.ms_linear = 0.0;
   f->hist_u.clear(); f->hist_lam.clear(); f->hist_ip.clear(); f->hist_t.clear();
   if (!cfg.lagrange_mult_method)
     for (double v : f->ess_table) if (v != 0.0) { f->err = "the Lagrange multiplier method is required for non-zero prescribed values"; return 1; }

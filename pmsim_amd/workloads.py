@@ -1,0 +1,37 @@
+"""Benchmark workloads (mesh + material + boundary conditions + increments)."""
+from .generators import hex20_block, qua8_block, mixed_2d
+from .mesh import At, Features
+from .problem import Config, Dof, Elem, Essential, FemBase, Natural, ParamSolid, StressStrain
+
+
+def hex20_cube(nx=64, ny=64, nz=32, n_inc=10, uz=-0.01, seed=20261203, jitter=0.05):
+    """Unit cube of nx x ny x nz Hex20 (27 IP), corner jitter ±5% of h (seeded), von Mises E=2e5 nu=0.3
+    z0=250 H=1e3; bottom face fixed, top face uz ramping linearly to `uz` in n_inc increments."""
+    mesh = hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=seed, jitter=jitter)
+    feat = Features(mesh)
+    bottom = feat.search_faces(At.Z(0.0))
+    top = feat.search_faces(At.Z(1.0))
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27)))])
+    ess = Essential()
+    ess.faces(bottom, Dof.Ux, 0.0).faces(bottom, Dof.Uy, 0.0).faces(bottom, Dof.Uz, 0.0)
+    ess.faces_fn(top, Dof.Uz, uz, lambda t: t / n_inc)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
+
+
+def strip_footing(nx=16, ny=8, ngauss=9, n_inc=20, young=1e4, poisson=0.3, z_ini=200.0, width=1.0, uy=-0.05,
+                  lx=10.0, ly=5.0):
+    """Plane-strain strip footing: lx x ly domain of nx x ny Qua8, von Mises; bottom fixed, lateral ux = 0,
+    uy ramping to `uy` over the top edges within x <= width, in n_inc increments."""
+    mesh = qua8_block(nx, ny, lx, ly)
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0)); right = feat.search_edges(At.X(lx)); bottom = feat.search_edges(At.Y(0.0))
+    foot = feat.search_edges(At.Y(ly), lambda x: x[0] <= width + 1e-12)
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.VonMises(young, poisson, 0.0, z_ini), ngauss)))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(right, Dof.Ux, 0.0).edges(bottom, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    ess.edges_fn(foot, Dof.Uy, uy, lambda t: t / n_inc)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()

@@ -184,6 +184,28 @@ def test_pcg_matches_host_direct():
     assert its > 0 and rel_err(x_pcg, x_dir) < 1e-8
 
 
+@pytest.mark.parametrize("nx,ny", [(16, 8), (96, 48)])
+def test_pcg_paths_match_host_direct(nx, ny):
+    """Single-workgroup PCG (small systems) and the fused multi-workgroup PCG (larger ones) against the
+    host envelope Cholesky on the footing tangent after one plastic update."""
+    from pmsim_amd.workloads import strip_footing
+    mesh, base, cfg, ess, nat = strip_footing(nx, ny)
+    d = describe(mesh, base, cfg, ess, nat)
+    sols = []
+    for lin in ("pcg", "host"):
+        c = Config(mesh).set_lin_sol(lin).set_pcg(1e-12, 50000)
+        f = Fem(d, c)
+        rng = np.random.default_rng(4)
+        f.set_ddu(rng.uniform(-2e-3, 2e-3, f.ndof))
+        f.update_secondary_values()
+        f.assemble_kk()
+        b = rng.normal(size=f.ndof)
+        x, its = f.linear_solve(b)
+        sols.append((x, its))
+    assert sols[0][1] > 0
+    assert rel_err(sols[0][0], sols[1][0]) < 1e-8
+
+
 def test_hex20_full_size_properties():
     """64x64x32 Hex20 (131,072 elements, 3,538,944 IPs): for an elastic increment from zero stress,
     f_int = K ddu (linearity), K is symmetric, and two passes are bitwise identical."""
