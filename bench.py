@@ -26,12 +26,12 @@ FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
 
 
-def build_problem(nx, ny, nz, precision="f64"):
+def build_problem(nx, ny, nz, precision="f64", kernel="default"):
     from pmsim_amd import Config, Essential, FemBase, Natural, ParamSolid, StressStrain, describe
     from pmsim_amd import generators as G
     mesh = G.hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=20261203, jitter=0.05)
     base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
-    cfg = Config(mesh).set_tangent_precision(precision)
+    cfg = Config(mesh).set_tangent_precision(precision).set_tangent_kernel(kernel)
     return mesh, base, cfg, describe(mesh, base, cfg, Essential(), Natural())
 
 
@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -78,7 +79,7 @@ def main():
         sys.exit(2)
     from pmsim_amd.solver import Fem
     t0 = time.time()
-    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision)
+    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel)
     f = Fem(d, cfg, keep_history=False)
     setup_s = time.time() - t0
     print(f"setup: {mesh.ncell} cells, {f.nip} IPs, {f.ndof} DOFs, {f.nnzb} blocks, {setup_s:.1f} s", file=sys.stderr, flush=True)
@@ -106,7 +107,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "f64" else "f32-tangent/f64",
+        "dtype": "f64" if a.precision == "f64" else f"{a.precision}-tangent/f64",
         "data": "synthetic (structured jittered Hex20 cube, seeded increment)",
         "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, one element pass",
                    "elements": int(mesh.ncell), "ips": int(nip), "dofs": int(f.ndof), "plastic_fraction": plastic,

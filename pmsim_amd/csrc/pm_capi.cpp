@@ -38,6 +38,7 @@ void pm_default_config(pm_config* c) {
   c->pcg_max_iter = 20000;
   c->keep_history = 0;
   c->tol_rr_scaled = 0.0;
+  c->tangent_kernel = 0;
 }
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
@@ -84,7 +85,10 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   if (c->dt <= 0.0) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
+  if (c->tangent_kernel < 0 || c->tangent_kernel > 3) { f->err = "unknown tangent kernel"; return 1; }
+  const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   f->cfg = *c;
+  if (retune) return tune_tangent(f);
   return 0;
 }
 
@@ -124,7 +128,7 @@ int pm_initialize(pm_fem* f, int32_t device) {
   assemble_f_ext(f);
   if (device_alloc(f)) { device_free(f); return 1; }
   f->initialized = true;
-  return 0;
+  return tune_tangent(f);
 }
 
 int pm_initialize_internal_values(pm_fem* f) {

@@ -58,6 +58,7 @@ struct Group {
   int64_t ke_base = 0;             // offset (in blocks) of this group's K_e blocks in the block buffer
   int nblk = 0;                    // nnode*nnode stored node blocks per element (both triangles, rank order)
   int tune_epb = 0, tune_nqg = 0;  // tangent work split chosen by the tuner (0: heuristic)
+  int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
 
 struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; double value; };
@@ -164,6 +165,10 @@ void device_free(pm_fem* f);
 void launch_update(pm_fem* f, Group& g, bool do_update);
 void launch_residual_gather(pm_fem* f);
 void launch_tangent(pm_fem* f, Group& g);
+// matrix-core tangent (prec: 0 FP64, 1 FP32, 2 BF16 with FP32 accumulation); false if the kind has none
+bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
+// choose the tangent kernel (and work split) of every group, timing the candidates when asked to
+int tune_tangent(pm_fem* f);
 void launch_scatter(pm_fem* f);
 void launch_init_state(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);

@@ -656,8 +656,17 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
                                       f->d_fe + g.fe_base, mp, f->axisym ? 1 : 0, f->thickness, do_update ? 1 : 0, f->d_err));
 }
 
+// tangent kernel of a group: the tuned choice, else the default (matrix cores for reduced precision)
+static int tangent_kernel_of(const pm_fem* f, const Group& g) {
+  const int prec = f->cfg.tangent_precision;
+  if (prec == 2) return 2;  // BF16 runs on the matrix cores only
+  if (g.tune_kernel) return g.tune_kernel;
+  return prec == 0 ? 1 : 2;
+}
+
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
+  if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
   int epb, nqg;
   split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, nqg);
   if (g.tune_epb > 0) { epb = g.tune_epb; nqg = g.tune_nqg; }
@@ -675,6 +684,54 @@ void launch_tangent(pm_fem* f, Group& g) {
                                         epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * (size_t)(g.ndim * g.ndim), mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   }
+}
+
+// Tangent kernel choice per group.  Modes 1 and 2 force a kernel; mode 3 times every candidate (the
+// matrix-core kernel and the vector kernel under each feasible work split) on the current state and keeps
+// the fastest; mode 0 keeps the default.
+int tune_tangent(pm_fem* f) {
+  const int mode = f->cfg.tangent_kernel;
+  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_epb = g.tune_nqg = 0; }
+  if (mode != 3) return 0;
+  struct Cand { int kern, epb, nqg; };
+  hipEvent_t ev0, ev1;
+  PM_HIP(hipEventCreate(&ev0));
+  PM_HIP(hipEventCreate(&ev1));
+  int rc = 0;
+  for (auto& g : f->groups) {
+    if (g.nel == 0) continue;
+    std::vector<Cand> cands;
+    cands.push_back({2, 0, 0});
+    if (f->cfg.tangent_precision != 2) {
+      const int nh = (g.nnode + 1) / 2, nt = nh * (nh + 1) / 2;
+      for (int nqg = 1; nqg <= std::min(g.nip, TPB / nt); nqg++) {
+        int emax = std::max(1, TPB / (nt * nqg));
+        for (int epb = emax; epb >= 1; epb = (epb > 1 ? epb / 2 : 0)) {
+          if (shm_tangent(g.ndim, g.nnode, g.nip, epb, nqg, f->axisym) > SHM_CAP) continue;
+          cands.push_back({1, epb, nqg});
+          break;  // the largest feasible epb for this nqg
+        }
+      }
+    }
+    float best = 1e30f;
+    Cand pick = cands[0];
+    for (const Cand& c : cands) {
+      g.tune_kernel = c.kern; g.tune_epb = c.epb; g.tune_nqg = c.nqg;
+      launch_tangent(f, g);  // warm-up
+      if (hipEventRecord(ev0, f->stream) != hipSuccess) { rc = 1; break; }
+      for (int r = 0; r < 3; r++) launch_tangent(f, g);
+      if (hipEventRecord(ev1, f->stream) != hipSuccess || hipEventSynchronize(ev1) != hipSuccess) { rc = 1; break; }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev0, ev1);
+      if (ms < best) { best = ms; pick = c; }
+    }
+    g.tune_kernel = pick.kern; g.tune_epb = pick.epb; g.tune_nqg = pick.nqg;
+    if (rc) break;
+  }
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (rc) { f->err = "tangent tuning failed: " + std::string(hipGetErrorString(hipGetLastError())); return 1; }
+  return check_device_error(f);
 }
 
 void launch_scatter(pm_fem* f) {

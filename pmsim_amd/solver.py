@@ -36,7 +36,7 @@ class PmConfig(C.Structure):
         ("t_ini", C.c_double), ("t_fin", C.c_double), ("dt", C.c_double), ("dt_out", C.c_double),
         ("tol_rr_abs", C.c_double), ("tol_mdu_rel", C.c_double), ("symmetry_check_tolerance", C.c_double),
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
-        ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double),
+        ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
     ]
 
 
@@ -102,6 +102,7 @@ class PmError(RuntimeError):
 
 
 PRECISION = {"f64": 0, "f32": 1, "bf16": 2}
+TANGENT_KERNEL = {"default": 0, "vector": 1, "mfma": 2, "tune": 3}
 
 
 def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmConfig:
@@ -123,6 +124,7 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
         c.pcg_rtol = cfg.pcg_tol
         c.pcg_max_iter = cfg.pcg_max_iter
         c.tol_rr_scaled = cfg.scaled_rr
+        c.tangent_kernel = TANGENT_KERNEL[cfg.tangent_kernel]
     return c
 
 
