@@ -30,6 +30,11 @@ static int upload(pm_fem* f, T* d, const T* h, int64_t n) {
 int device_alloc(pm_fem* f) {
   PM_HIP(hipSetDevice(f->device));
   PM_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, f->device) == hipSuccess && v > 0) f->n_cu = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, f->device) == hipSuccess && v > 0) f->lds_per_cu = v;
+  }
   const int nd = f->ndim, ns = f->nsig;
   const int64_t b2 = (int64_t)nd * nd;
   if (dalloc(f, &f->d_X, f->npoint * nd) || upload(f, f->d_X, f->X.data(), f->npoint * nd)) return 1;

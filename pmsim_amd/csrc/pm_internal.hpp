@@ -105,6 +105,8 @@ struct pm_fem {
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
   // device
   int device = 0;
+  int n_cu = 256;                      // compute units of the device
+  int64_t lds_per_cu = 160 * 1024;     // LDS bytes per compute unit
   hipStream_t stream = nullptr;
   bool initialized = false;
   double* d_X = nullptr;               // point coordinates

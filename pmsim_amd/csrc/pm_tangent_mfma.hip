@@ -4,12 +4,17 @@
 //   K_ab = M_ab^T + tr(M_ab) 1 + C_ab,
 //   M = sum_q (c alpha/2)_q  g_q g_q^T                (g_q[(a,i)] = G_q[a][i], the gradient vector)
 //   C = sum_q (c beta)_q tI_q tI_q^T + (c gamma)_q ts_q ts_q^T (+ (c alpha)_q h_q h_q^T, axisymmetric)
-// with tI = B^T I, ts = B^T s and h[(a,0)] = N_a / r.  Both are Gram products over rows built per IP,
-// i.e. batched small GEMMs (NN*ND x NN*ND, k = IPs x rows) that run as 16x16 MFMA tiles; only the
-// upper tiles are computed.  One wave owns one element; IPs are processed in chunks whose rows are
-// staged in LDS (column-major so each lane's operand slice is contiguous).  The epilogue reads the
-// upper tiles of M back from LDS, forms K and writes both triangles in rank order (nodes sorted by
-// global id), matching the layout consumed by the deterministic scatter.
+// with tI = B^T I (= g unless axisymmetric), ts = B^T s and h[(a,0)] = N_a / r.  Both are Gram products
+// over rows built per IP, i.e. small GEMMs (NN*ND x NN*ND, k = IPs) that run as 16x16 MFMA tiles; only
+// the upper tiles are computed.  The rows are laid out with the nodes in rank order (sorted by global
+// id), so the product comes out directly in the order the deterministic scatter reads.
+//
+// Organisation (persistent): each workgroup owns a contiguous range of units of EPW elements and keeps
+// the shape-function table in LDS.  While one unit is processed, the coordinates, state and ranks of
+// the next unit are already in flight in registers.  Per element: one lane per IP forms J, its inverse
+// and the D coefficients; all threads of the element stage the gradient rows; WPE waves share the
+// upper tiles (tile t on wave t % WPE); the epilogue parks M in LDS, forms K, stages the node blocks
+// in LDS and writes the element matrix contiguously.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,9 +28,17 @@ namespace {
 constexpr double ISQ2 = 0.70710678118654752440;
 enum { ERR_SIGD = 1, ERR_NORMS = 2, ERR_DETJ = 4 };
 
+// material constants, with the products the D coefficients need formed on the host (they then stay in
+// scalar registers instead of occupying vector registers across the persistent loop)
 struct MatM {
   int model, plane_stress;
-  double K, G, H, z0, E, nu;
+  double G3;      // 3 G
+  double G2;      // 2 G
+  double G6G;     // 6 G^2
+  double inv_d;   // 1 / (3 G + H)
+  double K;       // bulk modulus
+  double beta_e;  // K - 2G/3 (elastic)
+  double a_ps, b_ps;  // plane stress: E/(1+nu), E nu/(1-nu^2)
 };
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -39,363 +52,523 @@ __device__ __forceinline__ short to_bf16(float x) {  // round to nearest even
 }
 
 template <int PREC> struct MfmaT;
-template <> struct MfmaT<0> {  // FP64 16x16x4
-  typedef double T; typedef d4_t Acc; typedef double Out;
+template <> struct MfmaT<0> {  // FP64 16x16x4: output register r of lane l holds D[4r + l/16][l%16]
+  typedef double T; typedef d4_t Acc;
   static constexpr int KS = 4;
-  // output register r of lane l holds D[4r + l/16][l%16]
   __device__ static int row(int l, int r) { return 4 * r + (l >> 4); }
 };
-template <> struct MfmaT<1> {  // FP32 16x16x4
-  typedef float T; typedef f4_t Acc; typedef float Out;
+template <> struct MfmaT<1> {  // FP32 16x16x4: output register r of lane l holds D[4(l/16) + r][l%16]
+  typedef float T; typedef f4_t Acc;
   static constexpr int KS = 4;
   __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
-template <> struct MfmaT<2> {  // BF16 16x16x16, FP32 accumulation
-  typedef float T; typedef f4_t Acc; typedef float Out;
+template <> struct MfmaT<2> {  // BF16 16x16x16 with FP32 accumulation, same output layout as FP32
+  typedef float T; typedef f4_t Acc;
   static constexpr int KS = 16;
   __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-template <int ND, int NN, int PREC>
-struct MCfg {
+template <int ND, int NN>
+struct MShape {
+  static constexpr int NS = ND == 2 ? 4 : 6;
   static constexpr int NC = NN * ND;
   static constexpr int NTI = (NC + 15) / 16;
   static constexpr int NCP = NTI * 16;
   static constexpr int NTILE = NTI * (NTI + 1) / 2;
-  static constexpr int KS = MfmaT<PREC>::KS;
-  static constexpr int IPC = KS;             // IPs per chunk
-  static constexpr int QW = 3 + ND * ND + 6 + 1;
+  static constexpr int WPE = NTILE == 1 ? 1 : (NTILE <= 4 ? NTILE : 4);  // waves per element
+  static constexpr int EPW = WPE == 1 ? 4 : 1;                           // elements per workgroup (a unit)
+  static constexpr int TPW = (NTILE + WPE - 1) / WPE;                     // tiles per wave
+  static constexpr int TPE = 64 * WPE;                                    // threads per element
+  static constexpr int NTHR = TPE * EPW;
+  static constexpr int QW = 3 + ND * ND + 6 + 1;  // c alpha, c beta, c gamma, J^-1, s, r
+  static constexpr int SV = NS + 2;               // state values read per IP: sigma, lambda, elastic flag (z skipped)
+  static constexpr int MINW = ND == 3 ? 3 : 4;      // waves per SIMD the register budget is sized for
 };
 
-// LDS layout (per wave = per workgroup): [X (NN*ND) | per-IP data (nip*QW) | union(rows, M tiles)]
+// LDS, in doubles: [table: w | dN (stride DSN per IP) | N (axisymmetric)] then EPW element slots of
+// [X | state | per-IP data | union(rows + weights of a chunk, M tiles, node-block staging)]
+struct MLayout {
+  int rsk, nrow, dsn;            // row-slot stride per column (one row kind), row kinds, dN stride per IP
+  size_t tdn, tn, tab;           // table offsets and size
+  size_t s, q, uni, per_el;      // element-slot offsets and size
+  size_t total;
+};
+
 template <int ND, int NN, int PREC>
-__host__ __device__ constexpr int mfma_rows_stride() {
-  // rows per chunk: IPC (M) + 3*IPC (C: tI, ts, h); column-major with an odd stride
-  return 4 * MCfg<ND, NN, PREC>::IPC + 1;
+__host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
+  using S = MShape<ND, NN>;
+  using T = typename MfmaT<PREC>::T;
+  MLayout L;
+  L.nrow = axisym ? 4 : 2;  // g, ts  |  g, tI, ts, h
+  L.rsk = ipc + 1;          // ipc is a multiple of 4: odd stride
+  L.dsn = NN * ND + 1;
+  L.tdn = (size_t)nip;
+  L.tn = L.tdn + (size_t)nip * L.dsn;
+  L.tab = L.tn + (axisym ? (size_t)nip * NN : 0);
+  L.tab = (L.tab + 1) & ~(size_t)1;  // element slots and their union start 16-byte aligned
+  L.s = (size_t)NN * ND;
+  L.q = L.s + (size_t)nip * S::SV;
+  L.uni = (L.q + (size_t)nip * S::QW + 1) & ~(size_t)1;
+  const size_t rows = ((size_t)L.nrow * S::NCP * L.rsk + 4 * (size_t)ipc) * sizeof(T);
+  const size_t mt = (size_t)S::NCP * (S::NCP + 1) * sizeof(T);  // dense M, both triangles
+  const size_t kst = (size_t)NN * NN * ND * ND * sizeof(T);
+  size_t u = rows > mt ? rows : mt;
+  u = u > kst ? u : kst;
+  L.per_el = (L.uni + (u + sizeof(double) - 1) / sizeof(double) + 1) & ~(size_t)1;
+  L.total = L.tab + (size_t)S::EPW * L.per_el;
+  return L;
 }
 
 template <int ND, int NN, int PREC>
-__global__ __launch_bounds__(64) void k_tangent_mfma(int64_t nel, int nip, const int32_t* __restrict__ conn,
-                                                     const uint8_t* __restrict__ rank, const double* __restrict__ X,
-                                                     const double* __restrict__ tab, const double* __restrict__ st,
-                                                     typename MfmaT<PREC>::Out* __restrict__ ke, MatM mp, int axisym,
-                                                     double thickness, int32_t* err) {
-  using C = MCfg<ND, NN, PREC>;
+__global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) void k_tangent_mfma(
+    int64_t nel, int nip, int ipc, int64_t units_per_wg, const int32_t* __restrict__ conn, const uint8_t* __restrict__ rank,
+    const double* __restrict__ X, const double* __restrict__ tab, const double* __restrict__ st,
+    typename MfmaT<PREC>::T* __restrict__ ke, MatM mp, int axisym, double thickness, int32_t* err) {
+  using S = MShape<ND, NN>;
   using MT = MfmaT<PREC>;
   using T = typename MT::T;
   using Acc = typename MT::Acc;
-  constexpr int NS = ND == 2 ? 4 : 6;
-  constexpr int NC = C::NC, NTI = C::NTI, NCP = C::NCP, NTILE = C::NTILE, KS = C::KS, IPC = C::IPC, QW = C::QW;
-  constexpr int RS = mfma_rows_stride<ND, NN, PREC>();  // row-slot stride per column
-  constexpr int MS = 17;                                 // M tile row stride
+  constexpr int NS = S::NS, NC = S::NC, NTI = S::NTI, NCP = S::NCP, NTILE = S::NTILE, WPE = S::WPE, EPW = S::EPW;
+  constexpr int TPW = S::TPW, TPE = S::TPE, NTHR = S::NTHR, QW = S::QW, SV = S::SV;
+  constexpr int KS = MT::KS, B2 = ND * ND, EW = NN * NN * B2;
   extern __shared__ double shm[];
-  double* sX = shm;
-  double* sQ = sX + NN * ND;
-  double* uni = sQ + nip * QW;
-  T* sL = reinterpret_cast<T*>(uni);          // rows: sL[col * RS + k]
-  T* sW = sL + NCP * RS;                      // row weights
-  typename MT::Out* sM = reinterpret_cast<typename MT::Out*>(uni);  // epilogue: NTILE upper tiles of M, 16 x MS each
-  const int l = threadIdx.x;
-  const int64_t e = blockIdx.x;
-  if (e >= nel) return;
-  const double* __restrict__ tw = tab;
-  const double* __restrict__ tN = tab + nip;
-  const double* __restrict__ tdN = tN + nip * NN;
-  for (int i = l; i < NN; i += 64) {
-    const int64_t p = conn[e * NN + i];
-#pragma unroll
-    for (int d = 0; d < ND; d++) sX[i * ND + d] = X[p * ND + d];
-  }
-  __syncthreads();
-  // J entries of every IP
-  for (int w = l; w < nip * ND * ND; w += 64) {
-    const int q = w / (ND * ND), ij = w - q * ND * ND, i = ij / ND, j = ij - i * ND;
-    const double* dN = tdN + q * NN * ND;
-    double s = 0.0;
-#pragma unroll 4
-    for (int m = 0; m < NN; m++) s += sX[m * ND + i] * dN[m * ND + j];
-    sQ[q * QW + 3 + ij] = s;
-  }
-  __syncthreads();
+  __shared__ uint8_t srank[EPW * NN];
+  const MLayout L = mfma_layout<ND, NN, PREC>(nip, ipc, axisym);
+  const int RSK = L.rsk, NR = L.nrow, DSN = L.dsn;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  (void)wave;
+  const int64_t nunits = (nel + EPW - 1) / EPW;
+  const int64_t u0 = (int64_t)blockIdx.x * units_per_wg;
+  const int64_t u1 = min(nunits, u0 + units_per_wg);
+  if (u0 >= u1) return;  // uniform over the workgroup
   const int64_t nipg = nel * nip;
-  for (int q = l; q < nip; q += 64) {
-    double* Q = sQ + q * QW;
-    double J[ND][ND], Ji[ND][ND], det;
-#pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) J[i][j] = Q[3 + i * ND + j];
-    if constexpr (ND == 2) {
-      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      const double id = 1.0 / det;
-      Ji[0][0] = J[1][1] * id; Ji[0][1] = -J[0][1] * id; Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
-    } else {
-      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-      const double id = 1.0 / det;
-      Ji[0][0] = c00 * id; Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id; Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-      Ji[1][0] = c01 * id; Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id; Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-      Ji[2][0] = c02 * id; Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id; Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    }
-    if (!(det > 0.0)) *err = ERR_DETJ;
-    double r = 0.0;
-    if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * sX[m * ND];
-    const double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
-    const int64_t ip = e * nip + q;
-    double alpha, beta, gamma = 0.0;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    if (mp.plane_stress) {
-      alpha = mp.E / (1.0 + mp.nu);
-      beta = mp.E * mp.nu / (1.0 - mp.nu * mp.nu);
-    } else if (mp.model == 0 || st[(NS + 2) * nipg + ip] != 0.0) {
-      alpha = 2.0 * mp.G;
-      beta = mp.K - alpha / 3.0;
-    } else {
-      double sig[NS];
-#pragma unroll
-      for (int c = 0; c < NS; c++) sig[c] = st[c * nipg + ip];
-      const double lam = st[(NS + 1) * nipg + ip];
-      const double sm = (sig[0] + sig[1] + sig[2]) / 3.0;
-      double ss = 0.0;
-#pragma unroll
-      for (int c = 0; c < NS; c++) { s[c] = sig[c] - (c < 3 ? sm : 0.0); ss += s[c] * s[c]; }
-      const double sd = sqrt(1.5 * ss);
-      const double sdtr = sd + lam * 3.0 * mp.G;
-      const double norm_s = sd * 0.81649658092772603273;
-      if (norm_s < 1e-10) *err = ERR_NORMS;
-      const double d = 3.0 * mp.G + mp.H;
-      alpha = 2.0 * mp.G * (1.0 - lam * 3.0 * mp.G / sdtr);
-      gamma = 6.0 * mp.G * mp.G * (lam / sdtr - 1.0 / d) / (norm_s * norm_s);
-      beta = mp.K - alpha / 3.0;
-    }
-    Q[0] = coef * alpha;
-    Q[1] = coef * beta;
-    Q[2] = coef * gamma;
-#pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) Q[3 + i * ND + j] = Ji[i][j];
-#pragma unroll
-    for (int c = 0; c < 6; c++) Q[3 + ND * ND + c] = s[c];
-    Q[3 + ND * ND + 6] = r;
+
+  double* sw = shm;
+  double* sdN = shm + L.tdn;
+  double* sN = shm + L.tn;
+  for (int i = tid; i < nip; i += NTHR) sw[i] = tab[i];
+  for (int i = tid; i < nip * NN * ND; i += NTHR) {
+    const int q = i / (NN * ND);
+    sdN[q * DSN + (i - q * NN * ND)] = tab[nip + nip * NN + i];
   }
-  __syncthreads();
-  Acc accM[NTILE], accC[NTILE];
+  if (axisym) for (int i = tid; i < nip * NN; i += NTHR) sN[i] = tab[nip + i];
+
+  // prefetch roles (fixed per thread): one coordinate (xe, xm, xd) and one state value (se, component, q)
+  const bool xrole = tid < EPW * NN * ND;
+  const int xe = tid / (NN * ND), xm = (tid - xe * NN * ND) / ND, xd = tid - xe * NN * ND - xm * ND;
+  const int xoff = xe * NN + xm;
+  const bool srole = tid < EPW * SV * nip;
+  const int se = tid / (SV * nip), sc = (tid - se * SV * nip) / nip;
+  const int64_t soff = (int64_t)(sc < NS ? sc : sc + 1) * nipg + (tid - se * SV * nip - sc * nip) + (int64_t)se * nip;  // skips z
+  const int sl = se * (int)(L.per_el) + (int)L.s + (tid - se * SV * nip);  // LDS target of the state value
+  int rc = 0;
+  double rx = 0.0, rs = 0.0;
+  uint8_t rr = 0;
+  // The prefetch loads are issued unconditionally (surplus lanes re-read a valid entry) so the number of
+  // memory operations after each is static and waiting for a prefetch never waits for earlier stores.
+  // Offsets pass through an empty asm so that no 64-bit address is kept live, and spilled, across units.
+  auto load_conn = [&](int64_t u) {
+    const int n = (int)min((int64_t)EPW, nel - u * EPW);
+    int xo = (xrole && xe < n) ? xoff : 0;
+    asm volatile("" : "+v"(xo));
+    rc = conn[u * (EPW * NN) + xo];
+  };
+  auto load_rest = [&](int64_t u) {
+    const int n = (int)min((int64_t)EPW, nel - u * EPW);
+    int xo = (xrole && xe < n) ? xoff : 0;
+    int64_t so = (srole && se < n) ? soff : 0;
+    asm volatile("" : "+v"(xo), "+v"(so));
+    rx = X[(int64_t)rc * ND + xd];
+    rr = rank[u * (EPW * NN) + xo];
+    rs = st[so + u * (EPW * (int64_t)nip)];
+  };
+  load_conn(u0);
+  load_rest(u0);
+  load_conn(min(u0 + 1, u1 - 1));
+
+  for (int64_t u = u0; u < u1; u++) {
+    const int nloc = (int)min((int64_t)EPW, nel - u * EPW);
+    __syncthreads();  // previous unit finished with LDS (and the table is in place)
+    if (xrole && xe < nloc) {
+      double* dX = shm + L.tab + xe * L.per_el;
+      dX[xm * ND + xd] = rx;
+      if (xd == 0) srank[xe * NN + xm] = rr;
+    }
+    if (srole && se < nloc) shm[L.tab + sl] = rs;
+    load_rest(min(u + 1, u1 - 1));  // past the range: a harmless repeat
+    load_conn(min(u + 2, u1 - 1));
+    __syncthreads();
+    // Per-thread indices are re-derived each unit from an opaque copy of the thread id, so the compiler
+    // does not hoist every loop-invariant LDS address out of the unit loop (that would exhaust registers).
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const int l = tq & 63, el = (tq >> 6) / WPE, wv = (tq >> 6) - el * WPE, te = tq - el * TPE;
+    const bool active = EPW == 1 || el < nloc;  // a unit of one element is always full
+    double* slot = shm + L.tab + el * L.per_el;  // this thread's element slot
+    double* sX = slot;
+    double* sS = slot + L.s;  // state: [SV][nip]
+    double* sQ = slot + L.q;
+    T* sL = reinterpret_cast<T*>(slot + L.uni);  // row kind k: sL[(k * NCP + col) * RSK + slot]; weights after
+    int tIv[TPW], tJv[TPW];
 #pragma unroll
-  for (int t = 0; t < NTILE; t++) { accM[t] = Acc{0, 0, 0, 0}; accC[t] = Acc{0, 0, 0, 0}; }
-  const int nchunk = (nip + IPC - 1) / IPC;
-  const int ncr = axisym ? 3 : 2;  // C row kinds: tI, ts (, h)
-  for (int ch = 0; ch < nchunk; ch++) {
-    // stage the rows of this chunk: k = ql (M), IPC + t*IPC + ql (C, t = 0 tI, 1 ts, 2 h)
-    for (int w = l; w < IPC * (NCP / ND + 1); w += 64) {
-      const int ql = w / (NCP / ND + 1), m = w - ql * (NCP / ND + 1);
-      if (m * ND >= NCP) continue;
-      const int q = ch * IPC + ql;
-      if (q < nip && m < NN) {
-        const double* Q = sQ + q * QW;
-        const double* dN = tdN + (q * NN + m) * ND;
-        double G[ND];
+    for (int s = 0; s < TPW; s++) {
+      int t = wv + WPE * s, I = 0;
+      if (t >= NTILE) t = NTILE - 1;  // idle slot: computed, never stored
+      while (t >= NTI - I) { t -= NTI - I; I++; }
+      tIv[s] = I; tJv[s] = I + t;
+    }
+
+    // one lane per IP: J, J^-1, c times the D coefficients (von_mises.rs:124-167), deviator, radius
+    if (active && wv == 0 && l < nip) {
+      const int q = l;
+      double J[ND][ND];
 #pragma unroll
-        for (int j = 0; j < ND; j++) {
-          double sum = 0.0;
+      for (int i = 0; i < ND; i++)
 #pragma unroll
-          for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
-          G[j] = sum;
-        }
-        const double* sv = Q + 3 + ND * ND;
-        const double h = axisym ? tN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
-        double ts[ND];
-        if constexpr (ND == 2) {
-          const double sxy = sv[3] * ISQ2;
-          ts[0] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
-          ts[1] = sxy * G[0] + sv[1] * G[1];
-        } else {
-          const double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
-          ts[0] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
-          ts[1] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
-          ts[2] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
-        }
-        // rows: g (M), tI = G + h e_x, ts, h e_x
+        for (int j = 0; j < ND; j++) J[i][j] = 0.0;
+      const double* dN = sdN + q * DSN;
+#pragma unroll 2
+      for (int m = 0; m < NN; m++)
 #pragma unroll
-        for (int i = 0; i < ND; i++) {
-          const int col = m * ND + i;
-          sL[col * RS + ql] = (T)G[i];
-          sL[col * RS + IPC + ql] = (T)(G[i] + (i == 0 ? h : 0.0));
-          sL[col * RS + 2 * IPC + ql] = (T)ts[i];
-          sL[col * RS + 3 * IPC + ql] = (i == 0) ? (T)h : T(0);
-        }
+        for (int i = 0; i < ND; i++)
+#pragma unroll
+          for (int j = 0; j < ND; j++) J[i][j] += sX[m * ND + i] * dN[m * ND + j];
+      double Ji[ND][ND], det;
+      if constexpr (ND == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double id = 1.0 / det;
+        Ji[0][0] = J[1][1] * id; Ji[0][1] = -J[0][1] * id; Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
       } else {
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        Ji[0][0] = c00 * id; Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id; Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        Ji[1][0] = c01 * id; Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id; Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][0] = c02 * id; Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id; Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+      }
+      if (!(det > 0.0)) *err = ERR_DETJ;
+      double r = 0.0;
+      if (axisym) for (int m = 0; m < NN; m++) r += sN[q * NN + m] * sX[m * ND];
+      const double coef = det * sw[q] * thickness * (axisym ? r : 1.0);
+      double alpha, beta, gamma = 0.0;
+      double s[6] = {0, 0, 0, 0, 0, 0};
+      if (mp.plane_stress) {
+        alpha = mp.a_ps;
+        beta = mp.b_ps;
+      } else if (mp.model == 0 || sS[(NS + 1) * nip + q] != 0.0) {
+        alpha = mp.G2;
+        beta = mp.beta_e;
+      } else {
+        double sig[NS];
 #pragma unroll
-        for (int i = 0; i < ND; i++) {
-          const int col = m * ND + i;
-          if (col >= NCP) break;
-          sL[col * RS + ql] = T(0);
-          sL[col * RS + IPC + ql] = T(0);
-          sL[col * RS + 2 * IPC + ql] = T(0);
-          sL[col * RS + 3 * IPC + ql] = T(0);
+        for (int c = 0; c < NS; c++) sig[c] = sS[c * nip + q];
+        const double lam = sS[NS * nip + q];
+        const double sm = (sig[0] + sig[1] + sig[2]) / 3.0;
+        double ss = 0.0;
+#pragma unroll
+        for (int c = 0; c < NS; c++) { s[c] = sig[c] - (c < 3 ? sm : 0.0); ss += s[c] * s[c]; }
+        const double sd = sqrt(1.5 * ss);
+        const double sdtr = sd + lam * mp.G3;
+        const double norm_s = sd * 0.81649658092772603273;  // sqrt(2/3)
+        if (norm_s < 1e-10) *err = ERR_NORMS;
+        alpha = mp.G2 * (1.0 - lam * mp.G3 / sdtr);
+        gamma = mp.G6G * (lam / sdtr - mp.inv_d) / (norm_s * norm_s);
+        beta = mp.K - alpha / 3.0;
+      }
+      double* Q = sQ + q * QW;
+      Q[0] = coef * alpha;
+      Q[1] = coef * beta;
+      Q[2] = coef * gamma;
+#pragma unroll
+      for (int i = 0; i < ND; i++)
+#pragma unroll
+        for (int j = 0; j < ND; j++) Q[3 + i * ND + j] = Ji[i][j];
+#pragma unroll
+      for (int c = 0; c < 6; c++) Q[3 + ND * ND + c] = s[c];
+      Q[3 + ND * ND + 6] = r;
+    }
+
+    Acc accM[TPW], accC[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; s++) { accM[s] = Acc{0, 0, 0, 0}; accC[s] = Acc{0, 0, 0, 0}; }
+    const int KR = NCP * RSK;  // values per row kind
+    T* sW = sL + NR * KR;      // weights: 4 kinds x ipc
+    const int nchunk = (nip + ipc - 1) / ipc;
+    for (int ch = 0; ch < nchunk; ch++) {
+      __syncthreads();  // per-IP data ready / previous chunk consumed
+      const int nv = min(ipc, nip - ch * ipc);  // valid IPs of this chunk
+      if (active) {
+        // rows of the valid IPs, nodes in rank order: column ND * rank(m) + i
+        for (int w = te; w < nv * NN; w += TPE) {
+          const int ql = w / NN, m = w - ql * NN, q = ch * ipc + ql;
+          const double* Q = sQ + q * QW;
+          const double* dN = sdN + q * DSN + m * ND;
+          double G[ND];
+#pragma unroll
+          for (int j = 0; j < ND; j++) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
+            G[j] = sum;
+          }
+          const double* sv = Q + 3 + ND * ND;
+          const double h = axisym ? sN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
+          double ts[ND];
+          if constexpr (ND == 2) {
+            const double sxy = sv[3] * ISQ2;
+            ts[0] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
+            ts[1] = sxy * G[0] + sv[1] * G[1];
+          } else {
+            const double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
+            ts[0] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
+            ts[1] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
+            ts[2] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
+          }
+          T* c = sL + (ND * srank[el * NN + m]) * RSK + ql;
+#pragma unroll
+          for (int i = 0; i < ND; i++) {
+            c[i * RSK] = (T)G[i];
+            if (NR == 2) {
+              c[KR + i * RSK] = (T)ts[i];
+            } else {
+              c[KR + i * RSK] = (T)(G[i] + (i == 0 ? h : 0.0));
+              c[2 * KR + i * RSK] = (T)ts[i];
+              c[3 * KR + i * RSK] = i == 0 ? (T)h : T(0);
+            }
+          }
+        }
+        // zero padding: slots of IPs beyond nip (all columns), and columns [NC, NCP) of the valid slots
+        const int npad_q = (ipc - nv) * NCP, npad = npad_q + (NCP - NC) * nv;
+        for (int w = te; w < npad; w += TPE) {
+          int col, ql;
+          constexpr int PADC = NCP > NC ? NCP - NC : 1;
+          if (w < npad_q) { ql = nv + w / NCP; col = w - (ql - nv) * NCP; }
+          else { const int w2 = w - npad_q; ql = w2 / PADC; col = NC + (w2 - ql * PADC); }
+          for (int k = 0; k < NR; k++) sL[k * KR + col * RSK + ql] = T(0);
+        }
+        for (int ql = te; ql < ipc; ql += TPE) {
+          const bool ok = ql < nv;
+          const double* Q = sQ + (ok ? ch * ipc + ql : 0) * QW;
+          sW[ql] = ok ? (T)(0.5 * Q[0]) : T(0);        // g rows into M: c alpha / 2
+          sW[ipc + ql] = ok ? (T)Q[1] : T(0);          // tI rows: c beta
+          sW[2 * ipc + ql] = ok ? (T)Q[2] : T(0);      // ts rows: c gamma
+          sW[3 * ipc + ql] = ok ? (T)Q[0] : T(0);      // h rows: c alpha
         }
       }
-    }
-    for (int w = l; w < IPC; w += 64) {
-      const int q = ch * IPC + w;
-      const bool ok = q < nip;
-      const double* Q = sQ + (ok ? q : 0) * QW;
-      sW[w] = ok ? (T)(0.5 * Q[0]) : T(0);     // M rows: c alpha / 2
-      sW[IPC + w] = ok ? (T)Q[1] : T(0);       // tI rows: c beta
-      sW[2 * IPC + w] = ok ? (T)Q[2] : T(0);   // ts rows: c gamma
-      sW[3 * IPC + w] = ok ? (T)Q[0] : T(0);   // h rows: c alpha
-    }
-    __syncthreads();
-    // MFMA: M over rows [0, IPC), C over rows [IPC, IPC*(1+ncr))
-    const int col_in = l & 15;
-    for (int g = 0; g < 1 + ncr; g++) {
-      const int k0 = g * IPC;
-      if constexpr (PREC == 2) {
-        const int kq = k0 + 4 * (l >> 4);
-        s4_t av[NTI], bv[NTI];
+      __syncthreads();
+      if (active) {
+        const int col = l & 15;
+        const int tIk = NR == 2 ? 0 : KR;        // the tI rows (g itself unless axisymmetric)
+        const int tsk = NR == 2 ? KR : 2 * KR;   // the ts rows
+        if constexpr (PREC == 2) {
+#pragma unroll 1
+          for (int k0 = 0; k0 < ipc; k0 += KS) {
+            const int kq = k0 + 4 * (l >> 4);
 #pragma unroll
-        for (int I = 0; I < NTI; I++) {
-          const T* src = sL + (16 * I + col_in) * RS + kq;
+            for (int s = 0; s < TPW; s++) {
+              const T* ca = sL + (16 * tIv[s] + col) * RSK;
+              const T* cb = sL + (16 * tJv[s] + col) * RSK;
+              s4_t ga, gm, ia, gb, ta, tb;
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            av[I][j] = to_bf16(src[j]);
-            bv[I][j] = to_bf16(src[j] * sW[kq + j]);
-          }
-        }
-        int t = 0;
+              for (int j = 0; j < 4; j++) {
+                const int k = kq + j;
+                ga[j] = to_bf16(ca[k]);
+                gm[j] = to_bf16(cb[k] * sW[k]);
+                ia[j] = to_bf16(ca[tIk + k]);
+                gb[j] = to_bf16(cb[tIk + k] * sW[ipc + k]);
+                ta[j] = to_bf16(ca[tsk + k]);
+                tb[j] = to_bf16(cb[tsk + k] * sW[2 * ipc + k]);
+              }
+              accM[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga, gm, accM[s], 0, 0, 0);
+              accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ia, gb, accC[s], 0, 0, 0);
+              accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ta, tb, accC[s], 0, 0, 0);
+              if (NR == 4) {
+                s4_t ha, hb;
 #pragma unroll
-        for (int I = 0; I < NTI; I++)
-#pragma unroll
-          for (int J = I; J < NTI; J++, t++) {
-            if (g == 0) accM[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[I], bv[J], accM[t], 0, 0, 0);
-            else accC[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[I], bv[J], accC[t], 0, 0, 0);
-          }
-      } else {
-        // KS = 4: the group spans IPC/4 k-steps
-#pragma unroll
-        for (int ks = 0; ks < IPC / 4; ks++) {
-          const int kk = k0 + ks * 4 + (l >> 4);
-          T av[NTI], bv[NTI];
-          const T wk = sW[kk];
-#pragma unroll
-          for (int I = 0; I < NTI; I++) { av[I] = sL[(16 * I + col_in) * RS + kk]; bv[I] = av[I] * wk; }
-          int t = 0;
-#pragma unroll
-          for (int I = 0; I < NTI; I++)
-#pragma unroll
-            for (int J = I; J < NTI; J++, t++) {
-              if constexpr (PREC == 0) {
-                if (g == 0) accM[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], bv[J], accM[t], 0, 0, 0);
-                else accC[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I], bv[J], accC[t], 0, 0, 0);
-              } else {
-                if (g == 0) accM[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[I], bv[J], accM[t], 0, 0, 0);
-                else accC[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[I], bv[J], accC[t], 0, 0, 0);
+                for (int j = 0; j < 4; j++) {
+                  const int k = kq + j;
+                  ha[j] = to_bf16(ca[3 * KR + k]);
+                  hb[j] = to_bf16(cb[3 * KR + k] * sW[3 * ipc + k]);
+                }
+                accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ha, hb, accC[s], 0, 0, 0);
               }
             }
+          }
+        } else {
+#pragma unroll 1
+          for (int k0 = 0; k0 < ipc; k0 += KS) {
+            const int k = k0 + (l >> 4);
+            const T wm = sW[k], wb = sW[ipc + k], wg = sW[2 * ipc + k];
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+              const T* ca = sL + (16 * tIv[s] + col) * RSK;
+              const T* cb = sL + (16 * tJv[s] + col) * RSK;
+              const T ga = ca[k], gbv = cb[k], ia = ca[tIk + k], ib = cb[tIk + k] * wb, ta = ca[tsk + k], tb = cb[tsk + k] * wg;
+              if constexpr (PREC == 0) {
+                accM[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(ga, gbv * wm, accM[s], 0, 0, 0);
+                accC[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(ia, ib, accC[s], 0, 0, 0);
+                accC[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, tb, accC[s], 0, 0, 0);
+                if (NR == 4) accC[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[3 * KR + k], cb[3 * KR + k] * sW[3 * ipc + k], accC[s], 0, 0, 0);
+              } else {
+                accM[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga, gbv * wm, accM[s], 0, 0, 0);
+                accC[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ia, ib, accC[s], 0, 0, 0);
+                accC[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ta, tb, accC[s], 0, 0, 0);
+                if (NR == 4) accC[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[3 * KR + k], cb[3 * KR + k] * sW[3 * ipc + k], accC[s], 0, 0, 0);
+              }
+            }
+          }
         }
       }
     }
-    __syncthreads();  // rows consumed before the next chunk overwrites them
-  }
-  // epilogue: M upper tiles to LDS
-  {
-    int t = 0;
+    __syncthreads();  // rows consumed
+    // epilogue 1: park M densely (both triangles, row stride MDS)
+    constexpr int MDS = NCP + 1;
+    T* sM = sL;
+    if (active) {
 #pragma unroll
-    for (int I = 0; I < NTI; I++)
+      for (int s = 0; s < TPW; s++) {
+        if (wv + WPE * s >= NTILE) continue;
 #pragma unroll
-      for (int J = I; J < NTI; J++, t++)
+        for (int r = 0; r < 4; r++) {
+          const int R = 16 * tIv[s] + MT::row(l, r), Cc = 16 * tJv[s] + (l & 15);
+          sM[R * MDS + Cc] = accM[s][r];
+          sM[Cc * MDS + R] = accM[s][r];
+        }
+      }
+    }
+    __syncthreads();
+    // epilogue 2: K = M^T (per node block) + tr(M_ab) 1 + C, kept in the C accumulators.  For entry
+    // (R, Cc) = (ND a + i, ND b + j): M^T is M[R - i + j][Cc - j + i], the trace runs down the block diagonal.
+    if (active) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) sM[(t * 16 + MT::row(l, r)) * MS + (l & 15)] = accM[t][r];
-  }
-  __syncthreads();
-  auto tile_of = [&](int I, int J) { return I * NTI - I * (I - 1) / 2 + (J - I); };
-  // M is symmetric: entries below the tile diagonal are read from their mirror
-  auto Mread = [&](int R, int Cc) -> double {
-    if ((R >> 4) > (Cc >> 4)) { const int x = R; R = Cc; Cc = x; }
-    return (double)sM[(tile_of(R >> 4, Cc >> 4) * 16 + (R & 15)) * MS + (Cc & 15)];
-  };
-  const uint8_t* rk = rank + e * NN;
-  typename MT::Out* out = ke + e * (int64_t)NN * NN * ND * ND;
-  int t = 0;
+      for (int s = 0; s < TPW; s++) {
 #pragma unroll
-  for (int I = 0; I < NTI; I++)
-#pragma unroll
-    for (int J = I; J < NTI; J++, t++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int R = 16 * I + MT::row(l, r), Cc = 16 * J + (l & 15);
-        if (R >= NC || Cc >= NC) continue;
-        const int a = R / ND, i = R - a * ND, b = Cc / ND, j = Cc - b * ND;
-        if (a > b || (a == b && i > j)) continue;
-        double v = (double)accC[t][r] + Mread(ND * a + j, ND * b + i);
-        if (i == j) {
+        for (int r = 0; r < 4; r++) {
+          const int R = 16 * tIv[s] + MT::row(l, r), Cc = 16 * tJv[s] + (l & 15);
+          if (R >= NC || Cc >= NC) continue;
+          const int i = R % ND, j = Cc % ND;
+          const int base = R * MDS + Cc;
+          double v = (double)accC[s][r] + (double)sM[base + (j - i) * (MDS - 1)];
+          const int d0 = base - i * MDS - j;
           double tr = 0.0;
 #pragma unroll
-          for (int k = 0; k < ND; k++) tr += Mread(ND * a + k, ND * b + k);
-          v += tr;
+          for (int k = 0; k < ND; k++) tr += (double)sM[d0 + k * (MDS + 1)];
+          accC[s][r] = (T)(i == j ? v + tr : v);
         }
-        const int ra = rk[a], rb = rk[b];
-        out[(ra * NN + rb) * ND * ND + i * ND + j] = (typename MT::Out)v;
-        if (a != b) out[(rb * NN + ra) * ND * ND + j * ND + i] = (typename MT::Out)v;
-        else if (i != j) out[(ra * NN + ra) * ND * ND + j * ND + i] = (typename MT::Out)v;
       }
+    }
+    __syncthreads();  // M no longer needed: its space takes the node blocks
+    // epilogue 3: node blocks (a, b) at (a*NN + b)*B2 (rows and columns are already in rank order)
+    T* sK = sL;
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < TPW; s++) {
+        if (wv + WPE * s >= NTILE) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int R = 16 * tIv[s] + MT::row(l, r), Cc = 16 * tJv[s] + (l & 15);
+          if (R >= NC || Cc >= NC || R > Cc) continue;
+          const int i = R % ND, j = Cc % ND;
+          // (a*NN + b)*B2 + i*ND + j with a = (R - i)/ND, b = (Cc - j)/ND
+          const T v = accC[s][r];
+          sK[ND * (NN * (R - i) + (Cc - j)) + i * ND + j] = v;
+          sK[ND * (NN * (Cc - j) + (R - i)) + j * ND + i] = v;
+        }
+      }
+    }
+    __syncthreads();
+    // epilogue 4: contiguous write of the element matrix in 16-byte pieces.  The trip count is a
+    // compile-time constant so the store count is known where the next unit waits for its prefetch.
+    if (active) {
+      typedef T V __attribute__((ext_vector_type(16 / sizeof(T))));
+      static_assert(EW * sizeof(T) % 16 == 0, "element matrix is a whole number of 16-byte pieces");
+      constexpr int NV = EW * (int)sizeof(T) / 16, IT = (NV + TPE - 1) / TPE;
+      V* dst = reinterpret_cast<V*>(ke + (u * EPW + el) * (int64_t)EW);
+      const V* src = reinterpret_cast<const V*>(sK);
+#pragma unroll
+      for (int it = 0; it < IT; it++) {
+        const int i = min(te + it * TPE, NV - 1);  // surplus lanes repeat the last piece (same value)
+        dst[i] = src[i];
+      }
+    }
+  }
+}
+
+constexpr size_t MFMA_LDS_TARGET = 56 * 1024;  // dynamic LDS per workgroup that the chunk size aims for
+constexpr size_t MFMA_LDS_MAX = 64 * 1024;
+
+template <int ND, int NN, int PREC>
+int mfma_chunk(int nip, int axisym) {
+  constexpr int KS = MfmaT<PREC>::KS;
+  int ipc = (nip + KS - 1) / KS * KS;
+  while (ipc > KS && mfma_layout<ND, NN, PREC>(nip, ipc, axisym).total * sizeof(double) > MFMA_LDS_TARGET) ipc -= KS;
+  return ipc;
+}
+
+template <int ND, int NN, int PREC>
+bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
+  using S = MShape<ND, NN>;
+  using T = typename MfmaT<PREC>::T;
+  const int ax = f->axisym ? 1 : 0;
+  // one lane per IP, and one prefetched state value per thread
+  if (g.nip > 64 || (int64_t)S::EPW * S::SV * g.nip > (int64_t)S::NTHR) return false;
+  const int ipc = mfma_chunk<ND, NN, PREC>(g.nip, ax);
+  const size_t shm = mfma_layout<ND, NN, PREC>(g.nip, ipc, ax).total * sizeof(double);
+  if (shm > MFMA_LDS_MAX) return false;
+  const int64_t nunits = (g.nel + S::EPW - 1) / S::EPW;
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(f->lds_per_cu / (int64_t)(shm + 1024), 32 / (S::WPE * S::EPW)));
+  const int64_t nwg = std::min<int64_t>(nunits, per_cu * f->n_cu);
+  const int64_t upw = (nunits + nwg - 1) / nwg;
+  const int64_t grid = (nunits + upw - 1) / upw;
+  const size_t b2 = (size_t)ND * ND;
+  T* ke;
+  if constexpr (PREC == 0) ke = f->d_ke + g.ke_base * b2;
+  else ke = f->d_ke32 + g.ke_base * b2;
+  hipLaunchKernelGGL((k_tangent_mfma<ND, NN, PREC>), dim3((unsigned)grid), dim3(S::NTHR), shm, f->stream, g.nel, g.nip, ipc, upw,
+                     g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err);
+  return true;
+}
+
+template <int ND, int NN>
+bool launch_mfma_prec(pm_fem* f, Group& g, const MatM& mp, int prec) {
+  if (prec == 0) return launch_mfma_kind<ND, NN, 0>(f, g, mp);
+  if (prec == 1) return launch_mfma_kind<ND, NN, 1>(f, g, mp);
+  return launch_mfma_kind<ND, NN, 2>(f, g, mp);
 }
 
 }  // namespace
 
-template <int ND, int NN, int PREC>
-static size_t mfma_shm(int nip) {
-  using C = MCfg<ND, NN, PREC>;
-  using T = typename MfmaT<PREC>::T;
-  size_t rows = (size_t)C::NCP * mfma_rows_stride<ND, NN, PREC>() * sizeof(T) + 4 * C::IPC * sizeof(T);
-  size_t mt = (size_t)C::NTILE * 16 * 17 * sizeof(typename MfmaT<PREC>::Out);
-  return sizeof(double) * ((size_t)NN * ND + (size_t)nip * C::QW) + std::max(rows, mt);
-}
-
-// Launches the MFMA tangent for one batch; returns false when the kind has no MFMA variant.
+// Launches the matrix-core tangent of one group; false when the kind or its IP count has no variant.
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec) {
   if (g.nel == 0) return true;
-  MatM mp{g.param.model, g.plane_stress ? 1 : 0, g.K, g.G, g.param.hh, g.param.z_ini, g.param.young, g.param.poisson};
-  const int ax = f->axisym ? 1 : 0;
-  const size_t b2 = (size_t)g.ndim * g.ndim;
-#define PM_MF(ND, NN)                                                                                                        \
-  do {                                                                                                                       \
-    if (prec == 0) {                                                                                                         \
-      size_t shm = mfma_shm<ND, NN, 0>(g.nip);                                                                               \
-      hipLaunchKernelGGL((k_tangent_mfma<ND, NN, 0>), dim3((unsigned)g.nel), dim3(64), shm, f->stream, g.nel, g.nip, g.d_conn, \
-                         g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base * b2, mp, ax, f->thickness, f->d_err);    \
-    } else {                                                                                                                 \
-      if (prec == 1) {                                                                                                       \
-        size_t shm = mfma_shm<ND, NN, 1>(g.nip);                                                                             \
-        hipLaunchKernelGGL((k_tangent_mfma<ND, NN, 1>), dim3((unsigned)g.nel), dim3(64), shm, f->stream, g.nel, g.nip,       \
-                           g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * b2, mp, ax, f->thickness,  \
-                           f->d_err);                                                                                        \
-      } else {                                                                                                               \
-        size_t shm = mfma_shm<ND, NN, 2>(g.nip);                                                                             \
-        hipLaunchKernelGGL((k_tangent_mfma<ND, NN, 2>), dim3((unsigned)g.nel), dim3(64), shm, f->stream, g.nel, g.nip,       \
-                           g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * b2, mp, ax, f->thickness,  \
-                           f->d_err);                                                                                        \
-      }                                                                                                                      \
-    }                                                                                                                        \
-  } while (0)
+  MatM mp;
+  mp.model = g.param.model;
+  mp.plane_stress = g.plane_stress ? 1 : 0;
+  mp.G3 = 3.0 * g.G;
+  mp.G2 = 2.0 * g.G;
+  mp.G6G = 6.0 * g.G * g.G;
+  mp.inv_d = 1.0 / (3.0 * g.G + g.param.hh);
+  mp.K = g.K;
+  mp.beta_e = g.K - mp.G2 / 3.0;
+  const double E = g.param.young, nu = g.param.poisson;
+  mp.a_ps = E / (1.0 + nu);
+  mp.b_ps = E * nu / (1.0 - nu * nu);
   switch (g.kind) {
-    case TRI3: PM_MF(2, 3); break;
-    case TRI6: PM_MF(2, 6); break;
-    case TRI10: PM_MF(2, 10); break;
-    case TRI15: PM_MF(2, 15); break;
-    case QUA4: PM_MF(2, 4); break;
-    case QUA8: PM_MF(2, 8); break;
-    case QUA9: PM_MF(2, 9); break;
-    case TET4: PM_MF(3, 4); break;
-    case TET10: PM_MF(3, 10); break;
-    case HEX8: PM_MF(3, 8); break;
-    case HEX20: PM_MF(3, 20); break;
+    case TRI3: return launch_mfma_prec<2, 3>(f, g, mp, prec);
+    case TRI6: return launch_mfma_prec<2, 6>(f, g, mp, prec);
+    case TRI10: return launch_mfma_prec<2, 10>(f, g, mp, prec);
+    case TRI15: return launch_mfma_prec<2, 15>(f, g, mp, prec);
+    case QUA4: return launch_mfma_prec<2, 4>(f, g, mp, prec);
+    case QUA8: return launch_mfma_prec<2, 8>(f, g, mp, prec);
+    case QUA9: return launch_mfma_prec<2, 9>(f, g, mp, prec);
+    case TET4: return launch_mfma_prec<3, 4>(f, g, mp, prec);
+    case TET10: return launch_mfma_prec<3, 10>(f, g, mp, prec);
+    case HEX8: return launch_mfma_prec<3, 8>(f, g, mp, prec);
+    case HEX20: return launch_mfma_prec<3, 20>(f, g, mp, prec);
     default: return false;
   }
-#undef PM_MF
-  return true;
 }
 
 }  // namespace pm
