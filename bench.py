@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense FP64 / FP32 / BF16 matrix peaks
 FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
 
 
@@ -44,6 +44,23 @@ def increment(mesh, seed=7):
     u[:, 2] = -2.5e-3 * X[:, 0] * X[:, 2]
     u += rng.uniform(-2e-6, 2e-6, X.shape)
     return u.ravel()
+
+
+def measured_traffic(nx, ny, nz, precision, kernel):
+    """HBM bytes per launch of the tangent kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (scaled by the copy-kernel calibration stored with it), or None."""
+    if (nx, ny, nz) != (64, 64, 32):
+        return None, None
+    tag = {("f64", "default"): "f64_vector", ("f64", "vector"): "f64_vector", ("f64", "mfma"): "f64_mfma",
+           ("bf16", "default"): "bf16_mfma", ("bf16", "mfma"): "bf16_mfma"}.get((precision, kernel))
+    path = os.path.join(ROOT, "profiles", f"element_pass_hex20_64x64x32_{tag}.json")
+    if tag is None or not os.path.exists(path):
+        return None, None
+    prof = json.load(open(path))
+    for name, k in prof["kernels"].items():
+        if name.startswith("k_tangent"):
+            return k["read_bytes"] + k["write_bytes"], name
+    return None, None
 
 
 def cpu_baseline(seconds=12.0):
@@ -96,6 +113,7 @@ def main():
     value = nip / (ms["total"] * 1e-3)
     tangent_flops = FLOP_PER_IP * nip
     achieved_tf = tangent_flops / (ms["tangent"] * 1e-3) / 1e12
+    traffic, kname = measured_traffic(a.nx, a.ny, a.nz, a.precision, a.kernel)
     out = {
         "metric": "Hex20 IP-updates/s (K+R+return map)",
         "value": value,
@@ -113,9 +131,16 @@ def main():
                    "elements": int(mesh.ncell), "ips": int(nip), "dofs": int(f.ndof), "plastic_fraction": plastic,
                    "copy_calibration_bytes": int(2 * nip * (f.ns + 4) * 8)},
         "kernels_ms": {k: ms[k] for k in ("update", "residual", "tangent", "scatter")},
-        "roofline": {"bound": "mfma", "kernel": "k_tangent", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None},
+        "roofline": {"bound": "mfma", "kernel": kname or "k_tangent", "achieved": achieved_tf, "peak": PEAK_TFLOPS[a.precision],
+                     "unit": "TFLOP/s", "frac": achieved_tf / PEAK_TFLOPS[a.precision], "traffic": traffic,
+                     "flop_per_ip": FLOP_PER_IP},
     }
+    # the scatter is the HBM-bound half of the assembly: element blocks in, CSR blocks out, block map
+    nblk_el = f.nip // 27 * 400
+    scatter_bytes = nblk_el * 9 * (8 if a.precision == "f64" else 4) + f.nnzb * 9 * 8 * (1 if a.precision == "f64" else 1.5) \
+        + nblk_el * 4 + (f.nnzb + 1) * 4
+    out["hbm"] = {"kernel": "k_scatter", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": scatter_bytes}
     if not a.no_cpu:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb
