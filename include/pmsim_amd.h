@@ -50,6 +50,7 @@ typedef struct pm_config {
   int32_t tangent_kernel;           /* 0: default per element kind, 1: node-block vector kernel (FP64 arithmetic),
                                        2: matrix-core (MFMA) kernel in the tangent precision, 3: tune at pm_initialize
                                        (fastest measured; FP64 variants agree to rounding, not bitwise) */
+  int32_t use_graphs;               /* 1 (default): element passes and 16-iteration PCG chunks replay captured HIP graphs */
 } pm_config;
 
 /* Statistics of the last pm_solve / element pass. */
@@ -127,8 +128,10 @@ int pm_get_csr(const pm_fem* fem, int64_t* rowptr, int32_t* cols, double* vals);
 /* History of output steps kept with keep_history: count, then u (n x ndof) and IP states. */
 int pm_history_count(const pm_fem* fem, int32_t* n);
 int pm_get_history(const pm_fem* fem, int32_t index, double* u, double* lambda, double* ip_state, double* t);
-/* Times `reps` element passes with HIP events on the handle's stream; per kernel class in ms. */
-int pm_time_element_pass(pm_fem* fem, int32_t reps, double* ms /* [update, residual, tangent, scatter, total] */);
+/* Times `reps` element passes with HIP events on the handle's stream: per kernel class with events
+ * between the launches, then `reps` passes as the solver runs them (one captured graph per pass when
+ * use_graphs is set), in ms per pass. */
+int pm_time_element_pass(pm_fem* fem, int32_t reps, double* ms /* [update, residual, tangent, scatter, total, pass] */);
 /* Linear solve of the current K with a given right-hand side on the free DOFs (testing and
  * benchmarking the PCG): x = K_ff^-1 b_f, prescribed entries of x are zero. */
 int pm_linear_solve(pm_fem* fem, const double* b, double* x, int32_t* iterations);

@@ -113,6 +113,7 @@ int device_alloc(pm_fem* f) {
 }
 
 void device_free(pm_fem* f) {
+  release_graphs(f);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
   fr(f->d_X); fr(f->d_browptr); fr(f->d_bcols); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);

@@ -141,6 +141,9 @@ struct pm_fem {
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags
   int64_t pcg_small_max = 16384;       // systems up to this many DOFs use the single-workgroup PCG
+  hipGraphExec_t pass_graph[8] = {};   // captured element passes, by (update, residual, tangent)
+  hipGraphExec_t pcg_graph = nullptr;  // captured chunk of PCG iterations
+  const double* pcg_graph_x = nullptr; // solution vector the chunk was captured for
   int64_t device_bytes = 0;
   // state
   double t = 0;
@@ -171,6 +174,8 @@ void launch_tangent(pm_fem* f, Group& g);
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 // choose the tangent kernel (and work split) of every group, timing the candidates when asked to
 int tune_tangent(pm_fem* f);
+// drops every captured graph (configuration, tuning or buffers changed)
+void release_graphs(pm_fem* f);
 void launch_scatter(pm_fem* f);
 void launch_init_state(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);

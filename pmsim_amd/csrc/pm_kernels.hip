@@ -691,6 +691,7 @@ void launch_tangent(pm_fem* f, Group& g) {
 // the fastest; mode 0 keeps the default.
 int tune_tangent(pm_fem* f) {
   const int mode = f->cfg.tangent_kernel;
+  release_graphs(f);
   for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_epb = g.tune_nqg = 0; }
   if (mode != 3) return 0;
   struct Cand { int kern, epb, nqg; };

@@ -538,10 +538,9 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   if (bb == 0.0) return 0;
   if (!std::isfinite(bb)) { f->err = "Found NaN or Inf"; return 1; }
   const double tol2 = rtol2 * bb;
-  const int check_every = 16;
-  int it = 0;
-  double rr = bb;
-  while (it < f->cfg.pcg_max_iter) {
+  const int check_every = 16;  // even, so a chunk starts on the same rz slot every time
+  // one PCG iteration = 3 launches; iteration `it` reads rz from slot it&1 and writes slot (it+1)&1
+  auto iterate = [&](int it) {
     const double* rz_old = rzs + (it & 1);
     double* rz_new = rzs + ((it + 1) & 1);
     if (f->bs == 2) {
@@ -554,7 +553,30 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
       hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     }
     hipLaunchKernelGGL(k_pcg_p2, dim3(rb), dim3(PCG_TPB), 0, f->stream, n, rb, pb, pc, rz_old, rz_new, rr_s, z, p);
-    it++;
+  };
+  // a whole chunk of iterations between two convergence checks is captured once per handle (and
+  // solution buffer) and replayed as one graph launch
+  if (f->cfg.use_graphs && f->pcg_graph_x != d_x && f->pcg_graph) { (void)hipGraphExecDestroy(f->pcg_graph); f->pcg_graph = nullptr; }
+  if (f->cfg.use_graphs && !f->pcg_graph) {
+    hipGraph_t graph = nullptr;
+    PM_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < check_every; k++) iterate(k);
+    PM_HIP(hipStreamEndCapture(f->stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&f->pcg_graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { f->pcg_graph = nullptr; f->err = std::string("graph instantiation failed: ") + hipGetErrorString(e); return 1; }
+    f->pcg_graph_x = d_x;
+  }
+  int it = 0;
+  double rr = bb;
+  while (it < f->cfg.pcg_max_iter) {
+    if (f->cfg.use_graphs && it % check_every == 0 && it + check_every <= f->cfg.pcg_max_iter) {
+      PM_HIP(hipGraphLaunch(f->pcg_graph, f->stream));
+      it += check_every;
+    } else {
+      iterate(it);
+      it++;
+    }
     if (it % check_every == 0 || it == f->cfg.pcg_max_iter) {
       PM_HIP(hipMemcpyAsync(&rr, rr_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
       PM_HIP(hipStreamSynchronize(f->stream));

@@ -18,14 +18,45 @@ namespace pm {
 
 constexpr int64_t kHostDirectMaxDof = 300000;  // host envelope Cholesky is the fallback below this size
 
-int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
+static void launch_pass(pm_fem* f, bool update, bool residual, bool tangent) {
   for (auto& g : f->groups) launch_update(f, g, update);
   if (residual) launch_residual_gather(f);
   if (tangent) {
     for (auto& g : f->groups) launch_tangent(f, g);
     launch_scatter(f);
   }
-  if (hipGetLastError() != hipSuccess) { f->err = "kernel launch failed"; return 1; }
+}
+
+void release_graphs(pm_fem* f) {
+  for (auto& ge : f->pass_graph)
+    if (ge) { (void)hipGraphExecDestroy(ge); ge = nullptr; }
+  if (f->pcg_graph) { (void)hipGraphExecDestroy(f->pcg_graph); f->pcg_graph = nullptr; }
+  f->pcg_graph_x = nullptr;
+}
+
+// One element pass (every batch's update, then the gathers).  With use_graphs the launches of each
+// (update, residual, tangent) combination are captured once into a HIP graph and replayed: one submit
+// instead of 2 + 2 x (element batches) kernel launches.  The captured kernels read only buffers whose
+// addresses are fixed for the handle's lifetime; tuning or a configuration change drops the graphs.
+int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
+  if (!f->cfg.use_graphs) {
+    launch_pass(f, update, residual, tangent);
+    if (hipGetLastError() != hipSuccess) { f->err = "kernel launch failed"; return 1; }
+    return 0;
+  }
+  hipGraphExec_t& ge = f->pass_graph[(update ? 1 : 0) | (residual ? 2 : 0) | (tangent ? 4 : 0)];
+  if (!ge) {
+    hipGraph_t graph = nullptr;
+    PM_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeRelaxed));
+    launch_pass(f, update, residual, tangent);
+    const hipError_t launched = hipGetLastError();
+    PM_HIP(hipStreamEndCapture(f->stream, &graph));
+    if (launched != hipSuccess) { (void)hipGraphDestroy(graph); f->err = "kernel launch failed"; return 1; }
+    const hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { ge = nullptr; f->err = std::string("graph instantiation failed: ") + hipGetErrorString(e); return 1; }
+  }
+  PM_HIP(hipGraphLaunch(ge, f->stream));
   return 0;
 }
 

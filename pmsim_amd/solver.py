@@ -37,6 +37,7 @@ class PmConfig(C.Structure):
         ("tol_rr_abs", C.c_double), ("tol_mdu_rel", C.c_double), ("symmetry_check_tolerance", C.c_double),
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
         ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
+        ("use_graphs", C.c_int32),
     ]
 
 
@@ -125,6 +126,7 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
         c.pcg_max_iter = cfg.pcg_max_iter
         c.tol_rr_scaled = cfg.scaled_rr
         c.tangent_kernel = TANGENT_KERNEL[cfg.tangent_kernel]
+        c.use_graphs = 1 if cfg.use_graphs else 0
     return c
 
 
@@ -242,9 +244,9 @@ class Fem:
         return out
 
     def time_element_pass(self, reps=10):
-        ms = np.zeros(5)
+        ms = np.zeros(6)
         self._check(lib().pm_time_element_pass(self.h, int(reps), ms))
-        return dict(update=ms[0], residual=ms[1], tangent=ms[2], scatter=ms[3], total=ms[4])
+        return dict(update=ms[0], residual=ms[1], tangent=ms[2], scatter=ms[3], total=ms[4], pass_=ms[5])
 
     def linear_solve(self, b):
         x = np.zeros(self.ndof)

@@ -35,3 +35,21 @@ def strip_footing(nx=16, ny=8, ngauss=9, n_inc=20, young=1e4, poisson=0.3, z_ini
     cfg = Config(mesh)
     cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
     return mesh, base, cfg, ess, Natural()
+
+
+def mixed_compression(n_elements=200_000, n_inc=8, uy=-0.02, lx=100.0, ly=100.0, seed=20261203, jitter=0.1,
+                      young=1e4, poisson=0.25, z_ini=100.0):
+    """Mixed 2D mesh over lx x ly (30% Qua8, 25% Qua4, 20% Tri6, 15% Qua9, 10% Tri15 by element count,
+    seeded, node jitter), plane strain von Mises E=1e4 nu=0.25 z0=100 H=0 with each kind's default
+    integration rule; bottom edges fixed, top edges uy ramping to `uy` in n_inc increments."""
+    mesh = mixed_2d(n_elements, lx, ly, seed=seed, jitter=jitter)
+    feat = Features(mesh)
+    bottom = feat.search_edges(At.Y(0.0))
+    top = feat.search_edges(At.Y(ly))
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.VonMises(young, poisson, 0.0, z_ini), None)))])
+    ess = Essential()
+    ess.edges(bottom, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    ess.edges_fn(top, Dof.Uy, uy, lambda t: t / n_inc)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, Natural()
