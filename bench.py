@@ -117,6 +117,103 @@ def run_mixed2d(a):
     print(json.dumps(out), flush=True)
 
 
+def run_serve(a):
+    """Serving: Poisson arrivals of parametric strip-footing solves, admitted by HBM footprint and solved
+    concurrently (pm_serve); solves/s and latency percentiles, FP64 and FP32-tangent runs of the same
+    trace, and the largest relative difference of their final displacements."""
+    from pmsim_amd import describe
+    from pmsim_amd.solver import Fem, serve
+    from pmsim_amd.workloads import footing_request, poisson_arrivals
+    rng = np.random.default_rng(a.seed)
+    reqs = []
+    t0 = time.time()
+    for _ in range(a.requests):
+        (mesh, base, cfg, ess, nat), p = footing_request(rng)
+        reqs.append((mesh, base, cfg, ess, nat, p))
+    arrivals = poisson_arrivals(np.random.default_rng(a.seed + 1), a.requests, a.rate)
+    print(f"setup: {a.requests} requests, {sum(r[0].ncell for r in reqs)} elements in all, {time.time() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    # untimed warm-up: the first launches of each kernel load its code object
+    wrng = np.random.default_rng(1)
+    for prec in ("f64", "f32"):
+        (mesh, base, cfg, ess, nat), _ = footing_request(wrng, (8, 12), (4, 6))
+        cfg.set_tangent_precision(prec)
+        serve([Fem(describe(mesh, base, cfg, ess, nat), cfg, keep_history=False, initialize=False)], [0.0])
+    runs = {}
+    for prec in ("f64", "f32"):
+        hs = []
+        for mesh, base, cfg, ess, nat, p in reqs:
+            cfg.set_scaled_rr(1e-10).set_pcg(1e-12, 50000).set_tangent_precision(prec)
+            hs.append(Fem(describe(mesh, base, cfg, ess, nat), cfg, keep_history=False, initialize=False))
+        res = serve(hs, arrivals, hbm_budget_bytes=int(a.hbm_gb * (1 << 30)), max_active=a.max_active)
+        ok = [r for r in res if r["status"] == 0]
+        lat = np.array([r["t_done"] - r["t_arrival"] for r in ok])
+        span = max(r["t_done"] for r in res) - min(arrivals)
+        runs[prec] = dict(res=res, u=[h.get_u()[0] if r["status"] == 0 else None for h, r in zip(hs, res)],
+                          solves_per_s=len(ok) / span, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                          failed=len(res) - len(ok), newton=sum(r["n_iterations"] for r in ok),
+                          pcg=sum(r["n_pcg_iterations"] for r in ok))
+        print(f"{prec}: {len(ok)}/{len(res)} solved in {span:.1f} s, p50 {runs[prec]['p50']:.2f} s, p99 {runs[prec]['p99']:.2f} s",
+              file=sys.stderr, flush=True)
+    du = [float(np.abs(u2 - u1).max() / max(np.abs(u1).max(), 1e-300))
+          for u1, u2 in zip(runs["f64"]["u"], runs["f32"]["u"]) if u1 is not None and u2 is not None]
+    f64 = runs["f64"]
+    out = {"metric": "served solves/s", "value": f64["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": a.requests,
+           "warmup": 0, "ms_per_step": 1e3 / f64["solves_per_s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded parametric strip footings, Poisson arrivals)",
+           "config": {"workload": f"{a.requests} strip-footing solves (Qua8 nx in [8,256], ny in [4,128], 5-40 increments), "
+                                  f"Poisson arrivals at {a.rate}/s", "max_active": a.max_active, "hbm_budget_gb": a.hbm_gb,
+                      "elements": [int(r[0].ncell) for r in reqs]},
+           "latency_s": {"p50": f64["p50"], "p99": f64["p99"]},
+           "fp64": {k: f64[k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
+           "fp32_tangent": {k: runs["f32"][k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
+           "u_rel_diff_fp32_vs_fp64": {"max": max(du) if du else None, "median": float(np.median(du)) if du else None}}
+    print(json.dumps(out), flush=True)
+
+
+def run_precision(a):
+    """Inexact-Newton tangents on Hex20 96x96x48 (5.48 M DOF, 11.9 M IPs): the same incremental solve with
+    FP64, FP32 and BF16->FP32-accumulate tangents (residual, return map and PCG vectors stay FP64);
+    errors of u, sigma and epbar against the FP64 run, Newton and PCG iteration counts."""
+    from pmsim_amd import describe
+    from pmsim_amd.solver import Fem
+    from pmsim_amd.workloads import hex20_cube
+    t0 = time.time()
+    mesh, base, cfg, ess, nat = hex20_cube(a.nx, a.ny, a.nz, n_inc=a.n_inc, uz=-0.001 * a.n_inc)
+    print(f"mesh: {mesh.ncell} cells, {mesh.npoint} points, {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    ref = None
+    out = {"metric": "Hex20 inexact-Newton tangent precision", "n_gpus": 1, "data": "synthetic (jittered Hex20 cube, seeded)",
+           "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, {a.n_inc} increments of uz = -0.001",
+                      "elements": int(mesh.ncell)}, "runs": {}}
+    for prec in a.precisions.split(","):
+        cfg.set_tangent_precision(prec).set_scaled_rr(1e-9).set_pcg(1e-9, 200000).set_n_max_iterations(30).set_verbose(True, True)
+        d = describe(mesh, base, cfg, ess, nat)
+        f = Fem(d, cfg, keep_history=False)
+        print(f"{prec}: handle ready ({f.ndof} DOFs, {f.nnzb} K blocks), {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+        t1 = time.time()
+        f.solve()
+        secs = time.time() - t1
+        st = f.stats()
+        u = f.get_u()[0]
+        ip = f.get_ip_state()
+        run = {"seconds": secs, "steps": st["n_steps"], "newton_iterations": st["n_iterations"],
+               "pcg_iterations": st["n_pcg_iterations"], "ms_linear": st["ms_linear"], "ms_tangent": st["ms_tangent"],
+               "plastic_fraction": float(np.mean(ip[:, f.ns + 2] == 0.0)), "dofs": f.ndof, "ips": f.nip}
+        if ref is None:
+            ref = (u, ip, f.ns)
+        else:
+            u0, ip0, ns = ref
+            run["u_rel_err"] = float(np.abs(u - u0).max() / np.abs(u0).max())
+            run["sigma_rel_err"] = float(np.abs(ip[:, :ns] - ip0[:, :ns]).max() / np.abs(ip0[:, :ns]).max())
+            run["epbar_rel_err"] = float(np.abs(ip[:, ns + 3] - ip0[:, ns + 3]).max() / max(np.abs(ip0[:, ns + 3]).max(), 1e-300))
+            run["plastic_flags_differ"] = int(np.sum((ip[:, ns + 2] == 0.0) != (ip0[:, ns + 2] == 0.0)))
+        out["runs"][prec] = run
+        print(f"{prec}: {json.dumps(run)}", file=sys.stderr, flush=True)
+        f.close()
+        del f, ip
+    print(json.dumps(out), flush=True)
+
+
 M_NAMES = {0: "Tri3", 1: "Tri6", 2: "Tri10", 3: "Tri15", 4: "Qua4", 5: "Qua8", 6: "Qua9", 7: "Tet4", 8: "Tet10", 9: "Hex8", 10: "Hex20"}
 
 
@@ -131,7 +228,14 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d"])
+    ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision"])
+    ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
+    ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--max-active", type=int, default=16, help="serve: concurrently running requests")
+    ap.add_argument("--hbm-gb", type=float, default=200.0, help="serve: device-memory budget for admission")
+    ap.add_argument("--seed", type=int, default=20261203, help="serve: request and arrival seed")
     ap.add_argument("--elements", type=int, default=200_000, help="mixed2d: element count")
     ap.add_argument("--solve", action="store_true", help="mixed2d: also time the incremental Newton solve")
     a = ap.parse_args()
@@ -141,6 +245,12 @@ def main():
         sys.exit(2)
     if a.workload == "mixed2d":
         return run_mixed2d(a)
+    if a.workload == "precision":
+        return run_precision(a)
+    if a.workload == "serve":
+        # more hardware queues than the default 4 so the concurrent requests' streams do not serialise
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.max_active)))
+        return run_serve(a)
     from pmsim_amd.solver import Fem
     t0 = time.time()
     mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel)

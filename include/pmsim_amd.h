@@ -51,6 +51,9 @@ typedef struct pm_config {
                                        2: matrix-core (MFMA) kernel in the tangent precision, 3: tune at pm_initialize
                                        (fastest measured; FP64 variants agree to rounding, not bitwise) */
   int32_t use_graphs;               /* 1 (default): element passes and 16-iteration PCG chunks replay captured HIP graphs */
+  int32_t pcg_small_max;            /* systems with at most this many DOFs run the whole PCG in one workgroup; < 0: default */
+  int32_t verbose;                  /* to stderr; bit 0: a line per converged step, bit 1: a line per Newton iteration
+                                       (pmsim's verbose_timesteps / verbose_iterations, config.rs:166-173; default 0) */
 } pm_config;
 
 /* Statistics of the last pm_solve / element pass. */
@@ -135,6 +138,35 @@ int pm_time_element_pass(pm_fem* fem, int32_t reps, double* ms /* [update, resid
 /* Linear solve of the current K with a given right-hand side on the free DOFs (testing and
  * benchmarking the PCG): x = K_ff^-1 b_f, prescribed entries of x are zero. */
 int pm_linear_solve(pm_fem* fem, const double* b, double* x, int32_t* iterations);
+
+/* ---- serving: many independent solves on one GPU ------------------------------------------------
+ * pmsim solves one problem per SolverImplicit::new + solve (src/fem/solver_implicit.rs:27-178); the
+ * entry points below run many such problems (each a fully configured, not yet initialized handle)
+ * concurrently on one device. */
+
+/* Per-request record filled by pm_serve (times in seconds from the start of pm_serve). */
+typedef struct pm_request_stats {
+  double t_arrival, t_admit, t_done;
+  int32_t status;                   /* 0 = solved; otherwise the failing call's code (pm_last_error(handle) has the text) */
+  int32_t n_steps, n_iterations;    /* converged steps, Newton iterations */
+  int64_t n_pcg_iterations;
+  int64_t device_bytes;             /* footprint estimate used for admission */
+} pm_request_stats;
+
+/* Upper bound of the device memory pm_initialize will allocate for this handle (mesh, batches, state,
+ * element matrices, K pattern bound, vectors). */
+int pm_estimate_device_bytes(const pm_fem* fem, int64_t* bytes);
+/* Copies u, lambda and the IP state to host memory and frees every device buffer of the handle; the
+ * getters pm_get_u / pm_get_ip_state / pm_get_history / pm_get_stats keep working from the copies. */
+int pm_release_device(pm_fem* fem);
+/* Continuous batching of independent solves.  Request i arrives at arrival_s[i] (seconds after the
+ * call); requests are admitted in arrival order while fewer than max_active run and the sum of their
+ * footprint estimates stays within hbm_budget_bytes (a lone request is always admitted).  Each admitted
+ * request is initialized on `device` with its own stream, solved (pm_solve) and released
+ * (pm_release_device) on a host worker thread, so the element loops and PCG iterations of the running
+ * requests execute concurrently.  Returns 0 when every request was processed (check out[i].status). */
+int pm_serve(int32_t n, pm_fem* const* fems, const double* arrival_s, int32_t device, int64_t hbm_budget_bytes,
+             int32_t max_active, pm_request_stats* out);
 
 #ifdef __cplusplus
 }

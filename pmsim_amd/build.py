@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmsim_amd.so")
-SOURCES = ["pm_shapes.cpp", "pm_host.cpp", "pm_device.cpp", "pm_solver.cpp", "pm_capi.cpp", "pm_kernels.hip", "pm_tangent_mfma.hip", "pm_linalg.hip"]
+SOURCES = ["pm_shapes.cpp", "pm_host.cpp", "pm_device.cpp", "pm_solver.cpp", "pm_capi.cpp", "pm_serve.cpp", "pm_kernels.hip", "pm_tangent_mfma.hip", "pm_linalg.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -31,7 +31,7 @@ def build(verbose=False, jobs=8):
             procs = []
     _wait(procs, verbose)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)

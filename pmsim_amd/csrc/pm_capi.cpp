@@ -1,4 +1,5 @@
 // extern "C" entry points (include/pmsim_amd.h).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -40,6 +41,7 @@ void pm_default_config(pm_config* c) {
   c->tol_rr_scaled = 0.0;
   c->tangent_kernel = 0;
   c->use_graphs = 1;
+  c->pcg_small_max = -1;
 }
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
@@ -90,6 +92,7 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   if (f->initialized) release_graphs(f);  // captured launches depend on the configuration
   f->cfg = *c;
+  f->pcg_small_max = c->pcg_small_max >= 0 ? c->pcg_small_max : kPcgSmallMaxDefault;
   if (retune) return tune_tangent(f);
   return 0;
 }
@@ -121,7 +124,7 @@ int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* 
 
 int pm_initialize(pm_fem* f, int32_t device) {
   if (!f) { g_last_error = "null handle"; return 1; }
-  if (f->initialized) { f->err = "already initialized"; return 1; }
+  if (f->initialized || f->released) { f->err = "already initialized"; return 1; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { f->err = "no HIP device available"; return 1; }
   if (device < 0 || device >= ndev) { f->err = "invalid HIP device"; return 1; }
@@ -198,7 +201,7 @@ int pm_get_stats(const pm_fem* f, pm_stats* s) {
 }
 
 int pm_sizes(const pm_fem* f, int64_t* out) {
-  if (!f->initialized) return 1;
+  if (!f->initialized && !f->released) return 1;
   out[0] = f->ndof; out[1] = f->nprescribed; out[2] = f->nip; out[3] = f->nsig; out[4] = f->nnzb; out[5] = f->ndim;
   return 0;
 }
@@ -211,6 +214,11 @@ int pm_set_ddu(pm_fem* f, const double* ddu) {
 }
 
 int pm_get_u(const pm_fem* f, double* u, double* lambda) {
+  if (f->released) {
+    std::copy(f->rel_u.begin(), f->rel_u.end(), u);
+    if (lambda) std::copy(f->rel_lam.begin(), f->rel_lam.end(), lambda);
+    return 0;
+  }
   if (!f->initialized) return 1;
   if (hipMemcpyAsync(u, f->d_u, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
   if (lambda && f->nprescribed &&
@@ -225,6 +233,7 @@ int pm_get_f_int(const pm_fem* f, double* out) {
 }
 
 int pm_get_ip_state(const pm_fem* f, double* v) {
+  if (f->released) { std::copy(f->rel_ip.begin(), f->rel_ip.end(), v); return 0; }
   if (!f->initialized) return 1;
   return copy_ip_state_to_host(f, v);
 }

@@ -67,6 +67,10 @@ struct Timers { double update = 0, residual = 0, tangent = 0, scatter = 0, linea
 
 }  // namespace pm
 
+// measured on MI355X: one workgroup wins below ~1.5k DOF (7-10 us per iteration), the graph-replayed
+// multi-workgroup iteration (17-22 us, launch-bound up to 50k DOF) above
+constexpr int64_t kPcgSmallMaxDefault = 1536;
+
 struct pm_fem {
   // mesh
   int ndim = 2;
@@ -109,6 +113,8 @@ struct pm_fem {
   int64_t lds_per_cu = 160 * 1024;     // LDS bytes per compute unit
   hipStream_t stream = nullptr;
   bool initialized = false;
+  bool released = false;               // device buffers freed after a solve; results kept below
+  std::vector<double> rel_u, rel_lam, rel_ip;
   double* d_X = nullptr;               // point coordinates
   int32_t* d_browptr = nullptr;
   int32_t* d_bcols = nullptr;
@@ -140,7 +146,7 @@ struct pm_fem {
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags
-  int64_t pcg_small_max = 16384;       // systems up to this many DOFs use the single-workgroup PCG
+  int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
   hipGraphExec_t pass_graph[8] = {};   // captured element passes, by (update, residual, tangent)
   hipGraphExec_t pcg_graph = nullptr;  // captured chunk of PCG iterations
   const double* pcg_graph_x = nullptr; // solution vector the chunk was captured for

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "pm_internal.hpp"
@@ -188,6 +189,10 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
       if (!std::isfinite(norm_rr)) { f->err = "Found NaN or Inf"; return 1; }
       bool conv_rr = norm_rr < cfg.tol_rr_abs;
       if (!conv_rr && cfg.tol_rr_scaled > 0.0) conv_rr = norm_rr <= cfg.tol_rr_scaled * dev_max_abs(f, f->d_fint, n, nullptr);
+      if (cfg.verbose & 2) {
+        std::fprintf(stderr, "  step %d t=%g it %d max|R|=%.3e\n", timestep, f->t, it, norm_rr);
+        std::fflush(stderr);
+      }
       if (conv_rr) { converged = true; break; }
       if (it == 0 || !cfg.constant_tangent) {
         for (auto& gr : f->groups) launch_tangent(f, gr);
@@ -213,6 +218,10 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
         if (!std::isfinite(rel)) { f->err = "Found NaN or Inf in mdu"; return 1; }
         conv_mdu = rel < cfg.tol_mdu_rel;
       }
+      if (cfg.verbose & 2) {
+        std::fprintf(stderr, "  step %d t=%g it %d pcg %d%s\n", timestep, f->t, it, pits, conv_mdu ? " (mdu converged)" : "");
+        std::fflush(stderr);
+      }
       if (conv_mdu) { converged = true; break; }
       dev_update_primary(f, f->d_mdu, f->d_y);
       // backup at the first iteration; later iterations update from the backup (= restore)
@@ -225,6 +234,11 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
     if (!converged) { f->err = "Newton-Raphson did not converge"; return 1; }
     (*n_converged)++;
     f->stats.n_steps++;
+    if (cfg.verbose & 1) {
+      std::fprintf(stderr, "step %d t=%g converged: %d Newton iterations in all, %lld PCG iterations in all\n", timestep, f->t,
+                   f->stats.n_iterations, (long long)f->stats.n_pcg_iterations);
+      std::fflush(stderr);
+    }
     bool last = timestep == cfg.n_max_time_steps - 1;
     if ((converged && f->t >= t_out) || last) { if (save_output(f)) return 1; t_out += cfg.dt_out; }
     if (++step_n_failure > cfg.allowed_step_n_failure) { f->err = "too many step failures"; return 1; }

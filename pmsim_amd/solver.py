@@ -26,6 +26,7 @@ EXPORTED = [
     "pm_restore_secondary_values", "pm_update_secondary_values", "pm_assemble_f_int", "pm_assemble_kk", "pm_element_pass",
     "pm_solve", "pm_get_stats", "pm_sizes", "pm_set_ddu", "pm_get_u", "pm_get_f_int", "pm_get_ip_state", "pm_set_ip_state",
     "pm_get_csr", "pm_history_count", "pm_get_history", "pm_time_element_pass", "pm_linear_solve",
+    "pm_estimate_device_bytes", "pm_release_device", "pm_serve",
 ]
 
 
@@ -37,7 +38,14 @@ class PmConfig(C.Structure):
         ("tol_rr_abs", C.c_double), ("tol_mdu_rel", C.c_double), ("symmetry_check_tolerance", C.c_double),
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
         ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
-        ("use_graphs", C.c_int32),
+        ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+class PmRequestStats(C.Structure):
+    _fields_ = [
+        ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("status", C.c_int32),
+        ("n_steps", C.c_int32), ("n_iterations", C.c_int32), ("n_pcg_iterations", C.c_int64), ("device_bytes", C.c_int64),
     ]
 
 
@@ -94,6 +102,9 @@ def lib():
     L.pm_get_history.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_void_p, _dp, C.POINTER(C.c_double)]
     L.pm_time_element_pass.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.pm_linear_solve.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32)]
+    L.pm_estimate_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.pm_release_device.argtypes = [C.c_void_p]
+    L.pm_serve.argtypes = [C.c_int32, C.POINTER(C.c_void_p), _dp, C.c_int32, C.c_int64, C.c_int32, C.c_void_p]
     _lib = L
     return L
 
@@ -127,13 +138,16 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
         c.tol_rr_scaled = cfg.scaled_rr
         c.tangent_kernel = TANGENT_KERNEL[cfg.tangent_kernel]
         c.use_graphs = 1 if cfg.use_graphs else 0
+        c.pcg_small_max = int(cfg.pcg_small_max)
+        c.verbose = (1 if cfg.verbose_timesteps else 0) | (2 if cfg.verbose_iterations else 0)
     return c
 
 
 class Fem:
     """Owns one pm_fem handle built from a flat problem description (``problem.describe``)."""
 
-    def __init__(self, d: dict, cfg: Optional[Config] = None, device: int = 0, keep_history=True, pm_config=None):
+    def __init__(self, d: dict, cfg: Optional[Config] = None, device: int = 0, keep_history=True, pm_config=None,
+                 initialize=True):
         L = lib()
         self.d = d
         h = C.c_void_p()
@@ -159,10 +173,23 @@ class Fem:
         for (fk, pts, cell, v) in d["facets"]:
             p = np.asarray(pts, dtype=np.int32)
             self._check(L.pm_add_facet_qn(h, int(fk), len(p), p, int(cell), float(v)))
-        self._check(L.pm_initialize(h, device))
+        if initialize:
+            self._check(L.pm_initialize(h, device))
+            self.refresh_sizes()
+
+    def refresh_sizes(self):
         sz = np.zeros(6, dtype=np.int64)
-        self._check(L.pm_sizes(h, sz))
+        self._check(lib().pm_sizes(self.h, sz))
         self.ndof, self.nprescribed, self.nip, self.ns, self.nnzb, self.ndim = (int(v) for v in sz)
+
+    def estimate_device_bytes(self):
+        b = C.c_int64(0)
+        self._check(lib().pm_estimate_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def release_device(self):
+        """Keep u, lambda and the IP state on the host and free the handle's device memory."""
+        self._check(lib().pm_release_device(self.h))
 
     def close(self):
         if getattr(self, "h", None):
@@ -286,3 +313,23 @@ class SolverImplicit:
             if state.history:
                 state.t = state.history[-1][0]
         return n
+
+
+def serve(fems, arrivals, device=0, hbm_budget_bytes=200 << 30, max_active=16):
+    """Continuous batching of independent solves (pm_serve).  `fems` are handles built with
+    initialize=False; request i arrives `arrivals[i]` seconds after the call.  Returns per-request
+    dicts (arrival / admit / done times, status, iteration counts, footprint estimate)."""
+    n = len(fems)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    hs = (C.c_void_p * n)(*[f.h for f in fems])
+    out = (PmRequestStats * n)()
+    if lib().pm_serve(n, hs, arr, int(device), int(hbm_budget_bytes), int(max_active), C.cast(out, C.c_void_p)) != 0:
+        raise PmError(lib().pm_global_last_error().decode())
+    res = []
+    for f, o in zip(fems, out):
+        r = {k: getattr(o, k) for (k, _) in PmRequestStats._fields_}
+        r["error"] = lib().pm_last_error(f.h).decode() if o.status != 0 else ""
+        if o.status == 0:
+            f.refresh_sizes()
+        res.append(r)
+    return res

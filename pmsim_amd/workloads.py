@@ -1,4 +1,6 @@
 """Benchmark workloads (mesh + material + boundary conditions + increments)."""
+import numpy as np
+
 from .generators import hex20_block, qua8_block, mixed_2d
 from .mesh import At, Features
 from .problem import Config, Dof, Elem, Essential, FemBase, Natural, ParamSolid, StressStrain
@@ -53,3 +55,24 @@ def mixed_compression(n_elements=200_000, n_inc=8, uy=-0.02, lx=100.0, ly=100.0,
     cfg = Config(mesh)
     cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
     return mesh, base, cfg, ess, Natural()
+
+
+def footing_request(rng, nx_range=(8, 256), ny_range=(4, 128)):
+    """One parametric strip-footing solve (10 m x 5 m, Qua8, 3x3 IP): nx, ny log-uniform in their ranges,
+    E in [5e3, 5e4], nu in [0.2, 0.4], z0 in [50, 500], footing width B in [0.5, 3] m, 5-40 increments
+    of a footing settlement of 0.05 m.  Returns (problem, parameters)."""
+    def logu(lo, hi):
+        return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+    p = dict(nx=logu(*nx_range), ny=logu(*ny_range), young=float(rng.uniform(5e3, 5e4)),
+             poisson=float(rng.uniform(0.2, 0.4)), z_ini=float(rng.uniform(50.0, 500.0)),
+             width=float(rng.uniform(0.5, 3.0)), n_inc=int(rng.integers(5, 41)))
+    # the footing covers at least one element edge (10/nx), so coarse meshes still carry the load
+    p["width_used"] = max(p["width"], 10.0 / p["nx"])
+    prob = strip_footing(p["nx"], p["ny"], ngauss=9, n_inc=p["n_inc"], young=p["young"], poisson=p["poisson"],
+                         z_ini=p["z_ini"], width=p["width_used"], uy=-0.05)
+    return prob, p
+
+
+def poisson_arrivals(rng, n, rate):
+    """Arrival times (s) of n requests from a Poisson process of the given rate (requests/s)."""
+    return np.cumsum(rng.exponential(1.0 / rate, n))

@@ -228,3 +228,35 @@ def test_hex20_full_size_properties():
     assert abs(v @ (K @ w) - w @ (K @ v)) <= 1e-11 * abs(v @ (K @ w))
     f.element_pass()
     assert np.array_equal(f.get_csr()[2], vals) and np.array_equal(f.get_f_int(), fint)
+
+
+def test_hex20_full_size_consistent_tangent():
+    """64x64x32 Hex20 after a partly plastic increment (64% of the IPs yield): K v against central
+    differences of f_int (return map from the same backup state) in a random direction.  The error is
+    set by the few IPs that switch between elastic and plastic within +-eps v; a wrong tangent term would
+    show as an O(1) error."""
+    import scipy.sparse as sp
+    mesh = G.hex20_block(64, 64, 32, seed=20261203, jitter=0.05)
+    base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
+    d = describe(mesh, base, Config(mesh), Essential(), Natural())
+    f = Fem(d, keep_history=False)
+    rng = np.random.default_rng(7)
+    X = mesh.coords
+    u0 = np.zeros_like(X)
+    u0[:, 2] = -2.5e-3 * X[:, 0] * X[:, 2]
+    u0 = u0.ravel() + rng.uniform(-2e-6, 2e-6, X.size)
+    f.backup_secondary_values()
+    f.set_ddu(u0)
+    f.element_pass()
+    assert 0.5 < np.mean(f.get_ip_state()[:, f.ns + 2] == 0.0) < 0.8
+    rp, cols, vals = f.get_csr()
+    Kv = sp.csr_matrix((vals, cols, rp), shape=(f.ndof, f.ndof)) @ (v := rng.normal(size=f.ndof) * 1e-6)
+    eps = 1e-4
+    fs = []
+    for sgn in (1.0, -1.0):
+        f.set_ddu(u0 + sgn * eps * v)
+        f.update_secondary_values()
+        f.assemble_f_int()
+        fs.append(f.get_f_int())
+    fd = (fs[0] - fs[1]) / (2 * eps)
+    assert np.linalg.norm(fd - Kv) / np.linalg.norm(Kv) < 5e-4

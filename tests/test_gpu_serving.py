@@ -1,0 +1,85 @@
+"""Serving: many independent footing solves admitted concurrently on one GPU (pm_serve).
+
+Each request's result must be bitwise the same whether it runs alone or in the mix (its kernels, their
+order and its stream are its own), admission must keep the sum of running footprints within the HBM
+budget, a failing request must not disturb the others, and served solutions must match the oracle.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import rel_err
+from oracle import oracle as orc
+from pmsim_amd import describe
+from pmsim_amd.solver import Fem, serve
+from pmsim_amd.workloads import footing_request
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed, nx=(8, 48), ny=(4, 24)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        (mesh, base, cfg, ess, nat), p = footing_request(rng, nx, ny)
+        cfg.set_scaled_rr(1e-10).set_pcg(1e-12, 50000)
+        out.append((describe(mesh, base, cfg, ess, nat), cfg, p))
+    return out
+
+
+def _handles(reqs, keep_history=False):
+    return [Fem(d, cfg, keep_history=keep_history, initialize=False) for d, cfg, _ in reqs]
+
+
+def test_served_results_same_alone_and_in_the_mix():
+    reqs = _requests(10, 7)
+    mix = _handles(reqs)
+    res = serve(mix, np.zeros(len(mix)), max_active=8)
+    assert all(r["status"] == 0 for r in res), [r["error"] for r in res]
+    admits = sorted(r["t_admit"] for r in res)
+    assert admits[7] < min(r["t_done"] for r in res)  # the first 8 really ran together
+    for k, (d, cfg, p) in enumerate(reqs):
+        alone = _handles([reqs[k]])
+        r = serve(alone, [0.0], max_active=1)[0]
+        assert r["status"] == 0
+        assert r["n_iterations"] == res[k]["n_iterations"] and r["n_pcg_iterations"] == res[k]["n_pcg_iterations"]
+        (ua, la), (um, lm) = alone[0].get_u(), mix[k].get_u()
+        assert np.array_equal(ua, um) and np.array_equal(la, lm)
+        assert np.array_equal(alone[0].get_ip_state(), mix[k].get_ip_state())
+
+
+def test_admission_respects_hbm_budget():
+    reqs = _requests(12, 11)
+    hs = _handles(reqs)
+    est = [h.estimate_device_bytes() for h in hs]
+    budget = int(2.2 * max(est))
+    res = serve(hs, np.linspace(0.0, 0.05, len(hs)), hbm_budget_bytes=budget, max_active=12)
+    assert all(r["status"] == 0 for r in res)
+    for r in res:  # at every admission, the running footprints (including the admitted one) fit
+        t = r["t_admit"]
+        running = sum(q["device_bytes"] for q in res if q["t_admit"] <= t < q["t_done"])
+        assert running <= budget
+        assert r["t_admit"] >= r["t_arrival"] - 1e-6
+
+
+def test_failing_request_is_isolated():
+    reqs = _requests(4, 13)
+    hs = _handles(reqs)
+    bad = reqs[1][0].copy()
+    bad["params"] = {a: (1, E, nu, H, 0.0, ng) for a, (m, E, nu, H, z0, ng) in bad["params"].items()}  # z_ini = 0
+    hs[1] = Fem(bad, reqs[1][1], keep_history=False, initialize=False)
+    res = serve(hs, np.zeros(4), max_active=4)
+    assert res[1]["status"] != 0 and "yield surface" in res[1]["error"]
+    assert all(res[k]["status"] == 0 for k in (0, 2, 3))
+
+
+def test_served_solutions_match_oracle():
+    reqs = _requests(3, 17, nx=(8, 24), ny=(4, 12))
+    hs = _handles(reqs, keep_history=True)
+    res = serve(hs, [0.0, 0.01, 0.02], max_active=3)
+    for (d, cfg, p), h, r in zip(reqs, hs, res):
+        assert r["status"] == 0
+        uo, ipo, _ = orc.Oracle(d).solve()
+        hist = h.history()
+        assert len(hist) == len(uo)
+        assert rel_err(hist[-1][1], uo[-1][: h.ndof]) < 1e-9
+        assert rel_err(h.get_u()[0], uo[-1][: h.ndof]) < 1e-9
