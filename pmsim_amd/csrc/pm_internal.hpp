@@ -38,6 +38,8 @@ struct ParamSolid {
 };
 
 // Per-kind element batch on the device.
+struct PcgEngine;
+
 struct Group {
   int kind = 0, ndim = 2, nnode = 0, nip = 0, attribute = 0;
   ParamSolid param;
@@ -147,6 +149,7 @@ struct pm_fem {
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags
   int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
+  pm::PcgEngine* engine = nullptr;     // batched PCG of a serving run (set by pm_serve), else per handle
   hipGraphExec_t pass_graph[8] = {};   // captured element passes, by (update, residual, tangent)
   hipGraphExec_t pcg_graph = nullptr;  // captured chunk of PCG iterations
   const double* pcg_graph_x = nullptr; // solution vector the chunk was captured for
@@ -182,6 +185,10 @@ bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 int tune_tangent(pm_fem* f);
 // drops every captured graph (configuration, tuning or buffers changed)
 void release_graphs(pm_fem* f);
+// batched PCG engine shared by the requests of one serving run
+PcgEngine* engine_create(int device);
+void engine_destroy(PcgEngine* e);
+int engine_solve(PcgEngine* e, pm_fem* f, const double* d_b, double* d_x, int* iters);
 void launch_scatter(pm_fem* f);
 void launch_init_state(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);

@@ -520,6 +520,7 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
     f->err = "PCG did not converge";
     return 2;
   }
+  if (f->engine) return engine_solve(f->engine, f, d_b, d_x, iters);  // batched with the other served requests
   // grids: rows for the SpMV (rbq), nodes for the vector updates (rb); partial arrays sized accordingly
   const int rb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + PCG_TPB - 1) / PCG_TPB));
   const int rbq = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + PCG_TPB - 1) / PCG_TPB));

@@ -85,6 +85,11 @@ int pm_serve(int32_t n, pm_fem* const* fems, const double* arrival_s, int32_t de
     if (!fems[i] || pm_estimate_device_bytes(fems[i], &bytes[i])) return 1;
     out[i].device_bytes = bytes[i];
   }
+  // one batched PCG engine for the run: every served request's multi-workgroup PCG iterations go
+  // through it (systems small enough for the single-workgroup PCG keep that one-launch path)
+  PcgEngine* engine = engine_create(device);
+  if (!engine) return 1;
+  for (int i = 0; i < n; i++) fems[i]->engine = engine;
   std::mutex m;
   std::condition_variable cv;
   int active = 0, done = 0;
@@ -133,6 +138,8 @@ int pm_serve(int32_t n, pm_fem* const* fems, const double* arrival_s, int32_t de
   }
   lk.unlock();
   for (auto& w : workers) w.join();
+  engine_destroy(engine);
+  for (int i = 0; i < n; i++) fems[i]->engine = nullptr;
   return 0;
 }
 

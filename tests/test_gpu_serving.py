@@ -83,3 +83,19 @@ def test_served_solutions_match_oracle():
         assert len(hist) == len(uo)
         assert rel_err(hist[-1][1], uo[-1][: h.ndof]) < 1e-9
         assert rel_err(h.get_u()[0], uo[-1][: h.ndof]) < 1e-9
+
+
+def test_batched_pcg_matches_standalone_solve():
+    """Requests large enough for the multi-workgroup PCG go through the batched PCG engine when served;
+    their iterates are the single-request ones, so served results equal pm_solve's bit for bit."""
+    reqs = _requests(6, 19, nx=(40, 80), ny=(16, 40))
+    hs = _handles(reqs)
+    res = serve(hs, np.zeros(len(hs)), max_active=6)
+    assert all(r["status"] == 0 for r in res)
+    for (d, cfg, p), h, r in zip(reqs, hs, res):
+        assert h.ndof > 1536  # past the single-workgroup PCG
+        f = Fem(d, cfg, keep_history=False)
+        f.solve()
+        assert f.stats()["n_pcg_iterations"] == r["n_pcg_iterations"]
+        assert np.array_equal(f.get_u()[0], h.get_u()[0])
+        assert np.array_equal(f.get_ip_state(), h.get_ip_state())
