@@ -26,6 +26,8 @@ KIND_NAMES = {
 }
 NNODE = {TRI3: 3, TRI6: 6, TRI10: 10, TRI15: 15, QUA4: 4, QUA8: 8, QUA9: 9, TET4: 4, TET10: 10, HEX8: 8, HEX20: 20}
 NDIM_OF = {k: (2 if k <= QUA9 else 3) for k in NNODE}
+# default number of integration points per kind (the rules used when ParamSolid.ngauss is None)
+DEFAULT_NGAUSS = {TRI3: 1, TRI6: 3, TRI10: 6, TRI15: 12, QUA4: 4, QUA8: 9, QUA9: 9, TET4: 1, TET10: 4, HEX8: 8, HEX20: 27}
 
 # local edges of 2D cells: [end0, end1, interior nodes ordered from end0 to end1]
 EDGES_2D = {

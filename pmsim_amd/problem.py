@@ -288,6 +288,7 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
     return dict(
         ndim=mesh.ndim, coords=mesh.coords, kinds=mesh.kinds, attrs=mesh.attrs,
         cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts, params=params,
+        densities={a: float(p.density) for a, p in base.amap.items()},
         axisym=config.axisymmetric, plane_stress=config.plane_stress, thickness=config.thickness,
         lmm=config.lagrange_mult_method, linear_problem=config.linear_problem,
         t_ini=config.t_ini, t_fin=config.t_fin, dt=dt, n_max_iterations=config.n_max_iterations,

@@ -14,6 +14,7 @@ from typing import List, Optional
 import numpy as np
 
 from .problem import Config, Essential, FemBase, Natural, describe
+from . import mesh as M
 from .mesh import Mesh
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -300,11 +301,33 @@ class SolverImplicit:
         if not config.lagrange_mult_method and essential.has_non_zero():
             # checked again by the library at solve time; raised here with pmsim's text for early failure
             pass
+        self.mesh = mesh
         self.desc = describe(mesh, base, config, essential, natural)
         self.fem = Fem(self.desc, config, device=device, keep_history=keep_history)
 
-    def solve(self, state: Optional[FemState] = None) -> int:
+    def cell_layout(self):
+        """IPs and stress-strain model of every cell (cell order), as pmsim's state files hold them."""
+        d = self.desc
+        nip = [int(d["params"][int(a)][5]) or M.DEFAULT_NGAUSS[int(k)] for k, a in zip(d["kinds"], d["attrs"])]
+        model = [int(d["params"][int(a)][0]) for a in d["attrs"]]
+        return nip, model
+
+    def write_files(self, file_io):
+        """Mesh, base, one state file per output step and the summary, in pmsim's formats."""
+        from .file_io import state_dict
+        file_io.write_setup(self.mesh, self.desc)
+        nip, model = self.cell_layout()
+        prev = None
+        for (t, u, lam, ip) in self.fem.history():
+            ddu = u - prev if prev is not None else np.zeros_like(u)
+            file_io.write_state(state_dict(self.mesh.ndim, t, float(self.desc["dt"]), u, ddu, ip, nip, model))
+            prev = u
+        file_io.write_summary()
+
+    def solve(self, state: Optional[FemState] = None, file_io=None) -> int:
         n = self.fem.solve()
+        if file_io is not None:
+            self.write_files(file_io)
         if state is not None:
             state.history = self.fem.history()
             u, lam = self.fem.get_u()

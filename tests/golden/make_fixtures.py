@@ -12,6 +12,8 @@ Outputs (data only: inputs and expected outputs):
 * smith_meshes.json -- point coordinates and cell connectivity of the Smith sample meshes
                      (src/base/sample_meshes.rs:261-735)
 * smith_u.json    -- expected displacement vectors of tests/test_solid_smith_*.rs
+* artificial/     -- state, summary, base and mesh files written by pmsim itself for three small linear
+                     elastic runs (data/results/artificial/): the on-disk formats' known answers
 """
 import gzip
 import json
@@ -77,4 +79,10 @@ if __name__ == "__main__":
     shutil.copy(os.path.join(REF, "data/sgm/sgm_5d17_ref.json"), os.path.join(OUT, "sgm_5d17_ref.json"))
     meshes()
     smith_vectors()
+    art = os.path.join(OUT, "artificial")
+    os.makedirs(art, exist_ok=True)
+    for n in sorted(os.listdir(os.path.join(REF, "data/results/artificial"))):
+        if n.startswith("artificial-elastic-2d-") or n.startswith("artificial-elastic-3d-"):
+            shutil.copy(os.path.join(REF, "data/results/artificial", n), os.path.join(art, n))
+            os.chmod(os.path.join(art, n), 0o644)
     print("fixtures written to", OUT)
