@@ -26,12 +26,12 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense FP64 /
 FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
 
 
-def build_problem(nx, ny, nz, precision="f64", kernel="default"):
+def build_problem(nx, ny, nz, precision="f64", kernel="default", scatter="blocks"):
     from pmsim_amd import Config, Essential, FemBase, Natural, ParamSolid, StressStrain, describe
     from pmsim_amd import generators as G
     mesh = G.hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=20261203, jitter=0.05)
     base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
-    cfg = Config(mesh).set_tangent_precision(precision).set_tangent_kernel(kernel)
+    cfg = Config(mesh).set_tangent_precision(precision).set_tangent_kernel(kernel).set_scatter_kernel(scatter)
     return mesh, base, cfg, describe(mesh, base, cfg, Essential(), Natural())
 
 
@@ -191,9 +191,20 @@ def run_precision(a):
         f = Fem(d, cfg, keep_history=False)
         print(f"{prec}: handle ready ({f.ndof} DOFs, {f.nnzb} K blocks), {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
         t1 = time.time()
-        f.solve()
+        from pmsim_amd.solver import PmError
+        try:
+            f.solve()
+            failure = None
+        except PmError as e:  # reported, not fatal: a reduced-precision tangent may stall Newton or PCG
+            failure = str(e)
         secs = time.time() - t1
         st = f.stats()
+        if failure is not None:
+            out["runs"][prec] = {"failed": failure, "seconds": secs, "converged_steps": st["n_steps"],
+                                 "newton_iterations": st["n_iterations"], "pcg_iterations": st["n_pcg_iterations"]}
+            print(f"{prec}: {json.dumps(out['runs'][prec])}", file=sys.stderr, flush=True)
+            f.close()
+            continue
         u = f.get_u()[0]
         ip = f.get_ip_state()
         run = {"seconds": secs, "steps": st["n_steps"], "newton_iterations": st["n_iterations"],
@@ -227,6 +238,7 @@ def main():
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
+    ap.add_argument("--scatter", default="blocks", choices=["blocks", "rows"], help="K gather: by CSR blocks or by rows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision"])
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
@@ -253,7 +265,7 @@ def main():
         return run_serve(a)
     from pmsim_amd.solver import Fem
     t0 = time.time()
-    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel)
+    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel, a.scatter)
     f = Fem(d, cfg, keep_history=False)
     setup_s = time.time() - t0
     print(f"setup: {mesh.ncell} cells, {f.nip} IPs, {f.ndof} DOFs, {f.nnzb} blocks, {setup_s:.1f} s", file=sys.stderr, flush=True)

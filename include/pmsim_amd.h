@@ -54,6 +54,8 @@ typedef struct pm_config {
   int32_t pcg_small_max;            /* systems with at most this many DOFs run the whole PCG in one workgroup; < 0: default */
   int32_t verbose;                  /* to stderr; bit 0: a line per converged step, bit 1: a line per Newton iteration
                                        (pmsim's verbose_timesteps / verbose_iterations, config.rs:166-173; default 0) */
+  int32_t scatter_kernel;           /* K gather: 0 (default) one thread per CSR block, 1 one wave per CSR row reading
+                                       whole element block rows; both sum in ascending cell order (same bits) */
 } pm_config;
 
 /* Statistics of the last pm_solve / element pass. */
@@ -90,11 +92,16 @@ int pm_set_config(pm_fem* fem, const pm_config* cfg);
  * stations t_k = t_ini + k*dt, table is n x nstation row-major (value = c * m(t_k)). */
 int pm_set_essential(pm_fem* fem, int64_t n, const int32_t* points, const int32_t* dofs,
                      const double* table, int32_t nstation);
-/* Natural point loads (src/fem/bc_concentrated.rs:46-55). */
-int pm_set_point_loads(pm_fem* fem, int64_t n, const int32_t* points, const int32_t* dofs, const double* values);
-/* Natural normal distributed load Qn on one boundary facet (src/fem/bc_distributed.rs:166-174). */
+/* Natural point loads (src/fem/bc_concentrated.rs:46-55; Natural::points / points_fn,
+ * src/base/natural.rs:92-123): table is n x nstation row-major with the values at the time stations
+ * t_k (nstation = 1: constant in time). */
+int pm_set_point_loads(pm_fem* fem, int64_t n, const int32_t* points, const int32_t* dofs, const double* table,
+                       int32_t nstation);
+/* Natural normal distributed load Qn on one boundary facet (src/fem/bc_distributed.rs:166-174;
+ * Natural::edge(s)/face(s) and their _fn forms, src/base/natural.rs:56-140): values at the time
+ * stations (nstation = 1: constant). */
 int pm_add_facet_qn(pm_fem* fem, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell,
-                    double value);
+                    const double* values, int32_t nstation);
 
 /* Builds element batches, the node-block CSR pattern, the deterministic scatter maps and all
  * device buffers (replaces SolverImplicit::new / SolverData::new / LinearSystem::new,

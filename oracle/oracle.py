@@ -40,8 +40,8 @@ def lib():
         L.orc_set_ideal.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
         L.orc_set_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]
         L.orc_set_essential.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int]
-        L.orc_set_point_loads.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp]
-        L.orc_add_facet_qn.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int, C.c_double]
+        L.orc_set_point_loads.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int]
+        L.orc_add_facet_qn.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int]
         L.orc_setup.argtypes = [C.c_void_p]
         L.orc_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, C.c_int]
@@ -87,12 +87,13 @@ class Oracle:
         L.orc_set_essential(self.h, n, np.ascontiguousarray(d["ess_points"], dtype=np.int32),
                             np.ascontiguousarray(d["ess_dofs"], dtype=np.int32), tab.ravel() if n else np.zeros(1),
                             tab.shape[1] if n else 1)
-        npl = len(d["pl_values"])
-        if npl:
-            L.orc_set_point_loads(self.h, npl, d["pl_points"], d["pl_dofs"], d["pl_values"])
+        plt = np.ascontiguousarray(d["pl_table"], dtype=np.float64)
+        if plt.shape[0]:
+            L.orc_set_point_loads(self.h, plt.shape[0], d["pl_points"], d["pl_dofs"], plt.ravel(), plt.shape[1])
         for (fk, pts, cell, v) in d["facets"]:
             p = np.asarray(pts, dtype=np.int32)
-            L.orc_add_facet_qn(self.h, int(fk), len(p), p, int(cell), float(v))
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            L.orc_add_facet_qn(self.h, int(fk), len(p), p, int(cell), v, len(v))
         sz = np.zeros(4, dtype=np.int32)
         self._check(L.orc_sizes(self.h, sz))
         self.ndof, self.neq_total, self.nip, self.ns = (int(v) for v in sz)

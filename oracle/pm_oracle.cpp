@@ -385,7 +385,7 @@ const char* stiffness(const Param& p, const Ideal& id, int ns, const IPState& st
 // ---------------------------------------------------------------------------------------------
 // problem
 // ---------------------------------------------------------------------------------------------
-struct Facet { int kind; std::vector<int> pts; int cell; double value; };
+struct Facet { int kind; std::vector<int> pts; int cell; std::vector<double> values; };
 
 struct Problem {
   int ndim = 2, npoint = 0, ncell = 0;
@@ -401,8 +401,9 @@ struct Problem {
   std::vector<int> ess_point, ess_dof;
   std::vector<double> ess_table;
   int nstation = 0;
-  std::vector<int> pl_point, pl_dof; std::vector<double> pl_value;  // concentrated loads
-  std::vector<Facet> facets;                                         // Qn loads
+  std::vector<int> pl_point, pl_dof; std::vector<double> pl_table;  // concentrated loads (n x pl_nstation)
+  int pl_nstation = 1;
+  std::vector<Facet> facets;                                         // Qn loads (values per station)
   // derived
   int ns = 4, ndof = 0, neq_total = 0, nip_total = 0;
   std::vector<int> ip_ptr;            // per-cell first IP index
@@ -418,6 +419,16 @@ struct Problem {
   std::vector<int> iters;
   std::vector<double> norm_rr_hist;
 };
+
+// value of a natural load at time t from its station values (one value: constant), the station
+// being t_k = t_ini + k*dt; pmsim evaluates the load's function at the step's time
+// (src/fem/bc_distributed.rs:161-164, src/fem/bc_concentrated.rs:46-55)
+double load_value(const Problem& P, const double* v, int nst, double t) {
+  int ik = (int)std::llround((t - P.t_ini) / P.dt);
+  if (ik < 0) ik = 0;
+  if (ik >= nst) ik = nst - 1;
+  return v[ik];
+}
 
 double table_value(const Problem& P, int p, double t) {
   double k = (t - P.t_ini) / P.dt;
@@ -627,6 +638,7 @@ const char* calc_K(const Problem& P, int e, double* K) {
 
 // boundary facet Qn load: f = int N qn n dGamma (bc_distributed.rs:166-174)
 void add_facet_load(const Problem& P, const Facet& F, double* ff) {
+  const double value = load_value(P, F.values.data(), (int)F.values.size(), P.t);
   const Basis& b = basis(F.kind);
   int nn = b.nnode, nd = P.ndim;
   std::vector<IP> rule;
@@ -663,7 +675,7 @@ void add_facet_load(const Problem& P, const Facet& F, double* ff) {
       double r = 0; for (int m = 0; m < nn; m++) r += N[m] * P.X[(size_t)F.pts[m] * 2];
       w *= r;
     }
-    for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) ff[(size_t)F.pts[m] * nd + i] += N[m] * F.value * sgn * n[i] * w;
+    for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) ff[(size_t)F.pts[m] * nd + i] += N[m] * value * sgn * n[i] * w;
   }
 }
 
@@ -677,33 +689,63 @@ struct Band {
   double& at(int i, int j) { return a[(size_t)i * w + (j - i + kl)]; }
 };
 
-bool band_lu_solve(Band& B, std::vector<double>& x) {
+// factorization in place: U in the upper band, the multipliers of column k below its diagonal, row
+// interchanges in piv (applied to the right-hand side in elimination order, as LAPACK's gbtrf/gbtrs)
+bool band_lu_factor(Band& B) {
   int n = B.n, kl = B.kl;
   int ucols = B.ku + kl;  // upper extent after pivoting
+  B.piv.assign(n, 0);
   for (int k = 0; k < n; k++) {
     int last = std::min(n - 1, k + kl);
     int p = k; double best = std::fabs(B.at(k, k));
     for (int i = k + 1; i <= last; i++) if (std::fabs(B.at(i, k)) > best) { best = std::fabs(B.at(i, k)); p = i; }
     if (best == 0.0) return false;
+    B.piv[k] = p;
     int jend = std::min(n - 1, k + ucols);
-    if (p != k) {
-      for (int j = k; j <= jend; j++) std::swap(B.at(k, j), B.at(p, j));
-      std::swap(x[k], x[p]);
-    }
+    if (p != k) for (int j = k; j <= jend; j++) std::swap(B.at(k, j), B.at(p, j));
     double d = B.at(k, k);
     for (int i = k + 1; i <= last; i++) {
       double f = B.at(i, k) / d;
+      B.at(i, k) = f;
       if (f == 0.0) continue;
-      B.at(i, k) = 0.0;
       for (int j = k + 1; j <= jend; j++) B.at(i, j) -= f * B.at(k, j);
-      x[i] -= f * x[k];
     }
+  }
+  return true;
+}
+
+void band_lu_apply(Band& B, std::vector<double>& x) {
+  int n = B.n, kl = B.kl, ucols = B.ku + kl;
+  for (int k = 0; k < n; k++) {
+    if (B.piv[k] != k) std::swap(x[k], x[B.piv[k]]);
+    int last = std::min(n - 1, k + kl);
+    for (int i = k + 1; i <= last; i++) x[i] -= B.at(i, k) * x[k];
   }
   for (int k = n - 1; k >= 0; k--) {
     double s = x[k];
     int jend = std::min(n - 1, k + ucols);
     for (int j = k + 1; j <= jend; j++) s -= B.at(k, j) * x[j];
     x[k] = s / B.at(k, k);
+  }
+}
+
+// direct solve with two steps of iterative refinement against the unfactored matrix A (as UMFPACK's
+// default, UMFPACK_IRSTEP = 2, used by pmsim's default Genie; src/fem/linear_system.rs:185-201)
+bool band_lu_solve(Band& B, std::vector<double>& x) {
+  const Band A = B;
+  const std::vector<double> b = x;
+  if (!band_lu_factor(B)) return false;
+  band_lu_apply(B, x);
+  std::vector<double> r(B.n);
+  for (int step = 0; step < 2; step++) {
+    for (int i = 0; i < A.n; i++) {
+      double s = b[i];
+      const int j0 = std::max(0, i - A.kl), j1 = std::min(A.n - 1, i + A.kl);
+      for (int j = j0; j <= j1; j++) s -= A.a[(size_t)i * A.w + (j - i + A.kl)] * x[j];
+      r[i] = s;
+    }
+    band_lu_apply(B, r);
+    for (int i = 0; i < A.n; i++) x[i] += r[i];
   }
   return true;
 }
@@ -842,7 +884,8 @@ const char* solve(Problem& P, Out& o) {
         std::vector<double> fx(P.ndof, 0.0);
         for (auto& F : P.facets) add_facet_load(P, F, fx.data());
         for (int g = 0; g < P.ndof; g++) if (!P.ignore[g]) ff_ext[g] += fx[g];
-        for (size_t k = 0; k < P.pl_point.size(); k++) ff_ext[nd * P.pl_point[k] + P.pl_dof[k]] += P.pl_value[k];
+        for (size_t k = 0; k < P.pl_point.size(); k++)
+          ff_ext[nd * P.pl_point[k] + P.pl_dof[k]] += load_value(P, &P.pl_table[k * P.pl_nstation], P.pl_nstation, P.t);
       }
       for (int i = 0; i < nt; i++) rr[i] = ff_int[i] - ell * ff_ext[i];
       if (P.lmm) for (size_t p = 0; p < P.prescribed_eq.size(); p++) {
@@ -972,15 +1015,16 @@ int orc_set_essential(void* h, int n, const int* pts, const int* dofs, const dou
   return 0;
 }
 
-int orc_set_point_loads(void* h, int n, const int* pts, const int* dofs, const double* values) {
+int orc_set_point_loads(void* h, int n, const int* pts, const int* dofs, const double* table, int nstation) {
   auto* P = (Problem*)h;
-  P->pl_point.assign(pts, pts + n); P->pl_dof.assign(dofs, dofs + n); P->pl_value.assign(values, values + n);
+  P->pl_point.assign(pts, pts + n); P->pl_dof.assign(dofs, dofs + n);
+  P->pl_table.assign(table, table + (size_t)n * nstation); P->pl_nstation = nstation;
   return 0;
 }
 
-int orc_add_facet_qn(void* h, int facet_kind, int npts, const int* pts, int cell, double value) {
+int orc_add_facet_qn(void* h, int facet_kind, int npts, const int* pts, int cell, const double* values, int nstation) {
   auto* P = (Problem*)h;
-  P->facets.push_back({facet_kind, std::vector<int>(pts, pts + npts), cell, value});
+  P->facets.push_back({facet_kind, std::vector<int>(pts, pts + npts), cell, std::vector<double>(values, values + nstation)});
   return 0;
 }
 

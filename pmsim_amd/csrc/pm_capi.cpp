@@ -42,6 +42,7 @@ void pm_default_config(pm_config* c) {
   c->tangent_kernel = 0;
   c->use_graphs = 1;
   c->pcg_small_max = -1;
+  c->scatter_kernel = 0;
 }
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
@@ -89,6 +90,7 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
   if (c->tangent_kernel < 0 || c->tangent_kernel > 3) { f->err = "unknown tangent kernel"; return 1; }
+  if (c->scatter_kernel < 0 || c->scatter_kernel > 1) { f->err = "unknown scatter kernel"; return 1; }
   const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   if (f->initialized) release_graphs(f);  // captured launches depend on the configuration
   f->cfg = *c;
@@ -106,19 +108,28 @@ int pm_set_essential(pm_fem* f, int64_t n, const int32_t* points, const int32_t*
   return 0;
 }
 
-int pm_set_point_loads(pm_fem* f, int64_t n, const int32_t* points, const int32_t* dofs, const double* values) {
+int pm_set_point_loads(pm_fem* f, int64_t n, const int32_t* points, const int32_t* dofs, const double* table, int32_t nstation) {
+  if (f->initialized) { f->err = "natural conditions must be set before pm_initialize"; return 1; }
+  if (nstation < 1) { f->err = "a load table needs at least one station"; return 1; }
   for (int64_t i = 0; i < n; i++)
     if (points[i] < 0 || points[i] >= f->npoint || dofs[i] < 0 || dofs[i] >= f->ndim) { f->err = "cannot find (PointId, Dof) in the DOF map"; return 1; }
   f->pl_point.assign(points, points + n);
   f->pl_dof.assign(dofs, dofs + n);
-  f->pl_value.assign(values, values + n);
+  f->pl_table.assign(table, table + n * (int64_t)nstation);
+  f->pl_nstation = nstation;
+  if (nstation > 1) f->loads_vary = true;
   return 0;
 }
 
-int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell, double value) {
+int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell, const double* values,
+                    int32_t nstation) {
+  if (f->initialized) { f->err = "natural conditions must be set before pm_initialize"; return 1; }
   if (cell < 0 || cell >= f->ncell) { f->err = "facet owner cell out of range"; return 1; }
   if (kind_nnode(facet_kind) != npts) { f->err = "wrong number of points for facet kind"; return 1; }
-  f->facets.push_back(Facet{facet_kind, std::vector<int32_t>(points, points + npts), cell, value});
+  if (nstation < 1) { f->err = "a load table needs at least one station"; return 1; }
+  f->facets.push_back(Facet{facet_kind, std::vector<int32_t>(points, points + npts), cell,
+                            std::vector<double>(values, values + nstation)});
+  if (nstation > 1) f->loads_vary = true;
   return 0;
 }
 
@@ -130,7 +141,7 @@ int pm_initialize(pm_fem* f, int32_t device) {
   if (device < 0 || device >= ndev) { f->err = "invalid HIP device"; return 1; }
   f->device = device;
   if (build_groups(f) || build_patterns(f)) return 1;
-  assemble_f_ext(f);
+  assemble_f_ext(f, 0);
   if (device_alloc(f)) { device_free(f); return 1; }
   f->initialized = true;
   return tune_tangent(f);

@@ -182,6 +182,48 @@ int build_patterns(pm_fem* f) {
   // upload happens in device_alloc; keep host copies in temporaries stored on the handle
   f->h_kmap_ptr.swap(kptr);
   f->h_kmap.swap(kmap);
+  // K gather by rows: node a's contributions are its cells in ascending order (nptr / ncells); the
+  // cell's block row of a is contiguous in rank order, and ranks follow global ids like the CSR columns,
+  // so the positions come from one merge of the two sorted lists
+  {
+    const int64_t nc = nptr[np];
+    f->h_rsrc.assign(nc, 0);
+    f->h_rposoff.assign(nc, 0);
+    f->h_rnn.assign(nc, 0);
+    std::vector<int64_t> poff(nc + 1, 0);
+    int32_t maxrow = 0;
+    for (int64_t a = 0; a < np; a++) {
+      maxrow = std::max(maxrow, f->h_browptr[a + 1] - f->h_browptr[a]);
+      for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) poff[k + 1] = poff[k] + (f->cptr[ncells[k] + 1] - f->cptr[ncells[k]]);
+    }
+    f->max_row_blocks = maxrow;
+    if (poff[nc] >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
+    f->h_rpos.assign(poff[nc], 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t a = 0; a < np; a++) {
+      const int32_t* cols = f->h_bcols.data() + f->h_browptr[a];
+      const int32_t len = f->h_browptr[a + 1] - f->h_browptr[a];
+      std::vector<int32_t> sorted;
+      for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) {
+        const int64_t e = ncells[k];
+        const Group& g = f->groups[f->cell_group[e]];
+        const int64_t n = g.nnode, el = f->cell_local[e];
+        int rk[32];
+        cell_ranks(f, e, rk);
+        f->h_rsrc[k] = (int32_t)(g.ke_base + el * g.nblk + (int64_t)rk[nloc[k]] * n);
+        f->h_rposoff[k] = (int32_t)poff[k];
+        f->h_rnn[k] = (uint8_t)n;
+        sorted.assign(f->cpts.begin() + f->cptr[e], f->cpts.begin() + f->cptr[e + 1]);
+        std::sort(sorted.begin(), sorted.end());  // rank order
+        int32_t j = 0;
+        for (int64_t r = 0; r < n; r++) {
+          while (j < len && cols[j] < sorted[r]) j++;
+          f->h_rpos[poff[k] + r] = (uint16_t)j;
+        }
+      }
+    }
+    if (maxrow > 65535) { f->err = "a node couples to more than 65535 nodes"; return 1; }
+  }
   // R gather map
   std::vector<int32_t> fptr(np + 1, 0), fmap(nptr[np]);
   for (int64_t a = 0; a < np; a++) {
@@ -198,14 +240,18 @@ int build_patterns(pm_fem* f) {
   return 0;
 }
 
-// External forces: concentrated loads (bc_concentrated.rs:46-55) and normal distributed loads
-// (bc_distributed.rs:166-174: f = int N qn n dGamma with n the outward normal).
-void assemble_f_ext(pm_fem* f) {
+// External forces at time station k: concentrated loads (bc_concentrated.rs:46-55) and normal
+// distributed loads (bc_distributed.rs:166-174: f = int N qn n dGamma with n the outward normal), each
+// with its value at that station (pmsim evaluates value = f(t) at the step's time, bc_distributed.rs:161-164).
+void assemble_f_ext(pm_fem* f, int64_t k) {
   f->f_ext.assign(f->ndof, 0.0);
-  for (size_t i = 0; i < f->pl_point.size(); i++) f->f_ext[(int64_t)f->ndim * f->pl_point[i] + f->pl_dof[i]] += f->pl_value[i];
+  const int64_t kp = std::min<int64_t>(std::max<int64_t>(k, 0), f->pl_nstation - 1);
+  for (size_t i = 0; i < f->pl_point.size(); i++)
+    f->f_ext[(int64_t)f->ndim * f->pl_point[i] + f->pl_dof[i]] += f->pl_table[i * f->pl_nstation + kp];
   const int nd = f->ndim;
   for (const Facet& F : f->facets) {
     int nn = (int)F.pts.size();
+    const double value = F.values[std::min<int64_t>(std::max<int64_t>(k, 0), (int64_t)F.values.size() - 1)];
     Rule R;
     bool tri = (F.kind == TRI3 || F.kind == TRI6);
     if (F.kind >= LIN2) gauss_rule(F.kind, std::max(3, std::min(4, nn)), R);
@@ -241,7 +287,7 @@ void assemble_f_ext(pm_fem* f) {
         for (int m = 0; m < nn; m++) r += N[m] * f->X[(int64_t)F.pts[m] * 2];
         w *= r;
       }
-      for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) f->f_ext[(int64_t)F.pts[m] * nd + i] += N[m] * F.value * n[i] * w;
+      for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) f->f_ext[(int64_t)F.pts[m] * nd + i] += N[m] * value * n[i] * w;
     }
   }
 }
@@ -332,18 +378,42 @@ int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<doubl
       }
     }
   }
+  auto cholesky_solve = [&](std::vector<double>& y) {  // y: right-hand side in, solution out
+    for (int64_t i = 0; i < n; i++) {
+      double s = y[i];
+      for (int64_t k = first[i]; k < i; k++) s -= at(i, k) * y[k];
+      y[i] = s / at(i, i);
+    }
+    for (int64_t i = n - 1; i >= 0; i--) {
+      y[i] /= at(i, i);
+      for (int64_t k = first[i]; k < i; k++) y[k] -= at(i, k) * y[i];
+    }
+  };
   std::vector<double> y(n);
-  for (int64_t i = 0; i < n; i++) {
-    double s = b[eqs[i]];
-    for (int64_t k = first[i]; k < i; k++) s -= at(i, k) * y[k];
-    y[i] = s / at(i, i);
-  }
-  for (int64_t i = n - 1; i >= 0; i--) {
-    y[i] /= at(i, i);
-    for (int64_t k = first[i]; k < i; k++) y[k] -= at(i, k) * y[i];
-  }
+  for (int64_t i = 0; i < n; i++) y[i] = b[eqs[i]];
+  cholesky_solve(y);
   x.assign(ndof, 0.0);
   for (int64_t i = 0; i < n; i++) x[eqs[i]] = y[i];
+  // two steps of iterative refinement against K_ff (as UMFPACK's default, UMFPACK_IRSTEP = 2, behind
+  // pmsim's default direct solver; src/fem/linear_system.rs:185-201)
+  std::vector<double> r(ndof);
+  for (int step = 0; step < 2; step++) {
+    for (int64_t a = 0; a < np; a++)
+      for (int i = 0; i < nd; i++) {
+        const int64_t eq = (int64_t)nd * a + i;
+        if (pos[eq] < 0) continue;
+        double s = b[eq];
+        for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++) {
+          const int64_t c0 = (int64_t)nd * f->h_bcols[k];
+          for (int j = 0; j < nd; j++)
+            if (pos[c0 + j] >= 0) s -= kv[(size_t)k * nd * nd + i * nd + j] * x[c0 + j];
+        }
+        r[eq] = s;
+      }
+    for (int64_t i = 0; i < n; i++) y[i] = r[eqs[i]];
+    cholesky_solve(y);
+    for (int64_t i = 0; i < n; i++) x[eqs[i]] += y[i];
+  }
   return 0;
 }
 

@@ -63,7 +63,8 @@ struct Group {
   int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
 
-struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; double value; };
+// boundary facet with a normal distributed load: value at each time station (one entry: constant)
+struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; std::vector<double> values; };
 
 struct Timers { double update = 0, residual = 0, tangent = 0, scatter = 0, linear = 0, total = 0; };
 
@@ -90,8 +91,10 @@ struct pm_fem {
   std::vector<double> ess_table;
   int32_t nstation = 0;
   std::vector<int32_t> pl_point, pl_dof;
-  std::vector<double> pl_value;
+  std::vector<double> pl_table;        // point-load values at the time stations (n x pl_nstation)
+  int32_t pl_nstation = 1;
   std::vector<pm::Facet> facets;
+  bool loads_vary = false;             // some natural load changes with time: F_ext is rebuilt per step
   // derived sizes
   int nsig = 4;
   int64_t ndof = 0, nprescribed = 0, nip = 0;
@@ -101,7 +104,7 @@ struct pm_fem {
   std::vector<pm::Group> groups;
   std::vector<int64_t> prescribed_eq;
   std::vector<uint8_t> is_prescribed;  // per DOF
-  std::vector<double> f_ext;           // host-assembled external forces (constant in time)
+  std::vector<double> f_ext;           // host-assembled external forces at the current station
   // node-block CSR of K
   int bs = 2;                          // block size = ndim
   std::vector<int32_t> h_browptr;      // npoint+1
@@ -109,6 +112,12 @@ struct pm_fem {
   int64_t nnzb = 0;
   std::vector<int32_t> h_kmap_ptr, h_kmap;   // K gather map (host copy until upload)
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
+  // K gather by rows: per (node, cell) contribution in ascending cell order, the element's block row
+  // (start block index, node count) and the CSR position of each of its blocks within the node's row
+  std::vector<int32_t> h_rsrc, h_rposoff;
+  std::vector<uint8_t> h_rnn;
+  std::vector<uint16_t> h_rpos;
+  int32_t max_row_blocks = 0;
   // device
   int device = 0;
   int n_cu = 256;                      // compute units of the device
@@ -127,6 +136,10 @@ struct pm_fem {
   int64_t n_ke_blocks = 0;
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
   int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
+  int32_t* d_rsrc = nullptr;           // row gather: element block-row start per (node, cell)
+  int32_t* d_rposoff = nullptr;        // row gather: offset of the positions per (node, cell)
+  uint8_t* d_rnn = nullptr;            // row gather: blocks in the element row
+  uint16_t* d_rpos = nullptr;          // row gather: CSR position (within the row) of every element block
   double* d_fe = nullptr;              // element forces
   int64_t n_fe = 0;
   int32_t* d_fmap_ptr = nullptr;       // npoint+1
@@ -172,7 +185,7 @@ namespace pm {
 int build_groups(pm_fem* f);
 int build_patterns(pm_fem* f);
 void cell_ranks(const pm_fem* f, int64_t e, int* rk);
-void assemble_f_ext(pm_fem* f);
+void assemble_f_ext(pm_fem* f, int64_t station);
 // device (pm_kernels.hip)
 int device_alloc(pm_fem* f);
 void device_free(pm_fem* f);

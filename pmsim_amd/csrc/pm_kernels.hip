@@ -528,28 +528,117 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 
 // deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135): one thread
 // per CSR block sums its contributing element blocks in ascending cell order
+// K gather by rows: one wave per node row a.  For each cell of a (ascending), the cell's block row of a
+// is a contiguous run of nnode blocks (rank order); the wave reads it coalesced and adds every block into
+// the row accumulator in LDS at its CSR position.  Each CSR block thus receives its contributions in
+// ascending cell order, as in the per-block gather (identical sums); the finished row is written
+// contiguously.  Row contributions are the cells of the node: the same lists (and pointer array) as the
+// residual gather.
+template <int BS, typename KT>
+__global__ __launch_bounds__(256) void k_scatter_rows(int64_t np, int maxrow, const int32_t* __restrict__ browptr,
+                                                      const int32_t* __restrict__ cptr, const int32_t* __restrict__ rsrc,
+                                                      const int32_t* __restrict__ rposoff, const uint8_t* __restrict__ rnn,
+                                                      const uint16_t* __restrict__ rpos, const KT* __restrict__ ke,
+                                                      double* __restrict__ kval, float* __restrict__ kval32) {
+  constexpr int B2 = BS * BS;
+  constexpr int VPL = (20 * B2 + 63) / 64;  // values of one element block row per lane (at most 20 nodes)
+  constexpr int PF = 4;                     // contributions in flight
+  extern __shared__ double sacc[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t a = xcd_block() * (int64_t)(blockDim.x >> 6) + wave;
+  if (a >= np) return;  // whole wave; the kernel has no workgroup barrier
+  double* acc = sacc + (size_t)wave * maxrow * B2;
+  const int32_t r0 = browptr[a], nv = (browptr[a + 1] - r0) * B2;
+  for (int i = lane; i < nv; i += 64) acc[i] = 0.0;
+  const int32_t c0 = cptr[a], c1 = cptr[a + 1];
+  for (int32_t cc = c0; cc < c1; cc += 64) {
+    // one contribution's metadata per lane, broadcast below
+    const int nc = min(64, c1 - cc);
+    int msrc = 0, moff = 0, mnn = 0;
+    if (lane < nc) { msrc = rsrc[cc + lane]; moff = rposoff[cc + lane]; mnn = rnn[cc + lane]; }
+    for (int j0 = 0; j0 < nc; j0 += PF) {
+      double v[PF][VPL];
+      int tgt[PF][VPL];
+#pragma unroll
+      for (int j = 0; j < PF; j++) {
+        const int jj = min(j0 + j, nc - 1);
+        const int src = __builtin_amdgcn_readlane(msrc, jj), off = __builtin_amdgcn_readlane(moff, jj);
+        const int m = (j0 + j < nc) ? __builtin_amdgcn_readlane(mnn, jj) * B2 : 0;
+        const KT* __restrict__ sp = ke + (int64_t)src * B2;
+#pragma unroll
+        for (int t = 0; t < VPL; t++) {
+          const int i = lane + 64 * t;
+          tgt[j][t] = -1;
+          v[j][t] = 0.0;
+          if (i < m) {
+            const int rb = i / B2, comp = i - rb * B2;
+            tgt[j][t] = rpos[off + rb] * B2 + comp;
+            v[j][t] = (double)sp[i];
+          }
+        }
+      }
+      // ascending cell order per CSR block, as in the per-block gather
+#pragma unroll
+      for (int j = 0; j < PF; j++)
+#pragma unroll
+        for (int t = 0; t < VPL; t++)
+          if (tgt[j][t] >= 0) acc[tgt[j][t]] += v[j][t];
+    }
+  }
+  for (int i = lane; i < nv; i += 64) {
+    const double val = acc[i];
+    if (kval) kval[(int64_t)r0 * B2 + i] = val;
+    if (kval32) kval32[(int64_t)r0 * B2 + i] = (float)val;
+  }
+}
+
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ kptr, const int32_t* __restrict__ kmap,
                                                  const KT* __restrict__ ke, double* __restrict__ kval, float* __restrict__ kval32) {
   constexpr int B2 = BS * BS;
-  const int64_t blk = xcd_block() * blockDim.x + threadIdx.x;
-  if (blk >= nnzb) return;
+  constexpr int PF = 4;  // contributions whose map entries and blocks are in flight together
+  __shared__ double stage[TPB * B2];
+  const int64_t b0 = xcd_block() * (int64_t)blockDim.x;
+  const int64_t blk = b0 + threadIdx.x;
   double acc[B2];
 #pragma unroll
   for (int c = 0; c < B2; c++) acc[c] = 0.0;
-  const int32_t k1 = kptr[blk + 1];
-  for (int32_t k = kptr[blk]; k < k1; k++) {
-    const KT* src = ke + (int64_t)kmap[k] * B2;
+  if (blk < nnzb) {
+    const int32_t k1 = kptr[blk + 1];
+    for (int32_t k = kptr[blk]; k < k1; k += PF) {
+      int32_t idx[PF];
 #pragma unroll
-    for (int c = 0; c < B2; c++) acc[c] += (double)src[c];
+      for (int j = 0; j < PF; j++) idx[j] = (k + j < k1) ? kmap[k + j] : -1;
+      double v[PF][B2];
+#pragma unroll
+      for (int j = 0; j < PF; j++) {
+        const KT* src = ke + (int64_t)(idx[j] < 0 ? 0 : idx[j]) * B2;
+#pragma unroll
+        for (int c = 0; c < B2; c++) v[j][c] = idx[j] < 0 ? 0.0 : (double)src[c];
+      }
+      // ascending contribution order, as before: the sums are unchanged bit for bit
+#pragma unroll
+      for (int j = 0; j < PF; j++)
+        if (idx[j] >= 0) {
+#pragma unroll
+          for (int c = 0; c < B2; c++) acc[c] += v[j][c];
+        }
+    }
   }
-  if (kval) {
+  // stage the workgroup's blocks and write them out contiguously
 #pragma unroll
-    for (int c = 0; c < B2; c++) kval[blk * B2 + c] = acc[c];
+  for (int c = 0; c < B2; c++) stage[threadIdx.x * B2 + c] = acc[c];
+  __syncthreads();
+  const int64_t nb = min((int64_t)blockDim.x, nnzb - b0);
+  if (nb <= 0) return;
+  const int64_t nv = nb * B2;
+  if (kval) {
+    double* dst = kval + b0 * B2;
+    for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = stage[i];
   }
   if (kval32) {
-#pragma unroll
-    for (int c = 0; c < B2; c++) kval32[blk * B2 + c] = (float)acc[c];
+    float* dst = kval32 + b0 * B2;
+    for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = (float)stage[i];
   }
 }
 
@@ -735,7 +824,23 @@ int tune_tangent(pm_fem* f) {
   return check_device_error(f);
 }
 
+constexpr int ROWS_PER_WG = 4;  // one wave per CSR row
+
+static size_t scatter_rows_shm(const pm_fem* f) { return sizeof(double) * ROWS_PER_WG * (size_t)f->max_row_blocks * f->bs * f->bs; }
+
 void launch_scatter(pm_fem* f) {
+  const size_t shm = scatter_rows_shm(f);
+  if (f->cfg.scatter_kernel == 1 && shm <= 48 * 1024) {
+    const unsigned grid = (unsigned)((f->npoint + ROWS_PER_WG - 1) / ROWS_PER_WG);
+    const bool lowp = f->cfg.tangent_precision != 0;
+#define PM_SR(BS, KT, KE)                                                                                                       \
+  hipLaunchKernelGGL((k_scatter_rows<BS, KT>), dim3(grid), dim3(64 * ROWS_PER_WG), shm, f->stream, f->npoint, f->max_row_blocks,  \
+                     f->d_browptr, f->d_fmap_ptr, f->d_rsrc, f->d_rposoff, f->d_rnn, f->d_rpos, KE, f->d_kval, lowp ? f->d_kval32 : nullptr)
+    if (f->bs == 2) { if (lowp) PM_SR(2, float, f->d_ke32); else PM_SR(2, double, f->d_ke); }
+    else { if (lowp) PM_SR(3, float, f->d_ke32); else PM_SR(3, double, f->d_ke); }
+#undef PM_SR
+    return;
+  }
   const int64_t n = f->nnzb;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;

@@ -40,6 +40,7 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   b += blocks * b2 * ts;                          // element matrices
   b += blocks * (b2 * (8 + (ts == 4 ? 4 : 0)) + 4 + 4);  // K values (bound: nnzb <= element blocks), cols, map
   b += nfe * (8 + 4);                             // element forces + R map
+  b += blocks * 2 + conn * 9;                     // row-gather maps
   b += (f->npoint + 1) * 4 * 3;                   // row pointers
   b += ndof * 8 * 16 + np * 8 * 8;                // vectors, PCG work, multipliers
   b += f->npoint * b2 * 8;                        // block-Jacobi inverses

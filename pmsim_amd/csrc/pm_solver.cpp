@@ -133,6 +133,18 @@ static int save_output(pm_fem* f) {
   return 0;
 }
 
+// external forces at the step's time when a natural load varies with time
+static int set_external_forces(pm_fem* f) {
+  if (!f->loads_vary) return 0;
+  assemble_f_ext(f, (int64_t)std::llround((f->t - f->cfg.t_ini) / f->cfg.dt));
+  if (hipMemcpyAsync(f->d_fext, f->f_ext.data(), sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream) != hipSuccess ||
+      hipStreamSynchronize(f->stream) != hipSuccess) {
+    f->err = "uploading the external forces failed";
+    return 1;
+  }
+  return 0;
+}
+
 static void set_prescribed_values(pm_fem* f) {
   if (!f->nprescribed) return;
   std::vector<double> c(f->nprescribed);
@@ -177,6 +189,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
     if (f->t + cfg.dt > cfg.t_fin) { if (save_output(f)) return 1; break; }
     f->t += cfg.dt;
     set_prescribed_values(f);
+    if (set_external_forces(f)) return 1;
     dev_fill(f, f->d_ddu, 0.0, n);
     if (!cfg.linear_problem) for (auto& gr : f->groups) launch_reset_lambda(f, gr);
     bool converged = false;

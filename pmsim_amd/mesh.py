@@ -11,6 +11,7 @@ Kind codes are shared with the C ABI (``include/pmsim_amd.h``).
 from __future__ import annotations
 
 import dataclasses
+import math
 from typing import Callable, List, Optional, Sequence
 
 import numpy as np
@@ -162,18 +163,9 @@ class Features:
                     first[key] = Facet(gp, fk, e)
         self.boundary = [first[k] for k in first if count[k] == 1]
 
-    def _search(self, axis: int, value: float, flt: Callable, tol=1e-8):
+    def _search(self, at: Callable, flt: Callable):
         X = self.mesh.coords
-        out = []
-        for f in self.boundary:
-            ok = True
-            for p in f.points:
-                x = X[p]
-                if abs(x[axis] - value) > tol or not flt(x):
-                    ok = False
-                    break
-            if ok:
-                out.append(f)
+        out = [f for f in self.boundary if all(at(X[p]) and flt(X[p]) for p in f.points)]
         if not out:
             raise ValueError("cannot find any feature with the given constraints")
         return out
@@ -181,32 +173,64 @@ class Features:
     def search_edges(self, at, flt: Optional[Callable] = None) -> List[Facet]:
         if self.mesh.ndim != 2:
             raise ValueError("search_edges is implemented for 2D meshes")
-        return self._search(at[0], at[1], flt or any_x)
+        return self._search(at, flt or any_x)
 
     def search_faces(self, at, flt: Optional[Callable] = None) -> List[Facet]:
         if self.mesh.ndim != 3:
             raise ValueError("search_faces requires a 3D mesh")
-        return self._search(at[0], at[1], flt or any_x)
+        return self._search(at, flt or any_x)
+
+    def search_point_ids(self, at, flt: Optional[Callable] = None) -> List[int]:
+        """Ids (ascending) of the boundary points that satisfy ``at`` and the filter."""
+        X = self.mesh.coords
+        flt = flt or any_x
+        ids = sorted({p for f in self.boundary for p in f.points if at(X[p]) and flt(X[p])})
+        if not ids:
+            raise ValueError("cannot find any point with the given constraints")
+        return ids
 
 
 def any_x(_x) -> bool:
     return True
 
 
+TOL_AT = 1e-8  # distance below which a point lies on a searched plane, circle or point
+
+
 class At:
-    """Coordinate-plane selector: ``At.X(v)``, ``At.Y(v)``, ``At.Z(v)``."""
+    """Geometric selectors for boundary searches (gemlab's ``At``): each returns a predicate on a
+    point's coordinates.  ``At.X/Y/Z(v)``: coordinate plane; ``At.XY(x, y)`` / ``At.XYZ(x, y, z)``: one
+    point; ``At.Circle(cx, cy, r)``: points at distance r from the centre (2D)."""
 
     @staticmethod
     def X(v):
-        return (0, float(v))
+        v = float(v)
+        return lambda x: abs(x[0] - v) <= TOL_AT
 
     @staticmethod
     def Y(v):
-        return (1, float(v))
+        v = float(v)
+        return lambda x: abs(x[1] - v) <= TOL_AT
 
     @staticmethod
     def Z(v):
-        return (2, float(v))
+        v = float(v)
+        return lambda x: abs(x[2] - v) <= TOL_AT
+
+    @staticmethod
+    def XY(px, py):
+        px, py = float(px), float(py)
+        return lambda x: abs(x[0] - px) <= TOL_AT and abs(x[1] - py) <= TOL_AT
+
+    @staticmethod
+    def XYZ(px, py, pz):
+        px, py, pz = float(px), float(py), float(pz)
+        return lambda x: abs(x[0] - px) <= TOL_AT and abs(x[1] - py) <= TOL_AT and abs(x[2] - pz) <= TOL_AT
+
+    @staticmethod
+    def Circle(cx, cy, r):
+        cx, cy, r = float(cx), float(cy), float(r)
+        return lambda x: abs(math.hypot(x[0] - cx, x[1] - cy) - r) <= TOL_AT * max(1.0, r)
 
 
 def points_of(facets: Sequence[Facet]) -> List[int]:

@@ -138,14 +138,21 @@ class Essential:
 
 
 class Natural:
-    """Concentrated point loads and normal distributed loads (Qn) on boundary facets."""
+    """Concentrated point loads and normal distributed loads (Qn) on boundary facets
+    (src/base/natural.rs:40-140): each value is a constant or, with the ``_fn`` forms, ``f(t)``
+    evaluated at the step's time."""
 
     def __init__(self):
-        self.at_points: List[Tuple[int, int, float]] = []
-        self.on_facets: List[Tuple[Facet, int, float]] = []
+        self.at_points: List[Tuple[int, int, float, Optional[int]]] = []
+        self.on_facets: List[Tuple[Facet, int, float, Optional[int]]] = []
+        self.functions: List[Callable[[float], float]] = []
+
+    def _fn_index(self, f):
+        self.functions.append(f)
+        return len(self.functions) - 1
 
     def point(self, p, pbc, value):
-        self.at_points.append((int(p), int(pbc), float(value)))
+        self.at_points.append((int(p), int(pbc), float(value), None))
         return self
 
     def points(self, pts, pbc, value):
@@ -153,8 +160,18 @@ class Natural:
             self.point(p, pbc, value)
         return self
 
+    def point_fn(self, p, pbc, f):
+        self.at_points.append((int(p), int(pbc), 0.0, self._fn_index(f)))
+        return self
+
+    def points_fn(self, pts, pbc, f):
+        i = self._fn_index(f)
+        for p in pts:
+            self.at_points.append((int(p), int(pbc), 0.0, i))
+        return self
+
     def edge(self, f: Facet, nbc, value):
-        self.on_facets.append((f, int(nbc), float(value)))
+        self.on_facets.append((f, int(nbc), float(value), None))
         return self
 
     def edges(self, facets, nbc, value):
@@ -162,8 +179,27 @@ class Natural:
             self.edge(f, nbc, value)
         return self
 
+    def edge_fn(self, facet: Facet, nbc, f):
+        self.on_facets.append((facet, int(nbc), 0.0, self._fn_index(f)))
+        return self
+
+    def edges_fn(self, facets, nbc, f):
+        i = self._fn_index(f)
+        for facet in facets:
+            self.on_facets.append((facet, int(nbc), 0.0, i))
+        return self
+
     face = edge
     faces = edges
+    face_fn = edge_fn
+    faces_fn = edges_fn
+
+    def values(self, value, f_index, times):
+        """A load's values at the time stations (one value when it is constant)."""
+        if f_index is None:
+            return np.asarray([value], dtype=np.float64)
+        f = self.functions[f_index]
+        return np.asarray([f(float(t)) for t in times], dtype=np.float64)
 
 
 class Config:
@@ -197,6 +233,7 @@ class Config:
         self.pcg_small_max = -1  # DOF limit of the single-workgroup PCG (-1: library default)
         self.verbose_timesteps = False  # a line per converged step on stderr
         self.verbose_iterations = False  # a line per Newton iteration on stderr
+        self.scatter_kernel = "blocks"  # K gather: "blocks" (thread per CSR block) or "rows" (wave per CSR row)
         self.pcg_tol = 1e-12
         self.pcg_max_iter = 20000
         self.scaled_rr = 0.0           # > 0 enables ||R||inf < scaled_rr * ||F_int||inf as an extra test
@@ -237,6 +274,7 @@ class Config:
     def set_tangent_kernel(self, k: str): self.tangent_kernel = k; return self
     def set_use_graphs(self, v: bool): self.use_graphs = bool(v); return self
     def set_pcg_small_max(self, n: int): self.pcg_small_max = int(n); return self
+    def set_scatter_kernel(self, k: str): self.scatter_kernel = k; return self
     def set_verbose(self, timesteps: bool, iterations: bool): self.verbose_timesteps = timesteps; self.verbose_iterations = iterations; return self
     def set_pcg(self, tol=1e-12, max_iter=20000): self.pcg_tol = float(tol); self.pcg_max_iter = int(max_iter); return self
     def set_scaled_rr(self, v): self.scaled_rr = float(v); return self
@@ -279,12 +317,16 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
     for a, p in base.amap.items():
         ss = p.stress_strain
         params[a] = (ss.model, ss.young, ss.poisson, ss.hh, ss.z_ini, int(p.ngauss or 0))
-    pl = [(p, d, v) for (p, d, v) in natural.at_points]
+    pl = [(p, d, natural.values(v, fi, times)) for (p, d, v, fi) in natural.at_points]
+    nst_pl = max([len(x[2]) for x in pl], default=1)
+    pl_table = np.zeros((len(pl), nst_pl))
+    for i, (_, _, vals) in enumerate(pl):
+        pl_table[i, :] = vals  # a constant fills its row
     facets = []
-    for (f, nbc, v) in natural.on_facets:
+    for (f, nbc, v, fi) in natural.on_facets:
         if nbc != Nbc.Qn:
             raise ValueError("only Qn distributed loads are supported on the solid path")
-        facets.append((f.kind, tuple(f.points), f.cell, v))
+        facets.append((f.kind, tuple(f.points), f.cell, natural.values(v, fi, times)))
     return dict(
         ndim=mesh.ndim, coords=mesh.coords, kinds=mesh.kinds, attrs=mesh.attrs,
         cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts, params=params,
@@ -296,6 +338,6 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
         ess_points=pts, ess_dofs=dofs, ess_table=tab,
         pl_points=np.asarray([x[0] for x in pl], dtype=np.int32),
         pl_dofs=np.asarray([x[1] for x in pl], dtype=np.int32),
-        pl_values=np.asarray([x[2] for x in pl], dtype=np.float64),
+        pl_table=pl_table,
         facets=facets,
     )

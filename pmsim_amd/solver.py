@@ -39,7 +39,7 @@ class PmConfig(C.Structure):
         ("tol_rr_abs", C.c_double), ("tol_mdu_rel", C.c_double), ("symmetry_check_tolerance", C.c_double),
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
         ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
-        ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32),
+        ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32), ("scatter_kernel", C.c_int32),
     ]
 
 
@@ -83,8 +83,8 @@ def lib():
     L.pm_set_idealization.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double]
     L.pm_set_config.argtypes = [C.c_void_p, C.POINTER(PmConfig)]
     L.pm_set_essential.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp, C.c_int32]
-    L.pm_set_point_loads.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp]
-    L.pm_add_facet_qn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int64, C.c_double]
+    L.pm_set_point_loads.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp, C.c_int32]
+    L.pm_add_facet_qn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int64, _dp, C.c_int32]
     L.pm_initialize.argtypes = [C.c_void_p, C.c_int32]
     for name in ["pm_initialize_internal_values", "pm_reset_algorithmic_variables", "pm_backup_secondary_values",
                  "pm_restore_secondary_values", "pm_update_secondary_values", "pm_assemble_f_int", "pm_assemble_kk",
@@ -141,6 +141,7 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
         c.use_graphs = 1 if cfg.use_graphs else 0
         c.pcg_small_max = int(cfg.pcg_small_max)
         c.verbose = (1 if cfg.verbose_timesteps else 0) | (2 if cfg.verbose_iterations else 0)
+        c.scatter_kernel = {"blocks": 0, "rows": 1}[cfg.scatter_kernel]
     return c
 
 
@@ -169,11 +170,14 @@ class Fem:
         if n:
             self._check(L.pm_set_essential(h, n, np.ascontiguousarray(d["ess_points"], dtype=np.int32),
                                            np.ascontiguousarray(d["ess_dofs"], dtype=np.int32), tab.ravel(), tab.shape[1]))
-        if len(d["pl_values"]):
-            self._check(L.pm_set_point_loads(h, len(d["pl_values"]), d["pl_points"], d["pl_dofs"], d["pl_values"]))
+        plt = np.ascontiguousarray(d["pl_table"], dtype=np.float64)
+        if plt.shape[0]:
+            self._check(L.pm_set_point_loads(h, plt.shape[0], np.ascontiguousarray(d["pl_points"], dtype=np.int32),
+                                             np.ascontiguousarray(d["pl_dofs"], dtype=np.int32), plt.ravel(), plt.shape[1]))
         for (fk, pts, cell, v) in d["facets"]:
             p = np.asarray(pts, dtype=np.int32)
-            self._check(L.pm_add_facet_qn(h, int(fk), len(p), p, int(cell), float(v)))
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            self._check(L.pm_add_facet_qn(h, int(fk), len(p), p, int(cell), v, len(v)))
         if initialize:
             self._check(L.pm_initialize(h, device))
             self.refresh_sizes()

@@ -85,6 +85,51 @@ def spo_754_footing():
     return mesh, base, cfg, ess, Natural()
 
 
+P_751 = [0.0, 0.1, 0.14, 0.18, 0.19, 0.192]  # inner pressure per station
+
+
+def spo_751_pres_cylin():
+    """tests/test_spo_751_pres_cylin.rs:36-98: quarter ring a=100, b=200 (12 Qua4), internal pressure
+    Qn = -p(t) on the inner circle, RSM (no Lagrange multipliers), tol_mdu_rel = 1e-10."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "spo_751_pres_cylin_qua4.msh"))
+    feat = Features(mesh)
+    bottom = feat.search_edges(At.Y(0.0), any_x)
+    left = feat.search_edges(At.X(0.0), any_x)
+    inner = feat.search_edges(At.Circle(0.0, 0.0, 100.0), any_x)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(210.0, 0.3, 0.0, 0.24), 4)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges_fn(inner, Nbc.Qn, lambda t: -P_751[int(t)])
+    cfg = Config(mesh)
+    cfg.set_tol_mdu_rel(1e-10).set_lagrange_mult_method(False).set_incremental(len(P_751))
+    return mesh, base, cfg, ess, nat
+
+
+P_753 = [0.0, 100.0, 200.0, 220.0, 230.0, 240.0, 250.0, 255.0, 257.0, 259.0, 259.5, 259.75, 259.77]
+
+
+def spo_753_circ_plate():
+    """tests/test_spo_753_circ_plate.rs:26-86: axisymmetric circular plate R=10, h=1 (10 Qua8), uniform
+    pressure Qn = -P(t) on the top, Lagrange multipliers."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "spo_753_circ_plate.msh"))
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0), any_x)
+    top = feat.search_edges(At.Y(1.0), any_x)
+    right_corner = feat.search_point_ids(At.XY(10.0, 0.0))[0]
+    p1 = ParamSolid(1.0, StressStrain.VonMises(1e7, 0.24, 0.0, 16000.0), 4)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).point(right_corner, Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges_fn(top, Nbc.Qn, lambda t: -P_753[int(t)])
+    cfg = Config(mesh)
+    cfg.set_axisymmetric().set_incremental(len(P_753)).set_lagrange_mult_method(True)
+    cfg.set_symmetry_check_tolerance(1e-5).set_n_max_iterations(20)
+    return mesh, base, cfg, ess, nat
+
+
 def spo_755_tensile():
     """tests/test_spo_755_tensile.rs:19-85."""
     mesh = Mesh.read(os.path.join(GOLDEN, "spo_755_tensile.msh"))

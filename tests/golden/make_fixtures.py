@@ -8,6 +8,8 @@ Outputs (data only: inputs and expected outputs):
 * spo_*.json.gz   -- HYPLAS per-step displacements, IP stresses and plastic flags/epbar
                      (data/spo/*_ref.json, loaded by src/util/reference_data_spo.rs:38-151)
 * *.msh           -- meshes shipped with the reference (data/spo/*.msh)
+* *.tsv           -- SPO curves digitised from the book, shipped with the reference (data/spo/*.tsv):
+                     plate deflection-load and deflection profiles (753), footing load-displacement (754)
 * sgm_5d17_ref.json -- Smith-Griffiths-Margetts Example 5.17 reference data (data/sgm/)
 * smith_meshes.json -- point coordinates and cell connectivity of the Smith sample meshes
                      (src/base/sample_meshes.rs:261-735)
@@ -33,8 +35,9 @@ def spo(name):
         steps.append(dict(load_factor=s["load_factor"], n_iterations=len(s["iterations"]),
                           displacement=s["displacement"], stresses=s["stresses"],
                           plast_apex_epbar=s["plast_apex_epbar"]))
-    with gzip.open(os.path.join(OUT, name + ".json.gz"), "wt") as f:
-        json.dump(dict(all=steps), f)
+    # mtime=0: the same data gives the same bytes
+    with gzip.GzipFile(os.path.join(OUT, name + ".json.gz"), "wb", mtime=0) as f:
+        f.write(json.dumps(dict(all=steps)).encode())
 
 
 def meshes():
@@ -72,9 +75,10 @@ def smith_vectors():
 
 if __name__ == "__main__":
     for n in ["test_von_mises_single_element_2d_ref", "test_von_mises_2x2_elements_2d_ref",
-              "spo_754_footing_ref", "spo_755_tensile_perf_plast_ref"]:
+              "spo_751_pres_cylin_ref", "spo_753_circ_plate_ref", "spo_754_footing_ref", "spo_755_tensile_perf_plast_ref"]:
         spo(n)
-    for n in ["spo_754_footing.msh", "spo_755_tensile.msh"]:
+    for n in ["spo_751_pres_cylin_qua4.msh", "spo_753_circ_plate.msh", "spo_754_footing.msh", "spo_755_tensile.msh",
+              "spo_753_plate_deflection_load.tsv", "spo_753_profiles.tsv", "spo_754_footing_load_displacement.tsv"]:
         shutil.copy(os.path.join(REF, "data/spo", n), os.path.join(OUT, n))
     shutil.copy(os.path.join(REF, "data/sgm/sgm_5d17_ref.json"), os.path.join(OUT, "sgm_5d17_ref.json"))
     meshes()

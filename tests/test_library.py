@@ -54,6 +54,24 @@ def test_essential_override_and_tables():
                                               -0.08, -0.09, -0.11, -0.14, -0.2])
 
 
+def test_natural_load_tables():
+    """Natural loads (src/base/natural.rs): a constant keeps one value, an ``_fn`` load takes f(t) at every
+    station; point and circle selectors find the spo_751 ring's inner arc and the plate's corner."""
+    mesh, base, cfg, ess, nat = cases.spo_751_pres_cylin()
+    d = describe(mesh, base, cfg, ess, nat)
+    assert len(d["facets"]) == 3  # three Qua4 edges on the inner circle r = 100
+    for (_, pts, _, vals) in d["facets"]:
+        assert np.allclose(np.hypot(*mesh.coords[list(pts)].T), 100.0)
+        assert np.array_equal(vals, -np.asarray(cases.P_751))
+    nat.point(0, 0, 2.5).points_fn([1, 2], 1, lambda t: 10.0 * t)
+    d = describe(mesh, base, cfg, ess, nat)
+    assert d["pl_table"].shape == (3, 6)
+    assert np.array_equal(d["pl_table"][0], np.full(6, 2.5))
+    assert np.array_equal(d["pl_table"][2], 10.0 * np.arange(6.0))
+    plate = cases.spo_753_circ_plate()[0]
+    assert Features(plate).search_point_ids(At.XY(10.0, 0.0)) == [7]  # "7 0 10.0 0.0" in the .msh
+
+
 def test_footing_edge_selection_rule():
     """Edges are selected when all their points pass the filter: nodes x in {0, 0.3125, 0.625} for the
     16x8 footing (the 0.9375 mid-node belongs to an edge that leaves x <= 1)."""

