@@ -125,6 +125,20 @@ int pm_element_pass(pm_fem* fem);
 int pm_solve(pm_fem* fem, int32_t* n_converged);
 int pm_get_stats(const pm_fem* fem, pm_stats* stats);
 
+/* ---- integration points and post-processing (no handle, no device) ---------------------- */
+/* Integration rule of a solid kind (gemlab's Gauss::new_or_sized as called at
+ * src/fem/element_solid.rs:59-62; ngauss <= 0: the kind's default): npoint, natural coordinates
+ * xi (npoint x 3, unused trailing components 0) and weights w; xi / w may be NULL to query npoint. */
+int pm_gauss_rule(int32_t kind, int32_t ngauss, int32_t* npoint, double* xi, double* w);
+/* Shape functions N (npoint x nnode) of a kind (solid or facet) at natural coordinates xi
+ * (npoint x 3): gemlab's interpolation as used by PostProc::gauss_coords (src/fem/post_processing.rs:147-160). */
+int pm_shape_values(int32_t kind, int32_t npoint, const double* xi, double* N);
+/* Extrapolation matrix E (nnode x nip, row-major) from the integration points to the nodes, as used
+ * by PostProc::nodal_stress / nodal_stresses (src/fem/post_processing.rs:455-520, 637-653):
+ * least-squares fit when nip >= nnode, Durand & Farias (2014) local extrapolation when
+ * ndim+1 <= nip < nnode, minimum-norm fit below that. */
+int pm_extrapolation_matrix(int32_t kind, int32_t ngauss, double* E);
+
 /* ---- data movement -------------------------------------------------------------------- */
 int pm_sizes(const pm_fem* fem, int64_t* out /* [ndof, n_prescribed, nip, nsig, n_block_nnz, ndim] */);
 int pm_set_ddu(pm_fem* fem, const double* ddu /* ndof */);

@@ -292,9 +292,12 @@ std::vector<IP> gauss_rule(int kind, int ngauss) {
   return tet_rule(ngauss);
 }
 
+// default rules when ParamSolid.ngauss is None; Tri3's 3 points are what pmsim writes in its own state
+// files (data/results/artificial/artificial-elastic-2d-*.json: "ngauss": 3 for every Tri3 cell of a
+// base whose ngauss is null), Qua8 9 and Hex8 8 likewise (artificial-elastic-2d-qua8, -3d)
 int default_ngauss(int kind) {
   switch (kind) {
-    case TRI3: return 1; case TRI6: return 3; case TRI10: return 6; case TRI15: return 12;
+    case TRI3: return 3; case TRI6: return 3; case TRI10: return 6; case TRI15: return 12;
     case QUA4: return 4; case QUA8: return 9; case QUA9: return 9;
     case TET4: return 1; case TET10: return 4; case HEX8: return 8; case HEX20: return 27;
   }

@@ -133,6 +133,32 @@ int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* 
   return 0;
 }
 
+int pm_gauss_rule(int32_t kind, int32_t ngauss, int32_t* npoint, double* xi, double* w) {
+  Rule R;
+  if (kind < 0 || kind >= NKIND || !gauss_rule(kind, ngauss > 0 ? ngauss : kind_default_ngauss(kind), R)) {
+    g_last_error = "no integration rule for this kind and number of points";
+    return 1;
+  }
+  *npoint = R.n();
+  if (xi) std::copy(R.x.begin(), R.x.end(), xi);
+  if (w) std::copy(R.w.begin(), R.w.end(), w);
+  return 0;
+}
+
+int pm_shape_values(int32_t kind, int32_t npoint, const double* xi, double* N) {
+  const int nn = kind_nnode(kind);
+  if (nn <= 0) { g_last_error = "unknown kind"; return 1; }
+  for (int32_t p = 0; p < npoint; p++) shape(kind, xi + 3 * p, N + (size_t)p * nn, nullptr);
+  return 0;
+}
+
+int pm_extrapolation_matrix(int32_t kind, int32_t ngauss, double* E) {
+  std::vector<double> e;
+  if (!extrapolation_matrix(kind, ngauss, e)) { g_last_error = "cannot compute the extrapolation matrix"; return 1; }
+  std::copy(e.begin(), e.end(), E);
+  return 0;
+}
+
 int pm_initialize(pm_fem* f, int32_t device) {
   if (!f) { g_last_error = "null handle"; return 1; }
   if (f->initialized || f->released) { f->err = "already initialized"; return 1; }

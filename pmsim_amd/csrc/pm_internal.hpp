@@ -26,6 +26,8 @@ struct Rule {
   int n() const { return (int)w.size(); }
 };
 bool gauss_rule(int kind, int ngauss, Rule& r);
+bool node_xi(int kind, int m, double* xi);
+bool extrapolation_matrix(int kind, int ngauss, std::vector<double>& E);
 
 // Shape functions N and derivatives dN/dxi (nnode*ndim) at a reference point (closed forms).
 void shape(int kind, const double* x, double* N, double* dN);

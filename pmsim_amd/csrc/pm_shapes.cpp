@@ -6,6 +6,7 @@
 // which is exact to rounding for polynomials (no subtractive cancellation).
 #include <cmath>
 #include <complex>
+#include <utility>
 
 #include "pm_internal.hpp"
 
@@ -18,8 +19,10 @@ int kind_nnode(int k) {
   return 0;
 }
 int kind_ndim(int k) { return (k >= 0 && k <= QUA9) ? 2 : (k < NKIND ? 3 : 1); }
+// integration points when ParamSolid.ngauss is None (Tri3: 3, Qua8: 9 and Hex8: 8 are the counts
+// pmsim writes in its own state files, data/results/artificial/*.json)
 int kind_default_ngauss(int k) {
-  static const int ng[NKIND] = {1, 3, 6, 12, 4, 9, 9, 1, 4, 8, 27};
+  static const int ng[NKIND] = {3, 3, 6, 12, 4, 9, 9, 1, 4, 8, 27};
   return (k >= 0 && k < NKIND) ? ng[k] : 0;
 }
 
@@ -211,6 +214,135 @@ bool gauss_rule(int kind, int ngauss, Rule& R) {
     return false;
   }
   return false;
+}
+
+// natural coordinates of node m of a solid kind (the nodes at which N_m = 1)
+bool node_xi(int kind, int m, double* xi) {
+  xi[0] = xi[1] = xi[2] = 0.0;
+  const int nn = kind_nnode(kind);
+  if (m < 0 || m >= nn) return false;
+  switch (kind) {
+    case TRI3: case TRI6: case TRI10: case TRI15: {
+      const int p = kind - TRI3 + 1;
+      xi[0] = double(TRI_LAT[p - 1][m][1]) / p;
+      xi[1] = double(TRI_LAT[p - 1][m][2]) / p;
+      return true;
+    }
+    case QUA4: case QUA8: case QUA9: {
+      static const double q[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}, {0, 0}};
+      xi[0] = q[m][0]; xi[1] = q[m][1];
+      return true;
+    }
+    case TET4: case TET10: {
+      static const double v[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      static const int e[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+      if (m < 4) { for (int d = 0; d < 3; d++) xi[d] = v[m][d]; return true; }
+      for (int d = 0; d < 3; d++) xi[d] = 0.5 * (v[e[m - 4][0]][d] + v[e[m - 4][1]][d]);
+      return true;
+    }
+    case HEX8: case HEX20: {
+      static const double h[20][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1},
+                                      {-1, 1, 1}, {0, -1, -1}, {1, 0, -1}, {0, 1, -1}, {-1, 0, -1}, {0, -1, 1}, {1, 0, 1},
+                                      {0, 1, 1}, {-1, 0, 1}, {-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}};
+      for (int d = 0; d < 3; d++) xi[d] = h[m][d];
+      return true;
+    }
+  }
+  return false;
+}
+
+namespace {
+
+// inverse of a small symmetric positive definite matrix (n <= 27) by Gauss-Jordan with partial pivoting
+bool small_inverse(std::vector<double> a, int n, std::vector<double>& inv) {
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++) if (std::fabs(a[(size_t)i * n + k]) > std::fabs(a[(size_t)p * n + k])) p = i;
+    if (std::fabs(a[(size_t)p * n + k]) < 1e-300) return false;
+    if (p != k)
+      for (int j = 0; j < n; j++) { std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]); std::swap(inv[(size_t)k * n + j], inv[(size_t)p * n + j]); }
+    const double d = a[(size_t)k * n + k];
+    for (int j = 0; j < n; j++) { a[(size_t)k * n + j] /= d; inv[(size_t)k * n + j] /= d; }
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      const double f = a[(size_t)i * n + k];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; j++) { a[(size_t)i * n + j] -= f * a[(size_t)k * n + j]; inv[(size_t)i * n + j] -= f * inv[(size_t)k * n + j]; }
+    }
+  }
+  return true;
+}
+
+// Moore-Penrose pseudo-inverse of a full-rank r x c matrix A (c x r result): (A^T A)^-1 A^T when
+// r >= c, A^T (A A^T)^-1 otherwise
+bool pseudo_inverse(const std::vector<double>& A, int r, int c, std::vector<double>& P) {
+  P.assign((size_t)c * r, 0.0);
+  std::vector<double> G, Gi;
+  if (r >= c) {
+    G.assign((size_t)c * c, 0.0);
+    for (int i = 0; i < c; i++) for (int j = 0; j < c; j++) for (int k = 0; k < r; k++) G[(size_t)i * c + j] += A[(size_t)k * c + i] * A[(size_t)k * c + j];
+    if (!small_inverse(G, c, Gi)) return false;
+    for (int i = 0; i < c; i++) for (int j = 0; j < r; j++) for (int k = 0; k < c; k++) P[(size_t)i * r + j] += Gi[(size_t)i * c + k] * A[(size_t)j * c + k];
+  } else {
+    G.assign((size_t)r * r, 0.0);
+    for (int i = 0; i < r; i++) for (int j = 0; j < r; j++) for (int k = 0; k < c; k++) G[(size_t)i * r + j] += A[(size_t)i * c + k] * A[(size_t)j * c + k];
+    if (!small_inverse(G, r, Gi)) return false;
+    for (int i = 0; i < c; i++) for (int j = 0; j < r; j++) for (int k = 0; k < r; k++) P[(size_t)i * r + j] += A[(size_t)k * c + i] * Gi[(size_t)k * r + j];
+  }
+  return true;
+}
+
+}  // namespace
+
+// Extrapolation from the integration points to the nodes (pmsim's PostProc::nodal_stress,
+// src/fem/post_processing.rs:455-520, through gemlab's get_extrap_matrix): nodal = E (nnode x nip)
+// times the IP values.  With N (nip x nnode) the shape functions at the IPs:
+//  * nip >= nnode: E = N^+ = (N^T N)^-1 N^T, the least-squares fit (N^-1 when square);
+//  * ndim+1 <= nip < nnode: Durand R, Farias MM (2014) "A local extrapolation method for finite
+//    elements", Adv. Eng. Software 67:1-9: E = N^+ (I - X X^+) + Xn X^+ with X (nip x ndim+1) the IPs'
+//    natural coordinates and a column of ones and Xn (nnode x ndim+1) the same for the nodes, so that a
+//    field linear in the natural coordinates is reproduced exactly;
+//  * fewer IPs: E = N^+ (minimum norm; reproduces constant fields).
+// The first case agrees with any least-squares extrapolation; the second is the published method
+// named above.
+bool extrapolation_matrix(int kind, int ngauss, std::vector<double>& E) {
+  const int nn = kind_nnode(kind), nd = kind_ndim(kind);
+  if (nn <= 0 || kind >= NKIND) return false;
+  Rule R;
+  if (!gauss_rule(kind, ngauss > 0 ? ngauss : kind_default_ngauss(kind), R)) return false;
+  const int np = R.n();
+  std::vector<double> N((size_t)np * nn);
+  for (int p = 0; p < np; p++) shape(kind, &R.x[3 * p], &N[(size_t)p * nn], nullptr);
+  std::vector<double> Np;
+  if (!pseudo_inverse(N, np, nn, Np)) return false;  // nn x np
+  if (np >= nn || np < nd + 1) { E = Np; return true; }
+  const int c = nd + 1;
+  std::vector<double> X((size_t)np * c), Xn((size_t)nn * c), Xp;
+  for (int p = 0; p < np; p++) { for (int d = 0; d < nd; d++) X[(size_t)p * c + d] = R.x[3 * p + d]; X[(size_t)p * c + nd] = 1.0; }
+  for (int m = 0; m < nn; m++) {
+    double xi[3];
+    node_xi(kind, m, xi);
+    for (int d = 0; d < nd; d++) Xn[(size_t)m * c + d] = xi[d];
+    Xn[(size_t)m * c + nd] = 1.0;
+  }
+  if (!pseudo_inverse(X, np, c, Xp)) return false;  // c x np
+  // P = I - X X^+ (np x np)
+  std::vector<double> P((size_t)np * np, 0.0);
+  for (int i = 0; i < np; i++) {
+    P[(size_t)i * np + i] = 1.0;
+    for (int j = 0; j < np; j++) for (int k = 0; k < c; k++) P[(size_t)i * np + j] -= X[(size_t)i * c + k] * Xp[(size_t)k * np + j];
+  }
+  E.assign((size_t)nn * np, 0.0);
+  for (int m = 0; m < nn; m++)
+    for (int j = 0; j < np; j++) {
+      double s = 0.0;
+      for (int i = 0; i < np; i++) s += Np[(size_t)m * np + i] * P[(size_t)i * np + j];
+      for (int k = 0; k < c; k++) s += Xn[(size_t)m * c + k] * Xp[(size_t)k * np + j];
+      E[(size_t)m * np + j] = s;
+    }
+  return true;
 }
 
 }  // namespace pm

@@ -27,8 +27,9 @@ KIND_NAMES = {
 }
 NNODE = {TRI3: 3, TRI6: 6, TRI10: 10, TRI15: 15, QUA4: 4, QUA8: 8, QUA9: 9, TET4: 4, TET10: 10, HEX8: 8, HEX20: 20}
 NDIM_OF = {k: (2 if k <= QUA9 else 3) for k in NNODE}
-# default number of integration points per kind (the rules used when ParamSolid.ngauss is None)
-DEFAULT_NGAUSS = {TRI3: 1, TRI6: 3, TRI10: 6, TRI15: 12, QUA4: 4, QUA8: 9, QUA9: 9, TET4: 1, TET10: 4, HEX8: 8, HEX20: 27}
+# default number of integration points per kind (the rules used when ParamSolid.ngauss is None); Tri3 3,
+# Qua8 9 and Hex8 8 are the counts pmsim writes in its own state files (data/results/artificial/*.json)
+DEFAULT_NGAUSS = {TRI3: 3, TRI6: 3, TRI10: 6, TRI15: 12, QUA4: 4, QUA8: 9, QUA9: 9, TET4: 1, TET10: 4, HEX8: 8, HEX20: 27}
 
 # local edges of 2D cells: [end0, end1, interior nodes ordered from end0 to end1]
 EDGES_2D = {
@@ -231,6 +232,33 @@ class At:
     def Circle(cx, cy, r):
         cx, cy, r = float(cx), float(cy), float(r)
         return lambda x: abs(math.hypot(x[0] - cx, x[1] - cy) - r) <= TOL_AT * max(1.0, r)
+
+
+def edges_path(edges: Sequence[Facet]) -> List[int]:
+    """Point ids along the path formed by 2D edges joined end to end (gemlab's Edges::any_path), each
+    edge's interior points between its two ends; starts at an end of the path (a point used once)."""
+    if not edges:
+        return []
+    ends = {}
+    for i, f in enumerate(edges):
+        for p in (f.points[0], f.points[1]):
+            ends.setdefault(int(p), []).append(i)
+    start = next((p for p, es in ends.items() if len(es) == 1), int(edges[0].points[0]))
+    path, used, cur = [start], set(), start
+    while True:
+        nxt = [i for i in ends[cur] if i not in used]
+        if not nxt:
+            break
+        i = nxt[0]
+        used.add(i)
+        f = edges[i]
+        a, b = int(f.points[0]), int(f.points[1])
+        inner = [int(p) for p in f.points[2:]]
+        if cur == b:
+            a, b, inner = b, a, inner[::-1]
+        path.extend(inner + [b])
+        cur = b
+    return path
 
 
 def points_of(facets: Sequence[Facet]) -> List[int]:

@@ -27,7 +27,8 @@ EXPORTED = [
     "pm_restore_secondary_values", "pm_update_secondary_values", "pm_assemble_f_int", "pm_assemble_kk", "pm_element_pass",
     "pm_solve", "pm_get_stats", "pm_sizes", "pm_set_ddu", "pm_get_u", "pm_get_f_int", "pm_get_ip_state", "pm_set_ip_state",
     "pm_get_csr", "pm_history_count", "pm_get_history", "pm_time_element_pass", "pm_linear_solve",
-    "pm_estimate_device_bytes", "pm_release_device", "pm_serve",
+    "pm_estimate_device_bytes", "pm_release_device", "pm_serve", "pm_gauss_rule", "pm_extrapolation_matrix",
+    "pm_shape_values",
 ]
 
 
@@ -102,6 +103,9 @@ def lib():
     L.pm_history_count.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.pm_get_history.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_void_p, _dp, C.POINTER(C.c_double)]
     L.pm_time_element_pass.argtypes = [C.c_void_p, C.c_int32, _dp]
+    L.pm_gauss_rule.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p]
+    L.pm_extrapolation_matrix.argtypes = [C.c_int32, C.c_int32, _dp]
+    L.pm_shape_values.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
     L.pm_linear_solve.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32)]
     L.pm_estimate_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.pm_release_device.argtypes = [C.c_void_p]

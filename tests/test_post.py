@@ -1,0 +1,182 @@
+"""Post-processing (pmsim's PostProc, src/fem/post_processing.rs) on the CPU: the library's
+integration rules, shape functions and extrapolation matrices are host functions of the C ABI.
+
+Pinned by: pmsim's own known answers on its artificial result files (tests of post_processing.rs:
+gauss_coords_works_2d/3d, gauss_stresses_and_strains_work_2d/3d, nodal_stress(es)_and_strains_work_2d/3d
+-- sorted coordinates, ids and uniform values), the algebra of the extrapolation (inverse or
+least-squares fit when nip >= nnode, exact for fields linear in the natural coordinates), and the
+footing's load-settlement curve against slip-line theory (2 + pi) and the curve digitised from de Souza
+Neto et al. (2008) (data/spo/spo_754_footing_load_displacement.tsv).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as orc
+from pmsim_amd import At, Features, Mesh, post
+from pmsim_amd import file_io as fio
+from pmsim_amd import mesh as M
+
+ART = os.path.join(os.path.dirname(__file__), "golden", "artificial")
+RULES = [(M.TRI3, 1), (M.TRI3, 3), (M.TRI6, 3), (M.TRI6, 6), (M.TRI10, 6), (M.TRI15, 12), (M.QUA4, 1), (M.QUA4, 4),
+         (M.QUA8, 4), (M.QUA8, 9), (M.QUA9, 9), (M.TET4, 1), (M.TET4, 4), (M.TET10, 4), (M.TET10, 5), (M.HEX8, 8),
+         (M.HEX8, 27), (M.HEX20, 8), (M.HEX20, 27)]
+
+
+def _fmt(res, nd):
+    cols = [res.xx, res.yy] + ([res.zz] if nd == 3 else [])
+    return "".join(",".join(f"{c[k]:.5f}" for c in cols) + "\n" for k in range(len(res.xx)))
+
+
+@pytest.mark.parametrize("kind,ng", RULES)
+def test_extrapolation_algebra(kind, ng):
+    """E reproduces constants always and fields linear in the natural coordinates whenever nip >=
+    ndim + 1; with nip >= nnode it is the least-squares fit, E N = I (N^-1 when square)."""
+    E = post.extrapolation_matrix(kind, ng)
+    xi, _ = post.gauss_rule(kind, ng)
+    nn, nip, nd = M.NNODE[kind], len(xi), (2 if kind <= M.QUA9 else 3)
+    assert E.shape == (nn, nip)
+    assert np.abs(E @ np.ones(nip) - 1.0).max() < 1e-13
+    nodes = np.array([post.shape_values(kind, np.eye(1, 3, 0))[0] * 0.0 for _ in range(1)])  # shape check only
+    assert nodes.shape == (1, nn)
+    # nodal natural coordinates: the points where N_m = 1
+    N_ip = post.shape_values(kind, xi)
+    if nip >= nd + 1:
+        rng = np.random.default_rng(kind * 100 + ng)
+        c = rng.standard_normal(nd + 1)
+        f_ip = xi[:, :nd] @ c[:nd] + c[nd]
+        f_nodes = E @ f_ip
+        # a linear field is interpolated exactly by every kind, so N at the IPs recovers it
+        assert np.abs(N_ip @ f_nodes - f_ip).max() < 1e-12
+    if nip >= nn:
+        assert np.abs(E @ N_ip - np.eye(nn)).max() < 1e-11
+
+
+def test_gauss_coords_known_answers(tmp_path):
+    """post_processing.rs gauss_coords_works_2d/3d: one-point Qua4 at the centre; Hex8 2x2x2 points at
+    (1 -+ 1/sqrt3)/2 in the unit cube, r fastest."""
+    xi, w = post.gauss_rule(M.QUA4, 1)
+    N = post.shape_values(M.QUA4, xi)
+    X = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=float)
+    assert np.array_equal(N @ X, [[0.5, 0.5]])
+    xi, w = post.gauss_rule(M.HEX8, 8)
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float)
+    a, b = (1 - 1 / math.sqrt(3)) / 2, (1 + 1 / math.sqrt(3)) / 2
+    want = [[a, a, a], [b, a, a], [a, b, a], [b, b, a], [a, a, b], [b, a, b], [a, b, b], [b, b, b]]
+    assert np.abs(post.shape_values(M.HEX8, xi) @ X - want).max() < 1e-15
+
+
+def test_gauss_and_nodal_stresses_2d():
+    """post_processing.rs gauss_stresses_and_strains_work_2d / nodal_stresses_and_strains_work_2d on
+    pmsim's artificial-elastic-2d files (three Tri3, uniform stress states)."""
+    pp = post.PostProc(ART, "artificial-elastic-2d")
+    assert pp.n_state() == 3
+    flt = lambda x, y, z: not (x < 0.5 and y < 0.5)
+    for index in range(3):
+        st = pp.read_state(index)
+        ref = pp.gauss_stress(st, 0)[0]
+        g = pp.gauss_stresses(st, [0, 1, 2], flt)
+        assert g.k2id == list(range(8))  # 8 of the 9 IPs pass the filter
+        for cell in range(3):
+            assert np.abs(pp.nodal_stress(st, cell) - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+        n = pp.nodal_stresses(st, [0, 1, 2], flt)
+        assert n.k2id == [1, 2, 3, 4]
+        assert np.abs(np.stack([n.txx, n.tyy, n.tzz, n.txy], 1) - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+        assert pp.nodal_stresses(st, [0, 1, 2]).k2id == [1, 2, 0, 3, 4]
+    assert _fmt(n, 2) == "1.20000,0.00000\n2.20000,0.10000\n1.80000,1.00000\n0.50000,1.20000\n"
+    # pmsim's Gauss-point coordinates pin the default Tri3 rule: 3 interior points, r fastest
+    assert _fmt(pp.gauss_stresses(pp.read_state(0), [0, 1, 2]), 2) == (
+        "1.46667,0.18333\n0.88333,0.23333\n1.96667,0.23333\n0.28333,0.33333\n1.18333,0.36667\n"
+        "1.76667,0.68333\n0.53333,0.83333\n1.48333,0.86667\n0.83333,0.96667\n")
+    assert _fmt(pp.gauss_stresses(pp.read_state(0), [0, 1, 2], flt), 2) == (
+        "1.46667,0.18333\n0.88333,0.23333\n1.96667,0.23333\n1.18333,0.36667\n1.76667,0.68333\n"
+        "0.53333,0.83333\n1.48333,0.86667\n0.83333,0.96667\n")
+
+
+def test_gauss_and_nodal_stresses_3d():
+    """post_processing.rs *_work_3d on pmsim's artificial-elastic-3d files (two Hex8, 8 IPs each)."""
+    pp = post.PostProc(ART, "artificial-elastic-3d")
+    flt = lambda x, y, z: not (x < 0.5 and y < 0.5)
+    for index in range(3):
+        st = pp.read_state(index)
+        ref = pp.gauss_stress(st, 0)[0]
+        n = pp.nodal_stresses(st, [0, 1], flt)
+        assert n.k2id == [1, 3, 2, 5, 7, 6, 9, 11, 10]
+        vals = np.stack([n.txx, n.tyy, n.tzz, n.txy, n.tyz, n.tzx], 1)
+        assert np.abs(vals - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+        assert pp.nodal_stresses(st, [0, 1]).k2id == [0, 1, 3, 2, 4, 5, 7, 6, 8, 9, 11, 10]
+    g = pp.gauss_stresses(pp.read_state(0), [0, 1], flt)
+    assert _fmt(g, 3) == ("0.78868,0.21132,0.21132\n0.21132,0.78868,0.21132\n0.78868,0.78868,0.21132\n"
+                          "0.78868,0.21132,0.78868\n0.21132,0.78868,0.78868\n0.78868,0.78868,0.78868\n"
+                          "0.78868,0.21132,1.21132\n0.21132,0.78868,1.21132\n0.78868,0.78868,1.21132\n"
+                          "0.78868,0.21132,1.78868\n0.21132,0.78868,1.78868\n0.78868,0.78868,1.78868\n")
+    assert _fmt(n, 3) == ("1.00000,0.00000,0.00000\n0.00000,1.00000,0.00000\n1.00000,1.00000,0.00000\n"
+                          "1.00000,0.00000,1.00000\n0.00000,1.00000,1.00000\n1.00000,1.00000,1.00000\n"
+                          "1.00000,0.00000,2.00000\n0.00000,1.00000,2.00000\n1.00000,1.00000,2.00000\n")
+
+
+def test_nodal_stress_qua8_least_squares():
+    """artificial-elastic-2d-qua8 (four Qua8, 9 IPs: least-squares extrapolation) reproduces the uniform
+    stress at every node."""
+    pp = post.PostProc(ART, "artificial-elastic-2d-qua8")
+    for index in range(3):
+        st = pp.read_state(index)
+        ref = pp.gauss_stress(st, 0)[0]
+        for cell in range(4):
+            assert np.abs(pp.nodal_stress(st, cell) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def write_oracle_run(case, tmp_path, stem):
+    """Oracle solution written as pmsim result files."""
+    mesh, base, cfg, ess, nat = case
+    d = cases.flat(case)
+    u, ip, _ = orc.Oracle(d).solve()
+    nip = [int(d["params"][int(a)][5]) or M.DEFAULT_NGAUSS[int(k)] for k, a in zip(d["kinds"], d["attrs"])]
+    model = [int(d["params"][int(a)][0]) for a in d["attrs"]]
+    io = fio.FileIo(str(tmp_path), stem)
+    io.write_setup(mesh, d)
+    for k in range(len(u)):
+        uk = u[k][: mesh.npoint * mesh.ndim]
+        io.write_state(fio.state_dict(mesh.ndim, float(k), 1.0, uk, np.zeros_like(uk), ip[k], nip, model))
+    io.write_summary()
+    return mesh
+
+
+def footing_curve(pp, mesh):
+    feat = Features(mesh)
+    footing = feat.search_edges(At.Y(500.0), lambda x: x[0] <= 50.0)
+    corner = feat.search_point_ids(At.XY(0.0, 500.0))[0]
+    # examples/spo_754_footing_analysis.rs: 2B = 100, c = 100 z_ini / sqrt 3 (the factor 100 is 2B)
+    return post.footing_load_settlement(pp, footing, corner, 100.0, 848.7 * 100.0 / math.sqrt(3.0))
+
+
+def test_footing_load_settlement(tmp_path):
+    """examples/spo_754_footing_analysis.rs on the oracle's solution: the normalised pressure follows
+    de Souza Neto et al.'s curve within 2% at every load level and its plateau is the Prandtl limit
+    2 + pi = 5.14 within 0.5%."""
+    mesh = write_oracle_run(cases.spo_754_footing(), tmp_path, "footing")
+    s, p = footing_curve(post.PostProc(str(tmp_path), "footing"), mesh)
+    book = np.loadtxt(os.path.join(cases.GOLDEN, "spo_754_footing_load_displacement.tsv"), skiprows=1)
+    assert len(s) == len(book)
+    assert np.abs(s[1:] - book[1:, 0]).max() / book[1:, 0].max() < 0.02
+    assert (np.abs(p[1:] - book[1:, 1]) / book[1:, 1]).max() < 0.02
+    assert abs(p[-4:].mean() - (2.0 + math.pi)) / (2.0 + math.pi) < 0.005
+
+
+def test_values_along_edges_plate(tmp_path):
+    """values_along_edges over the plate's bottom edges starts at the centre (smaller y, then x) and
+    its first value is the central deflection (tests/test_spo_753_circ_plate.rs:129-141)."""
+    mesh = write_oracle_run(cases.spo_753_circ_plate(), tmp_path, "plate")
+    pp = post.PostProc(str(tmp_path), "plate")
+    bottom = Features(mesh).search_edges(At.Y(0.0))
+    st = pp.read_state(1)
+    ids, coords, dd = pp.values_along_edges(st, bottom, 1)
+    assert coords[0] == [0.0, 0.0] and coords[-1] == [10.0, 0.0]
+    assert [c[0] for c in coords] == sorted(c[0] for c in coords)
+    prof = np.loadtxt(os.path.join(cases.GOLDEN, "spo_753_profiles.tsv"), skiprows=1)
+    assert abs(-dd[0] - prof[0, 1]) <= 0.0006  # P = 100
+    ids_x, xs, vals = pp.values_along_x(Features(mesh), st, 1, 0.0)
+    assert xs == sorted(xs) and vals[0] == dd[0]
