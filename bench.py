@@ -26,12 +26,12 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense FP64 /
 FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
 
 
-def build_problem(nx, ny, nz, precision="f64", kernel="default", scatter="blocks"):
+def build_problem(nx, ny, nz, precision="f64", kernel="default"):
     from pmsim_amd import Config, Essential, FemBase, Natural, ParamSolid, StressStrain, describe
     from pmsim_amd import generators as G
     mesh = G.hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=20261203, jitter=0.05)
     base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
-    cfg = Config(mesh).set_tangent_precision(precision).set_tangent_kernel(kernel).set_scatter_kernel(scatter)
+    cfg = Config(mesh).set_tangent_precision(precision).set_tangent_kernel(kernel)
     return mesh, base, cfg, describe(mesh, base, cfg, Essential(), Natural())
 
 
@@ -238,7 +238,6 @@ def main():
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
-    ap.add_argument("--scatter", default="blocks", choices=["blocks", "rows"], help="K gather: by CSR blocks or by rows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision"])
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
@@ -265,7 +264,7 @@ def main():
         return run_serve(a)
     from pmsim_amd.solver import Fem
     t0 = time.time()
-    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel, a.scatter)
+    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel)
     f = Fem(d, cfg, keep_history=False)
     setup_s = time.time() - t0
     print(f"setup: {mesh.ncell} cells, {f.nip} IPs, {f.ndof} DOFs, {f.nnzb} blocks, {setup_s:.1f} s", file=sys.stderr, flush=True)
@@ -304,10 +303,13 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved_tf / PEAK_TFLOPS[a.precision], "traffic": traffic,
                      "flop_per_ip": FLOP_PER_IP},
     }
-    # the scatter is the HBM-bound half of the assembly: element blocks in, CSR blocks out, block map
-    nblk_el = f.nip // 27 * 400
-    scatter_bytes = nblk_el * 9 * (8 if a.precision == "f64" else 4) + f.nnzb * 9 * 8 * (1 if a.precision == "f64" else 1.5) \
-        + nblk_el * 4 + (f.nnzb + 1) * 4
+    # the scatter is the HBM-bound half of the assembly: the stored element matrices in (upper node blocks,
+    # 210 of the 400 per Hex20 element, 1,892 values with padding), CSR blocks out (FP64, plus the FP32
+    # copy for reduced-precision tangents), one map entry per element block pair and the map pointers
+    nel = f.nip // 27
+    ke_values = nel * (((20 * 21 // 2) * 9 + 3) // 4 * 4)
+    scatter_bytes = ke_values * (8 if a.precision == "f64" else 4) + f.nnzb * 9 * 8 * (1 if a.precision == "f64" else 1.5) \
+        + nel * 400 * 4 + (f.nnzb + 1) * 4
     out["hbm"] = {"kernel": "k_scatter", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": scatter_bytes}
     if not a.no_cpu:

@@ -54,8 +54,6 @@ typedef struct pm_config {
   int32_t pcg_small_max;            /* systems with at most this many DOFs run the whole PCG in one workgroup; < 0: default */
   int32_t verbose;                  /* to stderr; bit 0: a line per converged step, bit 1: a line per Newton iteration
                                        (pmsim's verbose_timesteps / verbose_iterations, config.rs:166-173; default 0) */
-  int32_t scatter_kernel;           /* K gather: 0 (default) one thread per CSR block, 1 one wave per CSR row reading
-                                       whole element block rows; both sum in ascending cell order (same bits) */
 } pm_config;
 
 /* Statistics of the last pm_solve / element pass. */

@@ -42,7 +42,6 @@ void pm_default_config(pm_config* c) {
   c->tangent_kernel = 0;
   c->use_graphs = 1;
   c->pcg_small_max = -1;
-  c->scatter_kernel = 0;
 }
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
@@ -90,7 +89,6 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
   if (c->tangent_kernel < 0 || c->tangent_kernel > 3) { f->err = "unknown tangent kernel"; return 1; }
-  if (c->scatter_kernel < 0 || c->scatter_kernel > 1) { f->err = "unknown scatter kernel"; return 1; }
   const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   if (f->initialized) release_graphs(f);  // captured launches depend on the configuration
   f->cfg = *c;

@@ -75,18 +75,13 @@ int device_alloc(pm_fem* f) {
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
   {
-    const int64_t nc = (int64_t)f->h_rsrc.size(), npos = (int64_t)f->h_rpos.size();
-    if (dalloc(f, &f->d_rsrc, nc) || upload(f, f->d_rsrc, f->h_rsrc.data(), nc)) return 1;
-    if (dalloc(f, &f->d_rposoff, nc) || upload(f, f->d_rposoff, f->h_rposoff.data(), nc)) return 1;
-    if (dalloc(f, &f->d_rnn, nc) || upload(f, f->d_rnn, f->h_rnn.data(), nc)) return 1;
-    if (dalloc(f, &f->d_rpos, npos) || upload(f, f->d_rpos, f->h_rpos.data(), npos)) return 1;
   }
   if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
   if (dalloc(f, &f->d_kval, f->nnzb * b2)) return 1;
   if (f->cfg.tangent_precision == 0) {
-    if (dalloc(f, &f->d_ke, f->n_ke_blocks * b2)) return 1;
+    if (dalloc(f, &f->d_ke, f->n_ke_values)) return 1;
   } else {
-    if (dalloc(f, &f->d_ke32, f->n_ke_blocks * b2) || dalloc(f, &f->d_kval32, f->nnzb * b2)) return 1;
+    if (dalloc(f, &f->d_ke32, f->n_ke_values) || dalloc(f, &f->d_kval32, f->nnzb * b2)) return 1;
   }
   if (dalloc(f, &f->d_fe, f->n_fe)) return 1;
   // vectors
@@ -116,10 +111,6 @@ int device_alloc(pm_fem* f) {
   std::vector<int32_t>().swap(f->h_kmap_ptr);
   std::vector<int32_t>().swap(f->h_fmap);
   std::vector<int32_t>().swap(f->h_fmap_ptr);
-  std::vector<int32_t>().swap(f->h_rsrc);
-  std::vector<int32_t>().swap(f->h_rposoff);
-  std::vector<uint8_t>().swap(f->h_rnn);
-  std::vector<uint16_t>().swap(f->h_rpos);
   return 0;
 }
 
@@ -129,8 +120,6 @@ void device_free(pm_fem* f) {
   for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
   fr(f->d_X); fr(f->d_browptr); fr(f->d_bcols); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
-  fr(f->d_rsrc); fr(f->d_rposoff); fr(f->d_rnn); fr(f->d_rpos);
-  f->d_rsrc = f->d_rposoff = nullptr; f->d_rnn = nullptr; f->d_rpos = nullptr;
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
   fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
   if (f->stream) (void)hipStreamDestroy(f->stream);

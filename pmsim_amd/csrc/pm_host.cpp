@@ -31,7 +31,7 @@ int build_groups(pm_fem* f) {
       Rule r;
       if (!gauss_rule(k, ng, r)) { f->err = "requested number of integration points is not available"; return 1; }
       g.nip = r.n();
-      g.nblk = g.nnode * g.nnode;
+      g.kew = packed_ke_values(g.nnode, g.ndim);
       const ParamSolid& p = g.param;
       if (p.model == 1) {
         if (f->plane_stress) { f->err = "von Mises model does not work in plane-stress"; return 1; }
@@ -56,10 +56,10 @@ int build_groups(pm_fem* f) {
   for (auto& g : f->groups) {
     g.nel = (int64_t)g.cells.size();
     g.fe_base = fe; fe += g.nel * g.nnode * g.ndim;
-    g.ke_base = ke; ke += g.nel * g.nblk;
+    g.ke_base = ke; ke += g.nel * g.kew;
   }
   f->n_fe = fe;
-  f->n_ke_blocks = ke;
+  f->n_ke_values = ke;
   // every point must belong to a cell (pure solid DOF map)
   std::vector<uint8_t> used(f->npoint, 0);
   for (auto p : f->cpts) {
@@ -149,7 +149,7 @@ int build_patterns(pm_fem* f) {
   // per element, block index of every (la, lb) local pair
   int64_t total = 0;
   for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; total += n * n; }
-  if (f->n_ke_blocks >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
+  if (f->n_ke_values >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
   std::vector<int32_t> pairblk(total);
   std::vector<int64_t> pbase(f->ncell + 1, 0);
   for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; pbase[e + 1] = pbase[e] + n * n; }
@@ -172,58 +172,23 @@ int build_patterns(pm_fem* f) {
       cell_ranks(f, e, rk);
       for (int64_t la = 0; la < n; la++)
         for (int64_t lb = 0; lb < n; lb++) {
-          // the element stores node block (la, lb) at rank position (rank[la], rank[lb])
-          int64_t blk = g.ke_base + el * g.nblk + (int64_t)rk[la] * n + rk[lb];
+          // node block (la, lb) is the stored upper block (rank[la], rank[lb]) or, below the diagonal,
+          // the transpose of (rank[lb], rank[la]): value offset, sign bit set for a transposed read
+          const int ra = rk[la], rb = rk[lb];
+          const int64_t off = g.ke_base + el * g.kew + (int64_t)packed_block((int)n, std::min(ra, rb), std::max(ra, rb)) * g.ndim * g.ndim;
           int32_t pb = pairblk[pbase[e] + la * n + lb];
-          kmap[fill[pb]++] = (int32_t)blk;
+          kmap[fill[pb]++] = ra <= rb ? (int32_t)off : (int32_t)((uint32_t)off | 0x80000000u);
         }
     }
+  }
+  // every map entry must address a whole block inside the element-matrix buffer
+  for (int32_t m : kmap) {
+    const int64_t off = (int64_t)((uint32_t)m & 0x7fffffffu);
+    if (off + (int64_t)f->ndim * f->ndim > f->n_ke_values) { f->err = "internal error: K gather map out of range"; return 1; }
   }
   // upload happens in device_alloc; keep host copies in temporaries stored on the handle
   f->h_kmap_ptr.swap(kptr);
   f->h_kmap.swap(kmap);
-  // K gather by rows: node a's contributions are its cells in ascending order (nptr / ncells); the
-  // cell's block row of a is contiguous in rank order, and ranks follow global ids like the CSR columns,
-  // so the positions come from one merge of the two sorted lists
-  {
-    const int64_t nc = nptr[np];
-    f->h_rsrc.assign(nc, 0);
-    f->h_rposoff.assign(nc, 0);
-    f->h_rnn.assign(nc, 0);
-    std::vector<int64_t> poff(nc + 1, 0);
-    int32_t maxrow = 0;
-    for (int64_t a = 0; a < np; a++) {
-      maxrow = std::max(maxrow, f->h_browptr[a + 1] - f->h_browptr[a]);
-      for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) poff[k + 1] = poff[k] + (f->cptr[ncells[k] + 1] - f->cptr[ncells[k]]);
-    }
-    f->max_row_blocks = maxrow;
-    if (poff[nc] >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
-    f->h_rpos.assign(poff[nc], 0);
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t a = 0; a < np; a++) {
-      const int32_t* cols = f->h_bcols.data() + f->h_browptr[a];
-      const int32_t len = f->h_browptr[a + 1] - f->h_browptr[a];
-      std::vector<int32_t> sorted;
-      for (int64_t k = nptr[a]; k < nptr[a + 1]; k++) {
-        const int64_t e = ncells[k];
-        const Group& g = f->groups[f->cell_group[e]];
-        const int64_t n = g.nnode, el = f->cell_local[e];
-        int rk[32];
-        cell_ranks(f, e, rk);
-        f->h_rsrc[k] = (int32_t)(g.ke_base + el * g.nblk + (int64_t)rk[nloc[k]] * n);
-        f->h_rposoff[k] = (int32_t)poff[k];
-        f->h_rnn[k] = (uint8_t)n;
-        sorted.assign(f->cpts.begin() + f->cptr[e], f->cpts.begin() + f->cptr[e + 1]);
-        std::sort(sorted.begin(), sorted.end());  // rank order
-        int32_t j = 0;
-        for (int64_t r = 0; r < n; r++) {
-          while (j < len && cols[j] < sorted[r]) j++;
-          f->h_rpos[poff[k] + r] = (uint16_t)j;
-        }
-      }
-    }
-    if (maxrow > 65535) { f->err = "a node couples to more than 65535 nodes"; return 1; }
-  }
   // R gather map
   std::vector<int32_t> fptr(np + 1, 0), fmap(nptr[np]);
   for (int64_t a = 0; a < np; a++) {

@@ -42,6 +42,17 @@ struct ParamSolid {
 // Per-kind element batch on the device.
 struct PcgEngine;
 
+// Element matrices are stored as their upper node blocks (a <= b in rank order: nodes sorted by global
+// id), row after row: block (a, b) starts at packed_block(nn, a, b) * nd * nd, each block row-major.
+// The lower blocks are the transposes.  Each element's storage is padded to a multiple of 4 values
+// so that every element starts on a 16-byte boundary in FP32 and FP64.
+constexpr int packed_block(int nn, int a, int b) { return a * (2 * nn - a + 1) / 2 + (b - a); }
+constexpr int packed_ke_values(int nn, int nd) { return ((nn * (nn + 1) / 2 * nd * nd + 3) / 4) * 4; }
+// true when a group's element matrices, ew values each, lie inside the element-matrix buffer
+inline bool ke_range_ok(int64_t ke_base, int64_t nel, int kew, int ew, int64_t n_ke_values) {
+  return kew == ew && ke_base >= 0 && ke_base + nel * (int64_t)ew <= n_ke_values;
+}
+
 struct Group {
   int kind = 0, ndim = 2, nnode = 0, nip = 0, attribute = 0;
   ParamSolid param;
@@ -59,8 +70,8 @@ struct Group {
   double* d_state = nullptr;       // SoA: [nsig+4][nel*nip]  (sigma..., z, lambda, elastic, epbar)
   double* d_backup = nullptr;
   int64_t fe_base = 0;             // offset (in doubles) of this group's f_e in the element force buffer
-  int64_t ke_base = 0;             // offset (in blocks) of this group's K_e blocks in the block buffer
-  int nblk = 0;                    // nnode*nnode stored node blocks per element (both triangles, rank order)
+  int64_t ke_base = 0;             // offset (in values) of this group's element matrices
+  int kew = 0;                     // values per element matrix: packed upper node blocks, padded (packed_ke_values)
   int tune_epb = 0, tune_nqg = 0;  // tangent work split chosen by the tuner (0: heuristic)
   int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
@@ -116,10 +127,6 @@ struct pm_fem {
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
   // K gather by rows: per (node, cell) contribution in ascending cell order, the element's block row
   // (start block index, node count) and the CSR position of each of its blocks within the node's row
-  std::vector<int32_t> h_rsrc, h_rposoff;
-  std::vector<uint8_t> h_rnn;
-  std::vector<uint16_t> h_rpos;
-  int32_t max_row_blocks = 0;
   // device
   int device = 0;
   int n_cu = 256;                      // compute units of the device
@@ -133,15 +140,12 @@ struct pm_fem {
   int32_t* d_bcols = nullptr;
   double* d_kval = nullptr;            // nnzb*bs*bs (FP64)
   float* d_kval32 = nullptr;           // FP32 copy when tangent precision < FP64
-  double* d_ke = nullptr;              // element K blocks (full, rank order) FP64: nblocks_total*bs*bs
+  double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order, packed (n_ke_values)
   float* d_ke32 = nullptr;             // FP32 element blocks
-  int64_t n_ke_blocks = 0;
+  int64_t n_ke_values = 0;
+  bool launch_refused = false;         // a launch whose output range did not fit its buffer was skipped
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
   int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
-  int32_t* d_rsrc = nullptr;           // row gather: element block-row start per (node, cell)
-  int32_t* d_rposoff = nullptr;        // row gather: offset of the positions per (node, cell)
-  uint8_t* d_rnn = nullptr;            // row gather: blocks in the element row
-  uint16_t* d_rpos = nullptr;          // row gather: CSR position (within the row) of every element block
   double* d_fe = nullptr;              // element forces
   int64_t n_fe = 0;
   int32_t* d_fmap_ptr = nullptr;       // npoint+1

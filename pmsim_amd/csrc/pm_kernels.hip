@@ -499,10 +499,12 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     }
     n = h;
   }
-  // stage the full element matrix in rank order: block (ra, rb) at (ra*NN + rb)*B2
+  // stage the element matrix as its upper node blocks in rank order (packed_block); a block computed
+  // below the rank diagonal is stored transposed in its mirror position
   __syncthreads();
   KT* stage = reinterpret_cast<KT*>(sP);
-  constexpr int EW = NN * NN * B2;  // values per element
+  constexpr int EW = packed_ke_values(NN, ND);  // values per element (padded)
+  constexpr int EU = NN * (NN + 1) / 2 * B2;    // values in use
   if (active && qg == 0) {
     const uint8_t* rk = srank + el * NN;
 #pragma unroll
@@ -512,13 +514,15 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
         const int a = 2 * I + u, b = 2 * J + v;
         if (a >= NN || b >= NN || a > b) continue;
         const int ra = rk[a], rb = rk[b];
-        KT* o1 = stage + el * EW + (ra * NN + rb) * B2;
-        KT* o2 = stage + el * EW + (rb * NN + ra) * B2;
+        const bool up = ra <= rb;
+        KT* o = stage + el * EW + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
 #pragma unroll
         for (int i = 0; i < ND; i++)
 #pragma unroll
-          for (int j = 0; j < ND; j++) { o1[i * ND + j] = (KT)K[u][v][i][j]; o2[j * ND + i] = (KT)K[u][v][i][j]; }
+          for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)K[u][v][i][j];
       }
+    if (tile == 0)
+      for (int i = EU; i < EW; i++) stage[el * EW + i] = (KT)0;
   }
   __syncthreads();
   const int64_t nv = (int64_t)min((int64_t)epb, nel - e0) * EW;
@@ -527,71 +531,9 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 }
 
 // deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135): one thread
-// per CSR block sums its contributing element blocks in ascending cell order
-// K gather by rows: one wave per node row a.  For each cell of a (ascending), the cell's block row of a
-// is a contiguous run of nnode blocks (rank order); the wave reads it coalesced and adds every block into
-// the row accumulator in LDS at its CSR position.  Each CSR block thus receives its contributions in
-// ascending cell order, as in the per-block gather (identical sums); the finished row is written
-// contiguously.  Row contributions are the cells of the node: the same lists (and pointer array) as the
-// residual gather.
-template <int BS, typename KT>
-__global__ __launch_bounds__(256) void k_scatter_rows(int64_t np, int maxrow, const int32_t* __restrict__ browptr,
-                                                      const int32_t* __restrict__ cptr, const int32_t* __restrict__ rsrc,
-                                                      const int32_t* __restrict__ rposoff, const uint8_t* __restrict__ rnn,
-                                                      const uint16_t* __restrict__ rpos, const KT* __restrict__ ke,
-                                                      double* __restrict__ kval, float* __restrict__ kval32) {
-  constexpr int B2 = BS * BS;
-  constexpr int VPL = (20 * B2 + 63) / 64;  // values of one element block row per lane (at most 20 nodes)
-  constexpr int PF = 4;                     // contributions in flight
-  extern __shared__ double sacc[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t a = xcd_block() * (int64_t)(blockDim.x >> 6) + wave;
-  if (a >= np) return;  // whole wave; the kernel has no workgroup barrier
-  double* acc = sacc + (size_t)wave * maxrow * B2;
-  const int32_t r0 = browptr[a], nv = (browptr[a + 1] - r0) * B2;
-  for (int i = lane; i < nv; i += 64) acc[i] = 0.0;
-  const int32_t c0 = cptr[a], c1 = cptr[a + 1];
-  for (int32_t cc = c0; cc < c1; cc += 64) {
-    // one contribution's metadata per lane, broadcast below
-    const int nc = min(64, c1 - cc);
-    int msrc = 0, moff = 0, mnn = 0;
-    if (lane < nc) { msrc = rsrc[cc + lane]; moff = rposoff[cc + lane]; mnn = rnn[cc + lane]; }
-    for (int j0 = 0; j0 < nc; j0 += PF) {
-      double v[PF][VPL];
-      int tgt[PF][VPL];
-#pragma unroll
-      for (int j = 0; j < PF; j++) {
-        const int jj = min(j0 + j, nc - 1);
-        const int src = __builtin_amdgcn_readlane(msrc, jj), off = __builtin_amdgcn_readlane(moff, jj);
-        const int m = (j0 + j < nc) ? __builtin_amdgcn_readlane(mnn, jj) * B2 : 0;
-        const KT* __restrict__ sp = ke + (int64_t)src * B2;
-#pragma unroll
-        for (int t = 0; t < VPL; t++) {
-          const int i = lane + 64 * t;
-          tgt[j][t] = -1;
-          v[j][t] = 0.0;
-          if (i < m) {
-            const int rb = i / B2, comp = i - rb * B2;
-            tgt[j][t] = rpos[off + rb] * B2 + comp;
-            v[j][t] = (double)sp[i];
-          }
-        }
-      }
-      // ascending cell order per CSR block, as in the per-block gather
-#pragma unroll
-      for (int j = 0; j < PF; j++)
-#pragma unroll
-        for (int t = 0; t < VPL; t++)
-          if (tgt[j][t] >= 0) acc[tgt[j][t]] += v[j][t];
-    }
-  }
-  for (int i = lane; i < nv; i += 64) {
-    const double val = acc[i];
-    if (kval) kval[(int64_t)r0 * B2 + i] = val;
-    if (kval32) kval32[(int64_t)r0 * B2 + i] = (float)val;
-  }
-}
-
+// per CSR block sums its contributing element blocks in ascending cell order.  A map entry is the value
+// offset of a stored (upper, rank-order) element block; with the sign bit set the block is read
+// transposed (the contribution lies below the element's diagonal).
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ kptr, const int32_t* __restrict__ kmap,
                                                  const KT* __restrict__ ke, double* __restrict__ kval, float* __restrict__ kval32) {
@@ -608,18 +550,24 @@ __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __
     for (int32_t k = kptr[blk]; k < k1; k += PF) {
       int32_t idx[PF];
 #pragma unroll
-      for (int j = 0; j < PF; j++) idx[j] = (k + j < k1) ? kmap[k + j] : -1;
+      for (int j = 0; j < PF; j++) idx[j] = (k + j < k1) ? kmap[k + j] : 0;
       double v[PF][B2];
 #pragma unroll
       for (int j = 0; j < PF; j++) {
-        const KT* src = ke + (int64_t)(idx[j] < 0 ? 0 : idx[j]) * B2;
+        const KT* src = ke + (idx[j] & 0x7fffffff);
+        KT raw[B2];
 #pragma unroll
-        for (int c = 0; c < B2; c++) v[j][c] = idx[j] < 0 ? 0.0 : (double)src[c];
+        for (int c = 0; c < B2; c++) raw[c] = src[c];  // contiguous: the address does not depend on the flag
+        const bool tr = idx[j] < 0;
+#pragma unroll
+        for (int r = 0; r < BS; r++)
+#pragma unroll
+          for (int c = 0; c < BS; c++) v[j][r * BS + c] = (double)(tr ? raw[c * BS + r] : raw[r * BS + c]);
       }
-      // ascending contribution order, as before: the sums are unchanged bit for bit
+      // ascending contribution order
 #pragma unroll
       for (int j = 0; j < PF; j++)
-        if (idx[j] >= 0) {
+        if (k + j < k1) {
 #pragma unroll
           for (int c = 0; c < B2; c++) acc[c] += v[j][c];
         }
@@ -704,7 +652,7 @@ static size_t shm_tangent(int nd, int nn, int nip, int epb, int nqg, bool axisym
   size_t table = (size_t)nip * (1 + nn * nd + 1) + (axisym ? (size_t)nip * nn : 0);
   size_t pblk = (size_t)epb * nip * 2 * nh * pw;                 // node vectors
   size_t red = (size_t)(nqg / 2) * epb * nt * (4 * b2 + 1);      // largest halving-tree level
-  size_t stg = (size_t)epb * nn * nn * b2;                        // output staging (FP64 bound)
+  size_t stg = (size_t)epb * packed_ke_values(nn, nd);            // output staging (FP64 bound)
   size_t uni = std::max(pblk, std::max(red, stg));
   return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + uni);
 }
@@ -762,15 +710,19 @@ void launch_tangent(pm_fem* f, Group& g) {
   const int64_t grid = (g.nel + epb - 1) / epb;
   const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, nqg, f->axisym);
   const Mat mp = mat_of(f, g);
+  if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) {
+    f->launch_refused = true;  // reported by check_device_error; never launched out of range
+    return;
+  }
   if (f->cfg.tangent_precision == 0) {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base * (size_t)(g.ndim * g.ndim), mp,
+                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   } else {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base * (size_t)(g.ndim * g.ndim), mp,
+                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   }
 }
@@ -824,23 +776,7 @@ int tune_tangent(pm_fem* f) {
   return check_device_error(f);
 }
 
-constexpr int ROWS_PER_WG = 4;  // one wave per CSR row
-
-static size_t scatter_rows_shm(const pm_fem* f) { return sizeof(double) * ROWS_PER_WG * (size_t)f->max_row_blocks * f->bs * f->bs; }
-
 void launch_scatter(pm_fem* f) {
-  const size_t shm = scatter_rows_shm(f);
-  if (f->cfg.scatter_kernel == 1 && shm <= 48 * 1024) {
-    const unsigned grid = (unsigned)((f->npoint + ROWS_PER_WG - 1) / ROWS_PER_WG);
-    const bool lowp = f->cfg.tangent_precision != 0;
-#define PM_SR(BS, KT, KE)                                                                                                       \
-  hipLaunchKernelGGL((k_scatter_rows<BS, KT>), dim3(grid), dim3(64 * ROWS_PER_WG), shm, f->stream, f->npoint, f->max_row_blocks,  \
-                     f->d_browptr, f->d_fmap_ptr, f->d_rsrc, f->d_rposoff, f->d_rnn, f->d_rpos, KE, f->d_kval, lowp ? f->d_kval32 : nullptr)
-    if (f->bs == 2) { if (lowp) PM_SR(2, float, f->d_ke32); else PM_SR(2, double, f->d_ke); }
-    else { if (lowp) PM_SR(3, float, f->d_ke32); else PM_SR(3, double, f->d_ke); }
-#undef PM_SR
-    return;
-  }
   const int64_t n = f->nnzb;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;
@@ -886,6 +822,11 @@ void dev_fill(pm_fem* f, double* x, double v, int64_t n) {
 }
 
 int check_device_error(pm_fem* f) {
+  if (f->launch_refused) {
+    f->launch_refused = false;
+    f->err = "internal error: an element-matrix range did not fit its buffer (kernel not launched)";
+    return 1;
+  }
   int32_t e = 0;
   PM_HIP(hipMemcpyAsync(&e, f->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
   PM_HIP(hipStreamSynchronize(f->stream));

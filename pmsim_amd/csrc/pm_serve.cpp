@@ -19,7 +19,7 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   if (!f) return 1;
   const int nd = f->ndim, ns = f->nsig;
   const int64_t b2 = (int64_t)nd * nd;
-  int64_t blocks = 0, nip = 0, nfe = 0, conn = 0;
+  int64_t blocks = 0, kev = 0, nip = 0, nfe = 0, conn = 0;
   for (int64_t e = 0; e < f->ncell; e++) {
     const int k = f->kind[e];
     const int nn = kind_nnode(k);
@@ -27,6 +27,7 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
     auto it = f->params.find(f->attr[e]);
     const int ng = (it != f->params.end() && it->second.ngauss > 0) ? it->second.ngauss : kind_default_ngauss(k);
     blocks += (int64_t)nn * nn;
+    kev += packed_ke_values(nn, nd);
     nip += ng;
     nfe += (int64_t)nn * nd;
     conn += nn;
@@ -37,10 +38,9 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   b += f->npoint * nd * 8;                       // coordinates
   b += conn * 5;                                  // connectivity + ranks
   b += nip * (ns + 4) * 8 * 2;                    // IP state + backup
-  b += blocks * b2 * ts;                          // element matrices
+  b += kev * ts;                                  // element matrices (packed upper blocks)
   b += blocks * (b2 * (8 + (ts == 4 ? 4 : 0)) + 4 + 4);  // K values (bound: nnzb <= element blocks), cols, map
   b += nfe * (8 + 4);                             // element forces + R map
-  b += blocks * 2 + conn * 9;                     // row-gather maps
   b += (f->npoint + 1) * 4 * 3;                   // row pointers
   b += ndof * 8 * 16 + np * 8 * 8;                // vectors, PCG work, multipliers
   b += f->npoint * b2 * 8;                        // block-Jacobi inverses

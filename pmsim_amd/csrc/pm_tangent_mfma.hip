@@ -111,7 +111,7 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   L.uni = (L.q + (size_t)nip * S::QW + 1) & ~(size_t)1;
   const size_t rows = ((size_t)L.nrow * S::NCP * L.rsk + 4 * (size_t)ipc) * sizeof(T);
   const size_t mt = (size_t)S::NCP * (S::NCP + 1) * sizeof(T);  // dense M, both triangles
-  const size_t kst = (size_t)NN * NN * ND * ND * sizeof(T);
+  const size_t kst = (size_t)packed_ke_values(NN, ND) * sizeof(T);
   size_t u = rows > mt ? rows : mt;
   u = u > kst ? u : kst;
   L.per_el = (L.uni + (u + sizeof(double) - 1) / sizeof(double) + 1) & ~(size_t)1;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
   using Acc = typename MT::Acc;
   constexpr int NS = S::NS, NC = S::NC, NTI = S::NTI, NCP = S::NCP, NTILE = S::NTILE, WPE = S::WPE, EPW = S::EPW;
   constexpr int TPW = S::TPW, TPE = S::TPE, NTHR = S::NTHR, QW = S::QW, SV = S::SV;
-  constexpr int KS = MT::KS, B2 = ND * ND, EW = NN * NN * B2;
+  constexpr int KS = MT::KS, B2 = ND * ND, EW = packed_ke_values(NN, ND), EU = NN * (NN + 1) / 2 * B2;
   extern __shared__ double shm[];
   __shared__ uint8_t srank[EPW * NN];
   const MLayout L = mfma_layout<ND, NN, PREC>(nip, ipc, axisym);
@@ -460,7 +460,8 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
       }
     }
     __syncthreads();  // M no longer needed: its space takes the node blocks
-    // epilogue 3: node blocks (a, b) at (a*NN + b)*B2 (rows and columns are already in rank order)
+    // epilogue 3: the upper node blocks (a <= b; rows and columns are already in rank order), packed
+    // (packed_block); the lower half of a diagonal block comes from its mirror entry
     T* sK = sL;
     if (active) {
 #pragma unroll
@@ -470,13 +471,14 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         for (int r = 0; r < 4; r++) {
           const int R = 16 * tIv[s] + MT::row(l, r), Cc = 16 * tJv[s] + (l & 15);
           if (R >= NC || Cc >= NC || R > Cc) continue;
-          const int i = R % ND, j = Cc % ND;
-          // (a*NN + b)*B2 + i*ND + j with a = (R - i)/ND, b = (Cc - j)/ND
+          const int i = R % ND, j = Cc % ND, a = R / ND, b = Cc / ND;
           const T v = accC[s][r];
-          sK[ND * (NN * (R - i) + (Cc - j)) + i * ND + j] = v;
-          sK[ND * (NN * (Cc - j) + (R - i)) + j * ND + i] = v;
+          T* blk = sK + packed_block(NN, a, b) * B2;
+          blk[i * ND + j] = v;
+          if (a == b) blk[j * ND + i] = v;
         }
       }
+      for (int i = EU + te; i < EW; i += TPE) sK[i] = (T)0;
     }
     __syncthreads();
     // epilogue 4: contiguous write of the element matrix in 16-byte pieces.  The trip count is a
@@ -522,10 +524,13 @@ bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
   const int64_t nwg = std::min<int64_t>(nunits, per_cu * f->n_cu);
   const int64_t upw = (nunits + nwg - 1) / nwg;
   const int64_t grid = (nunits + upw - 1) / upw;
-  const size_t b2 = (size_t)ND * ND;
-  T* ke;
-  if constexpr (PREC == 0) ke = f->d_ke + g.ke_base * b2;
-  else ke = f->d_ke32 + g.ke_base * b2;
+  if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(NN, ND), f->n_ke_values)) {
+    f->launch_refused = true;  // reported by check_device_error; never launched out of range
+    return true;
+  }
+  T* ke;  // ke_base counts values
+  if constexpr (PREC == 0) ke = f->d_ke + g.ke_base;
+  else ke = f->d_ke32 + g.ke_base;
   hipLaunchKernelGGL((k_tangent_mfma<ND, NN, PREC>), dim3((unsigned)grid), dim3(S::NTHR), shm, f->stream, g.nel, g.nip, ipc, upw,
                      g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err);
   return true;

@@ -233,7 +233,6 @@ class Config:
         self.pcg_small_max = -1  # DOF limit of the single-workgroup PCG (-1: library default)
         self.verbose_timesteps = False  # a line per converged step on stderr
         self.verbose_iterations = False  # a line per Newton iteration on stderr
-        self.scatter_kernel = "blocks"  # K gather: "blocks" (thread per CSR block) or "rows" (wave per CSR row)
         self.pcg_tol = 1e-12
         self.pcg_max_iter = 20000
         self.scaled_rr = 0.0           # > 0 enables ||R||inf < scaled_rr * ||F_int||inf as an extra test
@@ -274,7 +273,6 @@ class Config:
     def set_tangent_kernel(self, k: str): self.tangent_kernel = k; return self
     def set_use_graphs(self, v: bool): self.use_graphs = bool(v); return self
     def set_pcg_small_max(self, n: int): self.pcg_small_max = int(n); return self
-    def set_scatter_kernel(self, k: str): self.scatter_kernel = k; return self
     def set_verbose(self, timesteps: bool, iterations: bool): self.verbose_timesteps = timesteps; self.verbose_iterations = iterations; return self
     def set_pcg(self, tol=1e-12, max_iter=20000): self.pcg_tol = float(tol); self.pcg_max_iter = int(max_iter); return self
     def set_scaled_rr(self, v): self.scaled_rr = float(v); return self

@@ -40,7 +40,7 @@ class PmConfig(C.Structure):
         ("tol_rr_abs", C.c_double), ("tol_mdu_rel", C.c_double), ("symmetry_check_tolerance", C.c_double),
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
         ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
-        ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32), ("scatter_kernel", C.c_int32),
+        ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32),
     ]
 
 
@@ -145,7 +145,6 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
         c.use_graphs = 1 if cfg.use_graphs else 0
         c.pcg_small_max = int(cfg.pcg_small_max)
         c.verbose = (1 if cfg.verbose_timesteps else 0) | (2 if cfg.verbose_iterations else 0)
-        c.scatter_kernel = {"blocks": 0, "rows": 1}[cfg.scatter_kernel]
     return c
 
 

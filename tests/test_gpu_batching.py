@@ -127,29 +127,3 @@ def test_tuner_on_mixed_batches():
         f.element_pass()
         vals.append(f.get_csr()[2])
     assert rel_err(vals[1], vals[0]) < 1e-13
-
-
-@pytest.mark.parametrize("which", ["mixed", "hex20"])
-def test_row_and_block_gathers_identical(which):
-    """The K gather by rows (one wave per CSR row, element block rows read whole) and by blocks (one
-    thread per CSR block) add the same contributions in the same (ascending cell) order: same bits."""
-    from pmsim_amd import Essential, FemBase, Natural, ParamSolid, StressStrain
-    from pmsim_amd import generators as G
-    if which == "mixed":
-        mesh, base, cfg, ess, nat = mixed_compression(20_000)
-        d = describe(mesh, base, cfg, ess, nat)
-    else:
-        mesh = G.hex20_block(12, 10, 8, 1.0, 1.0, 1.0, seed=3, jitter=0.05)
-        base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
-        d = describe(mesh, base, Config(mesh), Essential(), Natural())
-    rng = np.random.default_rng(13)
-    ddu = rng.uniform(-2e-3, 2e-3, mesh.coords.size)
-    vals = {}
-    for k in ("rows", "blocks"):
-        for prec in ("f64", "f32"):
-            f = Fem(d, Config(mesh).set_scatter_kernel(k).set_tangent_precision(prec), keep_history=False)
-            f.set_ddu(ddu)
-            f.element_pass()
-            vals[(k, prec)] = f.get_csr()[2]
-    assert np.array_equal(vals[("rows", "f64")], vals[("blocks", "f64")])
-    assert np.array_equal(vals[("rows", "f32")], vals[("blocks", "f32")])
