@@ -157,6 +157,33 @@ def run_serve(a):
               file=sys.stderr, flush=True)
     du = [float(np.abs(u2 - u1).max() / max(np.abs(u1).max(), 1e-300))
           for u1, u2 in zip(runs["f64"]["u"], runs["f32"]["u"]) if u1 is not None and u2 is not None]
+    plain = None
+    if a.plain:
+        # the plain FP64 GPU form: the same requests in arrival order, one at a time on one stream
+        # (set-up + Newton + single-request PCG each), no concurrency and no batching; latencies follow
+        # from FIFO service of the same arrival times
+        order = np.argsort(arrivals, kind="stable")
+        t_free, lat, solve_s, up = 0.0, [], [], []
+        for i in order:
+            mesh, base, cfg, ess, nat, p = reqs[i]
+            cfg.set_scaled_rr(1e-10).set_pcg(1e-12, 50000).set_tangent_precision("f64")
+            t1 = time.time()
+            h = Fem(describe(mesh, base, cfg, ess, nat), cfg, keep_history=False)
+            h.solve()
+            u = h.get_u()[0]
+            del h
+            s = time.time() - t1
+            start = max(t_free, float(arrivals[i]))
+            t_free = start + s
+            lat.append(t_free - float(arrivals[i]))
+            solve_s.append(s)
+            u64 = runs["f64"]["u"][i]
+            if u64 is not None:
+                up.append(float(np.abs(u - u64).max() / max(np.abs(u).max(), 1e-300)))
+            print(f"plain {len(solve_s)}/{a.requests}: {mesh.ncell} cells, {s:.2f} s", file=sys.stderr, flush=True)
+        span = t_free - float(np.min(arrivals))
+        plain = {"solves_per_s": a.requests / span, "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                 "sum_solve_s": float(np.sum(solve_s)), "u_rel_diff_served_vs_alone_max": max(up) if up else None}
     f64 = runs["f64"]
     out = {"metric": "served solves/s", "value": f64["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": a.requests,
            "warmup": 0, "ms_per_step": 1e3 / f64["solves_per_s"], "higher_is_better": True, "scaling": "weak",
@@ -168,6 +195,10 @@ def run_serve(a):
            "fp64": {k: f64[k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
            "fp32_tangent": {k: runs["f32"][k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
            "u_rel_diff_fp32_vs_fp64": {"max": max(du) if du else None, "median": float(np.median(du)) if du else None}}
+    if plain is not None:
+        out["plain_fp64_sequential"] = plain
+        out["speedup_vs_plain"] = {"served_fp64": f64["solves_per_s"] / plain["solves_per_s"],
+                                   "served_fp32_tangent": runs["f32"]["solves_per_s"] / plain["solves_per_s"]}
     print(json.dumps(out), flush=True)
 
 
@@ -245,6 +276,7 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--max-active", type=int, default=16, help="serve: concurrently running requests")
+    ap.add_argument("--plain", action="store_true", help="serve: also time the plain FP64 form (one request at a time)")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="serve: device-memory budget for admission")
     ap.add_argument("--seed", type=int, default=20261203, help="serve: request and arrival seed")
     ap.add_argument("--elements", type=int, default=200_000, help="mixed2d: element count")
