@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pm_internal.hpp"
@@ -417,58 +418,69 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     J = I + rem;
   }
   double K[2][2][ND][ND];
+  double Dg[2][2];  // sum over the IPs of alpha/2 (Ga . Gb): added to the block diagonals after the loop
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      Dg[u][v] = 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; i++)
+#pragma unroll
+        for (int j = 0; j < ND; j++) K[u][v][i][j] = 0.0;
+    }
+  // per IP and 2x2 tile: three rank-1 updates per block (27 FMA in 3D) and the scaled dot product; the
+  // hoop terms only when axisymmetric (the loop is specialised so plane problems carry none of them)
+  auto accumulate = [&](auto axc) {
+    constexpr bool AX = decltype(axc)::value;
+    for (int q = qg; q < nip; q += nqg) {
+      const double* Q = sQ + (el * nip + q) * QW;
+      const double ha = Q[0], be = Q[1], ga = Q[2];
+      const double* Pq = sP + (el * nip + q) * NNP * PW;
+      double Ga[2][ND], tIa[2][ND], ta[2][ND], hA[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const double* Pa = Pq + (2 * I + u) * PW;
+#pragma unroll
+        for (int i = 0; i < ND; i++) { Ga[u][i] = Pa[i]; ta[u][i] = Pa[ND + 1 + i]; tIa[u][i] = Ga[u][i]; }
+        hA[u] = AX ? Pa[ND] : 0.0;
+        if (AX) tIa[u][0] += hA[u];
+      }
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        const double* Pb = Pq + (2 * J + v) * PW;
+        double ub[ND], vb[ND], wb[ND];
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+          ub[i] = ha * Pb[i];
+          vb[i] = be * Pb[i];
+          wb[i] = ga * Pb[ND + 1 + i];
+        }
+        double hh = 0.0;
+        if (AX) { const double hb = Pb[ND]; vb[0] += be * hb; hh = 2.0 * ha * hb; }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+#pragma unroll
+          for (int i = 0; i < ND; i++) {
+            Dg[u][v] += Ga[u][i] * ub[i];
+#pragma unroll
+            for (int j = 0; j < ND; j++) K[u][v][i][j] += ub[i] * Ga[u][j] + tIa[u][i] * vb[j] + ta[u][i] * wb[j];
+          }
+          if (AX) K[u][v][0][0] += hA[u] * hh;
+        }
+      }
+    }
+  };
+  if (active) {
+    if (axisym) accumulate(std::true_type{});
+    else accumulate(std::false_type{});
+  }
 #pragma unroll
   for (int u = 0; u < 2; u++)
 #pragma unroll
     for (int v = 0; v < 2; v++)
 #pragma unroll
-      for (int i = 0; i < ND; i++)
-#pragma unroll
-        for (int j = 0; j < ND; j++) K[u][v][i][j] = 0.0;
-  if (active) {
-    for (int q = qg; q < nip; q += nqg) {
-      const double* Q = sQ + (el * nip + q) * QW;
-      const double ha = Q[0], be = Q[1], ga = Q[2];
-      const double* Pq = sP + (el * nip + q) * NNP * PW;
-      double Ga[2][ND], ta[2][ND], hA[2];
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const double* Pa = Pq + (2 * I + u) * PW;
-#pragma unroll
-        for (int i = 0; i < ND; i++) { Ga[u][i] = Pa[i]; ta[u][i] = Pa[ND + 1 + i]; }
-        hA[u] = Pa[ND];
-      }
-#pragma unroll
-      for (int v = 0; v < 2; v++) {
-        const double* Pb = Pq + (2 * J + v) * PW;
-        double Gb[ND], ub[ND], vb[ND], wb[ND];
-#pragma unroll
-        for (int i = 0; i < ND; i++) Gb[i] = Pb[i];
-        const double hb = Pb[ND];
-#pragma unroll
-        for (int i = 0; i < ND; i++) {
-          ub[i] = ha * Gb[i];
-          vb[i] = be * (Gb[i] + (i == 0 ? hb : 0.0));
-          wb[i] = ga * Pb[ND + 1 + i];
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          double dot = 0.0;
-#pragma unroll
-          for (int i = 0; i < ND; i++) dot += Ga[u][i] * Gb[i];
-          const double hd = ha * dot;
-#pragma unroll
-          for (int i = 0; i < ND; i++) {
-            const double tIa = Ga[u][i] + (i == 0 ? hA[u] : 0.0);
-#pragma unroll
-            for (int j = 0; j < ND; j++) K[u][v][i][j] += ub[i] * Ga[u][j] + tIa * vb[j] + ta[u][i] * wb[j];
-            K[u][v][i][i] += hd;
-          }
-          K[u][v][0][0] += 2.0 * ha * hA[u] * hb;
-        }
-      }
-    }
-  }
+      for (int i = 0; i < ND; i++) K[u][v][i][i] += Dg[u][v];
   // sum the IP groups: halving tree through LDS (the node vectors are no longer needed)
   constexpr int TW = 4 * B2 + 1;  // doubles per tile, padded to an odd stride
   for (int n = nqg; n > 1;) {
