@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense FP64 / FP32 / BF16 matrix peaks
 FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
+# what the closed-form tangent executes per Hex20 IP: 210 upper node blocks x 30 FMA (three rank-1
+# updates and the diagonal dot product), the per-tile scalings (55 tiles x 18) and the per-IP set-up
+# (J, J^-1, D coefficients, node vectors: about 1,200)
+EXECUTED_FLOP_PER_IP = 210 * 30 * 2 + 55 * 18 + 1200
 
 
 def build_problem(nx, ny, nz, precision="f64", kernel="default"):
@@ -333,7 +337,8 @@ def main():
         "kernels_ms": {k: ms[k] for k in ("update", "residual", "tangent", "scatter", "total")},
         "roofline": {"bound": "mfma", "kernel": kname or "k_tangent", "achieved": achieved_tf, "peak": PEAK_TFLOPS[a.precision],
                      "unit": "TFLOP/s", "frac": achieved_tf / PEAK_TFLOPS[a.precision], "traffic": traffic,
-                     "flop_per_ip": FLOP_PER_IP},
+                     "flop_per_ip": FLOP_PER_IP, "executed_flop_per_ip": EXECUTED_FLOP_PER_IP,
+                     "executed_frac": EXECUTED_FLOP_PER_IP * nip / (ms["tangent"] * 1e-3) / 1e12 / PEAK_TFLOPS[a.precision]},
     }
     # the scatter is the HBM-bound half of the assembly: the stored element matrices in (upper node blocks,
     # 210 of the 400 per Hex20 element, 1,892 values with padding), CSR blocks out (FP64, plus the FP32

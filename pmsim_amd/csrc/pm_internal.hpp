@@ -72,7 +72,7 @@ struct Group {
   int64_t fe_base = 0;             // offset (in doubles) of this group's f_e in the element force buffer
   int64_t ke_base = 0;             // offset (in values) of this group's element matrices
   int kew = 0;                     // values per element matrix: packed upper node blocks, padded (packed_ke_values)
-  int tune_epb = 0, tune_nqg = 0;  // tangent work split chosen by the tuner (0: heuristic)
+  int tune_epb = 0, tune_ch = 0;   // tangent work split chosen by the tuner: elements per workgroup, IPs per chunk (0: heuristic)
   int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
 

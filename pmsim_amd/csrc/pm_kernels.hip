@@ -252,10 +252,12 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
 // with tI = B^T I, ts = B^T s, h = N/r (axisymmetric hoop term): three rank-1 terms and a diagonal per
 // node block (33 FMA per block and IP instead of the dense 6x6 triple product).
 //
-// Work split per workgroup: epb elements x nqg IP groups x NT upper 2x2 tiles of node blocks.  The node
-// vectors (G, h, ts) of every IP are staged in LDS first; each thread accumulates its tile over its IP
-// group in registers; the IP groups are summed by a halving tree in LDS; the element matrix is then
-// staged in LDS in rank order (nodes sorted by global id, both triangles) and written contiguously.
+// Work split per workgroup: epb elements x NT upper 2x2 tiles of node blocks, one thread per tile.  The
+// J, J^-1 and D coefficients of every IP of the workgroup's elements are formed first; the IPs are
+// then taken in chunks of ch: the node vectors (G, h, ts) of a chunk are staged in LDS and every thread
+// adds the chunk's contributions to its tile in registers, so no reduction across threads is needed.
+// The element matrices are finally staged in LDS one element at a time, as their upper node blocks in
+// rank order (nodes sorted by global id; packed_block), and written contiguously.
 // ------------------------------------------------------------------------------------------------
 template <int ND>
 struct NodeVec { static constexpr int W = 2 * ND + 1; };  // G(ND), h, ts(ND): odd stride in dwords/2 -> no LDS bank conflicts
@@ -271,7 +273,7 @@ struct TanShape {
 };
 
 template <int ND, int NN, typename KT>
-__global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, int nqg, const int32_t* __restrict__ conn,
+__global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, int ch, const int32_t* __restrict__ conn,
                                                  const uint8_t* __restrict__ rank, const double* __restrict__ X,
                                                  const double* __restrict__ tab, const double* __restrict__ st,
                                                  KT* __restrict__ ke, Mat mp, int axisym, double thickness, int32_t* err) {
@@ -279,13 +281,16 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH;
   constexpr int DS = NN * ND + 1;  // padded stride of one IP's dN rows in LDS
-  extern __shared__ double sh[];
+  constexpr int EW = packed_ke_values(NN, ND);  // values per stored element matrix (padded)
+  constexpr int EU = NN * (NN + 1) / 2 * B2;    // values in use
+  extern __shared__ __attribute__((aligned(16))) double sh[];
   double* sw = sh;                       // nip weights
   double* sdN = sw + nip;                // nip * DS
   double* sN = sdN + nip * DS;           // nip * NN (axisymmetric only)
   double* sX = sN + (axisym ? nip * NN : 0);
   double* sQ = sX + epb * NN * ND;
-  double* sP = sQ + epb * nip * QW;  // node vectors; later the reduction buffer and the output staging
+  // node vectors of one IP chunk; at the end the output staging (16-byte aligned for the copy out)
+  double* sP = sQ + ((epb * nip * QW + (int)(sQ - sh)) % 2 == 0 ? epb * nip * QW : epb * nip * QW + 1);
   __shared__ uint8_t srank[2 * TPB];  // epb * NN <= TPB * NN / NT <= 4/3 TPB
   const int tid = threadIdx.x;
   const int64_t e0 = xcd_block() * epb;
@@ -371,45 +376,8 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     for (int c = 0; c < 6; c++) Q[3 + ND * ND + c] = s[c];
     Q[3 + ND * ND + 6] = r;
   }
-  __syncthreads();
-  // node vectors of every (element, IP, node); the padding node of odd NN stays zero
-  for (int w = tid; w < epb * nip * NNP; w += blockDim.x) {
-    const int el = w / (nip * NNP), rem = w - el * nip * NNP, q = rem / NNP, m = rem - q * NNP;
-    double* P = sP + w * PW;
-    if (e0 + el >= nel || m >= NN) {
-#pragma unroll
-      for (int k = 0; k < PW; k++) P[k] = 0.0;
-      continue;
-    }
-    const double* Q = sQ + (el * nip + q) * QW;
-    const double* dN = sdN + q * DS + m * ND;
-    double G[ND];
-#pragma unroll
-    for (int j = 0; j < ND; j++) {
-      double sum = 0.0;
-#pragma unroll
-      for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
-      G[j] = sum;
-    }
-    const double* sv = Q + 3 + ND * ND;
-    double h = axisym ? tN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
-    if constexpr (ND == 2) {
-      double sxy = sv[3] * ISQ2;
-      P[0] = G[0]; P[1] = G[1]; P[2] = h;
-      P[3] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
-      P[4] = sxy * G[0] + sv[1] * G[1];
-    } else {
-      double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
-      P[0] = G[0]; P[1] = G[1]; P[2] = G[2]; P[3] = h;
-      P[4] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
-      P[5] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
-      P[6] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
-    }
-  }
-  __syncthreads();
-  // this thread: element el, IP group qg, tile (I <= J) -> blocks (2I+u, 2J+v)
-  const int per_el = nqg * NT;
-  const int el = tid / per_el, rem0 = tid - el * per_el, qg = rem0 / NT, tile = rem0 - qg * NT;
+  // this thread: element el, tile (I <= J) -> blocks (2I+u, 2J+v)
+  const int el = tid / NT, tile = tid - el * NT;
   const bool active = el < epb && (e0 + el) < nel;
   int I = 0, J = 0;
   {
@@ -431,12 +399,13 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     }
   // per IP and 2x2 tile: three rank-1 updates per block (27 FMA in 3D) and the scaled dot product; the
   // hoop terms only when axisymmetric (the loop is specialised so plane problems carry none of them)
-  auto accumulate = [&](auto axc) {
+  auto accumulate = [&](auto axc, int q0, int nq) {
     constexpr bool AX = decltype(axc)::value;
-    for (int q = qg; q < nip; q += nqg) {
-      const double* Q = sQ + (el * nip + q) * QW;
+#pragma unroll 1
+    for (int ql = 0; ql < nq; ql++) {
+      const double* Q = sQ + (el * nip + q0 + ql) * QW;
       const double ha = Q[0], be = Q[1], ga = Q[2];
-      const double* Pq = sP + (el * nip + q) * NNP * PW;
+      const double* Pq = sP + (el * ch + ql) * NNP * PW;
       double Ga[2][ND], tIa[2][ND], ta[2][ND], hA[2];
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -471,75 +440,93 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
       }
     }
   };
-  if (active) {
-    if (axisym) accumulate(std::true_type{});
-    else accumulate(std::false_type{});
+  // the chunk loop is specialised as a whole (the branch on the uniform axisym flag sits outside it, so
+  // only one variant's loop invariants are live)
+  auto chunks = [&](auto axc) {
+  for (int q0 = 0; q0 < nip; q0 += ch) {
+    const int nq = min(ch, nip - q0);
+    __syncthreads();  // J^-1 and D ready (first chunk); the previous chunk's node vectors consumed
+    // node vectors of every (element, IP of the chunk, node); the padding node of odd NN stays zero.
+    // One item at a time: the tile accumulators are live here
+#pragma unroll 1
+    for (int w = tid; w < epb * nq * NNP; w += blockDim.x) {
+      const int e = w / (nq * NNP), rem = w - e * nq * NNP, ql = rem / NNP, m = rem - ql * NNP;
+      double* P = sP + ((e * ch + ql) * NNP + m) * PW;
+      if (e0 + e >= nel || m >= NN) {
+#pragma unroll
+        for (int k = 0; k < PW; k++) P[k] = 0.0;
+        continue;
+      }
+      const int q = q0 + ql;
+      const double* Q = sQ + (e * nip + q) * QW;
+      const double* dN = sdN + q * DS + m * ND;
+      double G[ND];
+#pragma unroll
+      for (int j = 0; j < ND; j++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
+        G[j] = sum;
+      }
+      const double* sv = Q + 3 + ND * ND;
+      double h = axisym ? tN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
+      if constexpr (ND == 2) {
+        double sxy = sv[3] * ISQ2;
+        P[0] = G[0]; P[1] = G[1]; P[2] = h;
+        P[3] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
+        P[4] = sxy * G[0] + sv[1] * G[1];
+      } else {
+        double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
+        P[0] = G[0]; P[1] = G[1]; P[2] = G[2]; P[3] = h;
+        P[4] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
+        P[5] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
+        P[6] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
+      }
+    }
+    __syncthreads();
+    if (active) accumulate(axc, q0, nq);
   }
+  };
+  if (axisym) chunks(std::true_type{});
+  else chunks(std::false_type{});
 #pragma unroll
   for (int u = 0; u < 2; u++)
 #pragma unroll
     for (int v = 0; v < 2; v++)
 #pragma unroll
       for (int i = 0; i < ND; i++) K[u][v][i][i] += Dg[u][v];
-  // sum the IP groups: halving tree through LDS (the node vectors are no longer needed)
-  constexpr int TW = 4 * B2 + 1;  // doubles per tile, padded to an odd stride
-  for (int n = nqg; n > 1;) {
-    const int h = (n + 1) / 2;
-    __syncthreads();
-    if (active && qg >= h && qg < n) {
-      double* buf = sP + (((qg - h) * epb + el) * NT + tile) * TW;
-#pragma unroll
-      for (int u = 0; u < 2; u++)
-#pragma unroll
-        for (int v = 0; v < 2; v++)
-#pragma unroll
-          for (int i = 0; i < ND; i++)
-#pragma unroll
-            for (int j = 0; j < ND; j++) buf[((u * 2 + v) * ND + i) * ND + j] = K[u][v][i][j];
-    }
-    __syncthreads();
-    if (active && qg < n - h) {
-      const double* buf = sP + ((qg * epb + el) * NT + tile) * TW;
-#pragma unroll
-      for (int u = 0; u < 2; u++)
-#pragma unroll
-        for (int v = 0; v < 2; v++)
-#pragma unroll
-          for (int i = 0; i < ND; i++)
-#pragma unroll
-            for (int j = 0; j < ND; j++) K[u][v][i][j] += buf[((u * 2 + v) * ND + i) * ND + j];
-    }
-    n = h;
-  }
-  // stage the element matrix as its upper node blocks in rank order (packed_block); a block computed
-  // below the rank diagonal is stored transposed in its mirror position
-  __syncthreads();
+  // one element at a time: its upper node blocks in rank order (a block computed below the rank
+  // diagonal is stored transposed in its mirror position), then a contiguous 16-byte write
   KT* stage = reinterpret_cast<KT*>(sP);
-  constexpr int EW = packed_ke_values(NN, ND);  // values per element (padded)
-  constexpr int EU = NN * (NN + 1) / 2 * B2;    // values in use
-  if (active && qg == 0) {
-    const uint8_t* rk = srank + el * NN;
+  typedef KT V __attribute__((ext_vector_type(16 / sizeof(KT))));
+  constexpr int NV = EW * (int)sizeof(KT) / 16;
+  static_assert(EW * sizeof(KT) % 16 == 0, "element matrix is a whole number of 16-byte pieces");
+  for (int se = 0; se < epb && e0 + se < nel; se++) {
+    __syncthreads();  // node vectors / previous element's staging consumed
+    if (active && el == se) {
+      const uint8_t* rk = srank + el * NN;
 #pragma unroll
-    for (int u = 0; u < 2; u++)
+      for (int u = 0; u < 2; u++)
 #pragma unroll
-      for (int v = 0; v < 2; v++) {
-        const int a = 2 * I + u, b = 2 * J + v;
-        if (a >= NN || b >= NN || a > b) continue;
-        const int ra = rk[a], rb = rk[b];
-        const bool up = ra <= rb;
-        KT* o = stage + el * EW + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
+        for (int v = 0; v < 2; v++) {
+          const int a = 2 * I + u, b = 2 * J + v;
+          if (a >= NN || b >= NN || a > b) continue;
+          const int ra = rk[a], rb = rk[b];
+          const bool up = ra <= rb;
+          KT* o = stage + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
 #pragma unroll
-        for (int i = 0; i < ND; i++)
+          for (int i = 0; i < ND; i++)
 #pragma unroll
-          for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)K[u][v][i][j];
-      }
-    if (tile == 0)
-      for (int i = EU; i < EW; i++) stage[el * EW + i] = (KT)0;
+            for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)K[u][v][i][j];
+        }
+      if (tile == 0)
+        for (int i = EU; i < EW; i++) stage[i] = (KT)0;
+    }
+    __syncthreads();
+    V* dst = reinterpret_cast<V*>(ke + (e0 + se) * (int64_t)EW);
+    const V* src = reinterpret_cast<const V*>(stage);
+    for (int i = tid; i < NV; i += blockDim.x) dst[i] = src[i];
   }
-  __syncthreads();
-  const int64_t nv = (int64_t)min((int64_t)epb, nel - e0) * EW;
-  KT* dst = ke + e0 * EW;
-  for (int64_t i = tid; i < nv; i += blockDim.x) dst[i] = stage[i];
 }
 
 // deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135): one thread
@@ -659,23 +646,22 @@ static int epb_update(int nd, int nn, int nip) {
   while (epb > 1 && shm_update(nd, nn, nip, epb) > SHM_CAP) epb--;
   return epb;
 }
-static size_t shm_tangent(int nd, int nn, int nip, int epb, int nqg, bool axisym) {
-  int nh = (nn + 1) / 2, nt = nh * (nh + 1) / 2, pw = 2 * nd + 1, b2 = nd * nd;
+static size_t shm_tangent(int nd, int nn, int nip, int epb, int ch, bool axisym) {
+  int nh = (nn + 1) / 2, pw = 2 * nd + 1;
   size_t table = (size_t)nip * (1 + nn * nd + 1) + (axisym ? (size_t)nip * nn : 0);
-  size_t pblk = (size_t)epb * nip * 2 * nh * pw;                 // node vectors
-  size_t red = (size_t)(nqg / 2) * epb * nt * (4 * b2 + 1);      // largest halving-tree level
-  size_t stg = (size_t)epb * packed_ke_values(nn, nd);            // output staging (FP64 bound)
-  size_t uni = std::max(pblk, std::max(red, stg));
-  return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + uni);
+  size_t pblk = (size_t)epb * ch * 2 * nh * pw;                   // node vectors of one IP chunk
+  size_t stg = (size_t)packed_ke_values(nn, nd);                  // output staging, one element (FP64 bound)
+  size_t uni = std::max(pblk, stg);
+  return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + 1 + uni);  // +1: alignment
 }
-// work split of the tangent kernel: elements per workgroup and IP groups per element
-static void split_tangent(int nd, int nn, int nip, bool axisym, int& epb, int& nqg) {
+constexpr size_t TANGENT_LDS_TARGET = 52 * 1024;  // three workgroups per CU (160 KB LDS)
+// work split of the tangent kernel: elements per workgroup (one thread per tile) and IPs per chunk
+static void split_tangent(int nd, int nn, int nip, bool axisym, int& epb, int& ch) {
   int nh = (nn + 1) / 2, nt = nh * (nh + 1) / 2;
-  nqg = std::max(1, std::min(nip, TPB / nt));
-  if (nqg > 4) nqg = 4;
-  epb = std::max(1, TPB / (nt * nqg));
-  while (epb > 1 && shm_tangent(nd, nn, nip, epb, nqg, axisym) > SHM_CAP) epb--;
-  while (nqg > 1 && shm_tangent(nd, nn, nip, epb, nqg, axisym) > SHM_CAP) nqg--;
+  epb = std::max(1, TPB / nt);
+  ch = std::min(nip, 8);
+  while (ch > 1 && shm_tangent(nd, nn, nip, epb, ch, axisym) > TANGENT_LDS_TARGET) ch--;
+  while (epb > 1 && shm_tangent(nd, nn, nip, epb, ch, axisym) > TANGENT_LDS_TARGET) epb--;
 }
 
 #define PM_DISPATCH_KIND(KIND, ND, NN, ...)          \
@@ -716,11 +702,11 @@ static int tangent_kernel_of(const pm_fem* f, const Group& g) {
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
   if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
-  int epb, nqg;
-  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, nqg);
-  if (g.tune_epb > 0) { epb = g.tune_epb; nqg = g.tune_nqg; }
+  int epb, ch;
+  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, ch);
+  if (g.tune_epb > 0) { epb = g.tune_epb; ch = g.tune_ch; }
   const int64_t grid = (g.nel + epb - 1) / epb;
-  const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, nqg, f->axisym);
+  const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym);
   const Mat mp = mat_of(f, g);
   if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) {
     f->launch_refused = true;  // reported by check_device_error; never launched out of range
@@ -729,12 +715,12 @@ void launch_tangent(pm_fem* f, Group& g) {
   if (f->cfg.tangent_precision == 0) {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp,
+                                        epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   } else {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, nqg, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp,
+                                        epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp,
                                         f->axisym ? 1 : 0, f->thickness, f->d_err));
   }
 }
@@ -745,9 +731,9 @@ void launch_tangent(pm_fem* f, Group& g) {
 int tune_tangent(pm_fem* f) {
   const int mode = f->cfg.tangent_kernel;
   release_graphs(f);
-  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_epb = g.tune_nqg = 0; }
+  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_epb = g.tune_ch = 0; }
   if (mode != 3) return 0;
-  struct Cand { int kern, epb, nqg; };
+  struct Cand { int kern, epb, ch; };
   hipEvent_t ev0, ev1;
   PM_HIP(hipEventCreate(&ev0));
   PM_HIP(hipEventCreate(&ev1));
@@ -758,19 +744,20 @@ int tune_tangent(pm_fem* f) {
     cands.push_back({2, 0, 0});
     if (f->cfg.tangent_precision != 2) {
       const int nh = (g.nnode + 1) / 2, nt = nh * (nh + 1) / 2;
-      for (int nqg = 1; nqg <= std::min(g.nip, TPB / nt); nqg++) {
-        int emax = std::max(1, TPB / (nt * nqg));
-        for (int epb = emax; epb >= 1; epb = (epb > 1 ? epb / 2 : 0)) {
-          if (shm_tangent(g.ndim, g.nnode, g.nip, epb, nqg, f->axisym) > SHM_CAP) continue;
-          cands.push_back({1, epb, nqg});
-          break;  // the largest feasible epb for this nqg
+      const int emax = std::max(1, TPB / nt);
+      std::vector<int> epbs = {emax};
+      if ((emax + 1) / 2 < emax) epbs.push_back((emax + 1) / 2);
+      for (int epb : epbs)
+        for (int ch : {1, 2, 3, 4, 6, 9, 14}) {
+          if (ch > g.nip) continue;
+          if (shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym) > SHM_CAP) continue;
+          cands.push_back({1, epb, ch});
         }
-      }
     }
     float best = 1e30f;
     Cand pick = cands[0];
     for (const Cand& c : cands) {
-      g.tune_kernel = c.kern; g.tune_epb = c.epb; g.tune_nqg = c.nqg;
+      g.tune_kernel = c.kern; g.tune_epb = c.epb; g.tune_ch = c.ch;
       launch_tangent(f, g);  // warm-up
       if (hipEventRecord(ev0, f->stream) != hipSuccess) { rc = 1; break; }
       for (int r = 0; r < 3; r++) launch_tangent(f, g);
@@ -779,7 +766,7 @@ int tune_tangent(pm_fem* f) {
       (void)hipEventElapsedTime(&ms, ev0, ev1);
       if (ms < best) { best = ms; pick = c; }
     }
-    g.tune_kernel = pick.kern; g.tune_epb = pick.epb; g.tune_nqg = pick.nqg;
+    g.tune_kernel = pick.kern; g.tune_epb = pick.epb; g.tune_ch = pick.ch;
     if (rc) break;
   }
   (void)hipEventDestroy(ev0);
