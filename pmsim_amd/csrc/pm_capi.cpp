@@ -281,11 +281,8 @@ int pm_set_ip_state(pm_fem* f, const double* v) {
 int pm_get_csr(const pm_fem* f, int64_t* rowptr, int32_t* cols, double* vals) {
   if (!f->initialized) return 1;
   const int nd = f->ndim;
-  std::vector<double> kv((size_t)f->nnzb * nd * nd);
-  if (vals) {
-    if (hipMemcpyAsync(kv.data(), f->d_kval, kv.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
-    if (hipStreamSynchronize(f->stream) != hipSuccess) return 1;
-  }
+  std::vector<double> kv;
+  if (vals && copy_kval_to_host(f, kv)) return 1;
   int64_t k = 0;
   rowptr[0] = 0;
   for (int64_t a = 0; a < f->npoint; a++)

@@ -74,11 +74,11 @@ int device_alloc(pm_fem* f) {
   if (dalloc(f, &f->d_kmap_ptr, f->nnzb + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzb + 1)) return 1;
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
-  {
-  }
   if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
-  if (dalloc(f, &f->d_kval, f->nnzb * b2)) return 1;
+  // K values: FP64 for FP64 tangents; FP32 only for reduced-precision tangents (formed, stored and
+  // multiplied in FP32; readers that need FP64 widen them)
   if (f->cfg.tangent_precision == 0) {
+    if (dalloc(f, &f->d_kval, f->nnzb * b2)) return 1;
     if (dalloc(f, &f->d_ke, f->n_ke_values)) return 1;
   } else {
     if (dalloc(f, &f->d_ke32, f->n_ke_values) || dalloc(f, &f->d_kval32, f->nnzb * b2)) return 1;

@@ -260,15 +260,26 @@ void assemble_f_ext(pm_fem* f, int64_t k) {
 // ----------------------------------------------------------------------------------------------
 // host direct solver: envelope (skyline) Cholesky of K_ff after a reverse Cuthill-McKee ordering
 // ----------------------------------------------------------------------------------------------
+int copy_kval_to_host(const pm_fem* f, std::vector<double>& kv) {
+  const size_t n = (size_t)f->nnzb * f->ndim * f->ndim;
+  kv.resize(n);
+  if (f->d_kval)
+    return (hipMemcpyAsync(kv.data(), f->d_kval, n * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
+            hipStreamSynchronize(f->stream) != hipSuccess) ? 1 : 0;
+  std::vector<float> k32(n);
+  if (hipMemcpyAsync(k32.data(), f->d_kval32, n * sizeof(float), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
+      hipStreamSynchronize(f->stream) != hipSuccess)
+    return 1;
+  for (size_t i = 0; i < n; i++) kv[i] = (double)k32[i];
+  return 0;
+}
+
 int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x) {
   const int nd = f->ndim;
   const int64_t ndof = f->ndof;
-  // K values to host (FP64 copy of whatever precision the tangent used)
-  std::vector<double> kv((size_t)f->nnzb * nd * nd);
-  if (hipMemcpyAsync(kv.data(), f->d_kval, kv.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
-      hipStreamSynchronize(f->stream) != hipSuccess) {
-    f->err = "HIP error while copying K to the host"; return 1;
-  }
+  // K values to host (FP64, widened from FP32 for reduced-precision tangents)
+  std::vector<double> kv;
+  if (copy_kval_to_host(f, kv)) { f->err = "HIP error while copying K to the host"; return 1; }
   // free DOF numbering in RCM node order
   const int64_t np = f->npoint;
   std::vector<int32_t> order;

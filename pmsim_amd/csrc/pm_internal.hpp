@@ -138,8 +138,8 @@ struct pm_fem {
   double* d_X = nullptr;               // point coordinates
   int32_t* d_browptr = nullptr;
   int32_t* d_bcols = nullptr;
-  double* d_kval = nullptr;            // nnzb*bs*bs (FP64)
-  float* d_kval32 = nullptr;           // FP32 copy when tangent precision < FP64
+  double* d_kval = nullptr;            // nnzb*bs*bs (FP64), FP64 tangents only
+  float* d_kval32 = nullptr;           // nnzb*bs*bs (FP32), reduced-precision tangents only
   double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order, packed (n_ke_values)
   float* d_ke32 = nullptr;             // FP32 element blocks
   int64_t n_ke_values = 0;
@@ -192,6 +192,7 @@ int build_groups(pm_fem* f);
 int build_patterns(pm_fem* f);
 void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 void assemble_f_ext(pm_fem* f, int64_t station);
+int copy_kval_to_host(const pm_fem* f, std::vector<double>& kv);  // K block values in FP64 (widened from FP32)
 // device (pm_kernels.hip)
 int device_alloc(pm_fem* f);
 void device_free(pm_fem* f);

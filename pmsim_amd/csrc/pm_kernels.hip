@@ -781,10 +781,10 @@ void launch_scatter(pm_fem* f) {
   const bool lowp = f->cfg.tangent_precision != 0;
   if (f->bs == 2) {
     if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, f->d_kval, f->d_kval32);
+    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   } else {
     if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, f->d_kval, f->d_kval32);
+    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   }
 }
 

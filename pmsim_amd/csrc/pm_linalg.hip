@@ -90,10 +90,11 @@ __global__ __launch_bounds__(TPB) void k_spmv(int64_t np, const int32_t* __restr
   y[t] = mask ? s * mask[t] : s;
 }
 
-// block-Jacobi inverse of the diagonal node blocks, identity on prescribed rows/columns
-template <int BS>
+// block-Jacobi inverse of the diagonal node blocks (from the K values in the tangent's precision),
+// identity on prescribed rows/columns
+template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_bjacobi(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                 const double* __restrict__ vals, const double* __restrict__ free_, double* __restrict__ binv) {
+                                                 const KT* __restrict__ vals, const double* __restrict__ free_, double* __restrict__ binv) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= np) return;
   double A[BS][BS];
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(TPB) void k_bjacobi(int64_t np, const int32_t* __re
   for (int i = 0; i < BS; i++)
     for (int j = 0; j < BS; j++) {
       bool fi = free_[a * BS + i] != 0.0, fj = free_[a * BS + j] != 0.0;
-      A[i][j] = (fi && fj) ? vals[(int64_t)k * BS * BS + i * BS + j] : (i == j ? 1.0 : 0.0);
+      A[i][j] = (fi && fj) ? (double)vals[(int64_t)k * BS * BS + i * BS + j] : (i == j ? 1.0 : 0.0);
     }
   double Ai[BS][BS];
   if constexpr (BS == 2) {
@@ -487,8 +488,14 @@ void dev_spmv(pm_fem* f, const double* x, double* y, bool masked) {
 }
 
 void build_block_jacobi(pm_fem* f) {
-  if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2>), dim3(nblk(f->npoint)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
-  else hipLaunchKernelGGL((k_bjacobi<3>), dim3(nblk(f->npoint)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
+  const unsigned g = nblk(f->npoint);
+  if (f->d_kval) {
+    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
+    else hipLaunchKernelGGL((k_bjacobi<3, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
+  } else {
+    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, f->d_binv);
+    else hipLaunchKernelGGL((k_bjacobi<3, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, f->d_binv);
+  }
 }
 
 // Preconditioned CG on the free DOFs of K (prescribed entries of b ignored, of x zero).
