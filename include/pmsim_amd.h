@@ -145,7 +145,8 @@ int pm_get_f_int(const pm_fem* fem, double* f /* ndof */);
 /* IP state in cell order: per IP [sigma (nsig Mandel), z, lambda, elastic(0/1), epbar]. */
 int pm_get_ip_state(const pm_fem* fem, double* v);
 int pm_set_ip_state(pm_fem* fem, const double* v);
-/* Global K (ndof x ndof) in scalar CSR: rowptr (ndof+1), cols (nnz), vals (nnz); vals in FP64. */
+/* Global K (ndof x ndof) in scalar CSR: rowptr (ndof+1), cols (nnz), vals (nnz) as FP64 numbers (for
+ * FP32 / BF16 tangents K is held in FP32 and its values are returned widened). */
 int pm_get_csr(const pm_fem* fem, int64_t* rowptr, int32_t* cols, double* vals);
 /* History of output steps kept with keep_history: count, then u (n x ndof) and IP states. */
 int pm_history_count(const pm_fem* fem, int32_t* n);
