@@ -214,14 +214,20 @@ def run_precision(a):
     from pmsim_amd.solver import Fem
     from pmsim_amd.workloads import hex20_cube
     t0 = time.time()
-    mesh, base, cfg, ess, nat = hex20_cube(a.nx, a.ny, a.nz, n_inc=a.n_inc, uz=-0.001 * a.n_inc)
+    if a.uz_stations:
+        stations = [0.0] + [float(v) for v in a.uz_stations.split(",")]
+        mesh, base, cfg, ess, nat = hex20_cube(a.nx, a.ny, a.nz, stations=stations)
+        loading = "top uz at the stations " + ", ".join(f"{v:g}" for v in stations)
+    else:
+        mesh, base, cfg, ess, nat = hex20_cube(a.nx, a.ny, a.nz, n_inc=a.n_inc, uz=-0.001 * a.n_inc)
+        loading = f"{a.n_inc} increments of uz = -0.001"
     print(f"mesh: {mesh.ncell} cells, {mesh.npoint} points, {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     ref = None
     out = {"metric": "Hex20 inexact-Newton tangent precision", "n_gpus": 1, "data": "synthetic (jittered Hex20 cube, seeded)",
-           "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, {a.n_inc} increments of uz = -0.001",
+           "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, {loading}",
                       "elements": int(mesh.ncell)}, "runs": {}}
     for prec in a.precisions.split(","):
-        cfg.set_tangent_precision(prec).set_scaled_rr(1e-9).set_pcg(1e-9, 200000).set_n_max_iterations(30).set_verbose(True, True)
+        cfg.set_tangent_precision(prec).set_scaled_rr(1e-9).set_pcg(1e-9, a.pcg_max).set_n_max_iterations(30).set_verbose(True, True)
         d = describe(mesh, base, cfg, ess, nat)
         f = Fem(d, cfg, keep_history=False)
         print(f"{prec}: handle ready ({f.ndof} DOFs, {f.nnzb} K blocks), {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -277,6 +283,8 @@ def main():
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision"])
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
     ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first")
+    ap.add_argument("--uz-stations", default="", help="precision: top uz after each increment (comma-separated) instead of --n-inc")
+    ap.add_argument("--pcg-max", type=int, default=200000, help="precision: PCG iteration limit per linear solve")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--max-active", type=int, default=16, help="serve: concurrently running requests")

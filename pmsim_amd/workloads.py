@@ -6,9 +6,10 @@ from .mesh import At, Features
 from .problem import Config, Dof, Elem, Essential, FemBase, Natural, ParamSolid, StressStrain
 
 
-def hex20_cube(nx=64, ny=64, nz=32, n_inc=10, uz=-0.01, seed=20261203, jitter=0.05):
+def hex20_cube(nx=64, ny=64, nz=32, n_inc=10, uz=-0.01, seed=20261203, jitter=0.05, stations=None):
     """Unit cube of nx x ny x nz Hex20 (27 IP), corner jitter ±5% of h (seeded), von Mises E=2e5 nu=0.3
-    z0=250 H=1e3; bottom face fixed, top face uz ramping linearly to `uz` in n_inc increments."""
+    z0=250 H=1e3; bottom face fixed, top face uz ramping linearly to `uz` in n_inc increments, or taking
+    the values `stations` (station 0 first) when given."""
     mesh = hex20_block(nx, ny, nz, 1.0, 1.0, 1.0, seed=seed, jitter=jitter)
     feat = Features(mesh)
     bottom = feat.search_faces(At.Z(0.0))
@@ -16,9 +17,15 @@ def hex20_cube(nx=64, ny=64, nz=32, n_inc=10, uz=-0.01, seed=20261203, jitter=0.
     base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27)))])
     ess = Essential()
     ess.faces(bottom, Dof.Ux, 0.0).faces(bottom, Dof.Uy, 0.0).faces(bottom, Dof.Uz, 0.0)
-    ess.faces_fn(top, Dof.Uz, uz, lambda t: t / n_inc)
     cfg = Config(mesh)
-    cfg.set_lagrange_mult_method(True).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    if stations is None:
+        ess.faces_fn(top, Dof.Uz, uz, lambda t: t / n_inc)
+        cfg.set_incremental(n_inc + 1)
+    else:
+        uzs = [float(v) for v in stations]
+        ess.faces_fn(top, Dof.Uz, 1.0, lambda t: uzs[int(t)])
+        cfg.set_incremental(len(uzs))
+    cfg.set_lagrange_mult_method(True).set_n_max_iterations(20)
     return mesh, base, cfg, ess, Natural()
 
 
