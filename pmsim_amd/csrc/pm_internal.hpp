@@ -46,6 +46,7 @@ struct PcgEngine;
 // id), row after row: block (a, b) starts at packed_block(nn, a, b) * nd * nd, each block row-major.
 // The lower blocks are the transposes.  Each element's storage is padded to a multiple of 4 values
 // so that every element starts on a 16-byte boundary in FP32 and FP64.
+constexpr int GATHER_CHUNK = 256;  // CSR blocks per gather workgroup (its block size)
 constexpr int packed_block(int nn, int a, int b) { return a * (2 * nn - a + 1) / 2 + (b - a); }
 constexpr int packed_ke_values(int nn, int nd) { return ((nn * (nn + 1) / 2 * nd * nd + 3) / 4) * 4; }
 // true when a group's element matrices, ew values each, lie inside the element-matrix buffer
@@ -124,6 +125,7 @@ struct pm_fem {
   std::vector<int32_t> h_bcols;        // nnzb
   int64_t nnzb = 0;
   std::vector<int32_t> h_kmap_ptr, h_kmap;   // K gather map (host copy until upload)
+  std::vector<int32_t> h_korder;             // gather chunks (TPB CSR blocks each) in processing order
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
   // K gather by rows: per (node, cell) contribution in ascending cell order, the element's block row
   // (start block index, node count) and the CSR position of each of its blocks within the node's row
@@ -145,6 +147,7 @@ struct pm_fem {
   int64_t n_ke_values = 0;
   bool launch_refused = false;         // a launch whose output range did not fit its buffer was skipped
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
+  int32_t* d_korder = nullptr;         // gather chunk processing order
   int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
   double* d_fe = nullptr;              // element forces
   int64_t n_fe = 0;

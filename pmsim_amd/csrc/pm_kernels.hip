@@ -534,12 +534,13 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 // offset of a stored (upper, rank-order) element block; with the sign bit set the block is read
 // transposed (the contribution lies below the element's diagonal).
 template <int BS, typename KT>
-__global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ kptr, const int32_t* __restrict__ kmap,
-                                                 const KT* __restrict__ ke, double* __restrict__ kval, float* __restrict__ kval32) {
+__global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ korder, const int32_t* __restrict__ kptr,
+                                                 const int32_t* __restrict__ kmap, const KT* __restrict__ ke, double* __restrict__ kval,
+                                                 float* __restrict__ kval32) {
   constexpr int B2 = BS * BS;
-  constexpr int PF = 4;  // contributions whose map entries and blocks are in flight together
+  constexpr int PF = 2;  // contributions whose map entries and blocks are in flight together
   __shared__ double stage[TPB * B2];
-  const int64_t b0 = xcd_block() * (int64_t)blockDim.x;
+  const int64_t b0 = (int64_t)korder[xcd_block()] * blockDim.x;  // chunks in their processing order
   const int64_t blk = b0 + threadIdx.x;
   double acc[B2];
 #pragma unroll
@@ -775,16 +776,17 @@ int tune_tangent(pm_fem* f) {
   return check_device_error(f);
 }
 
+static_assert(TPB == GATHER_CHUNK, "the gather's chunk order assumes one workgroup of TPB blocks per chunk");
 void launch_scatter(pm_fem* f) {
   const int64_t n = f->nnzb;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;
   if (f->bs == 2) {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+    if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
+    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   } else {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+    if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
+    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   }
 }
 
