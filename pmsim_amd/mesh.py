@@ -181,6 +181,17 @@ class Features:
             raise ValueError("search_faces requires a 3D mesh")
         return self._search(at, flt or any_x)
 
+    def get_edge(self, a: int, b: int) -> Facet:
+        """The boundary edge whose end points are a and b (gemlab's Features::get_edge)."""
+        for f in self.boundary:
+            if sorted((int(f.points[0]), int(f.points[1]))) == sorted((int(a), int(b))):
+                return f
+        raise ValueError("cannot find edge with given key")
+
+    def cells_via_edges(self, edges: Sequence[Facet]) -> List[int]:
+        """Ids (ascending) of the cells that own the given boundary edges (gemlab's get_cells_via_2d_edges)."""
+        return sorted({int(f.cell) for f in edges})
+
     def search_point_ids(self, at, flt: Optional[Callable] = None) -> List[int]:
         """Ids (ascending) of the boundary points that satisfy ``at`` and the filter."""
         X = self.mesh.coords

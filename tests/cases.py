@@ -85,6 +85,82 @@ def spo_754_footing():
     return mesh, base, cfg, ess, Natural()
 
 
+def bhatti_1d6():
+    """tests/test_solid_bhatti_1d6_plane_stress.rs:38-51: four Tri3 bracket, plane stress (thickness
+    0.25), fixed at points 0 and 1, Qn = -20 on the edges (1,3) and (3,5)."""
+    mesh = smith_mesh("bhatti_example_1d6_bracket")
+    f = Features(mesh)
+    top = [f.get_edge(1, 3), f.get_edge(3, 5)]
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.LinearElastic(10_000.0, 0.2), None)))])
+    ess = Essential()
+    ess.points([0, 1], Dof.Ux, 0.0).points([0, 1], Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges(top, Nbc.Qn, -20.0)
+    cfg = Config(mesh)
+    cfg.set_plane_stress(0.25)
+    return mesh, base, cfg, ess, nat
+
+
+def felippa_thick_cylinder():
+    """tests/test_solid_felippa_thick_cylinder_axisym.rs:38-86: axisymmetric tube r in [4, 10], height 2
+    (two Qua8, 4 IPs), pressure 10 on the inner face, top and bottom fixed vertically."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "meshes", "test_solid_felippa_thick_cylinder_axisym.msh"))
+    f = Features(mesh)
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.LinearElastic(1000.0, 0.0), 4)))])
+    ess = Essential()
+    ess.edges(f.search_edges(At.Y(0.0)), Dof.Uy, 0.0).edges(f.search_edges(At.Y(2.0)), Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges(f.search_edges(At.X(4.0)), Nbc.Qn, -10.0)
+    cfg = Config(mesh)
+    cfg.set_axisymmetric()
+    return mesh, base, cfg, ess, nat
+
+
+def lame_tube_ur(r, p=10.0, rin=4.0, rout=10.0, young=1000.0, poisson=0.0):
+    """Radial displacement of a thick tube under internal pressure p (Lame; Felippa, Advanced Finite
+    Elements, Eq. 14.2): u_r = p rin^2 (1+nu) (rout^2 + (1-2nu) r^2) / (E (rout^2 - rin^2) r)."""
+    return p * rin * rin * (1.0 + poisson) * (rout * rout + (1.0 - 2.0 * poisson) * r * r) / (young * (rout * rout - rin * rin) * r)
+
+
+DF_B, DF_QN = 2.5, 200.0
+DF_W = DF_H = 15.0 * DF_B
+
+
+def durand_farias_example4():
+    """tests/test_durand_farias_example4.rs:22-66: flexible strip footing (half-width 2.5) on an elastic
+    layer 37.5 x 37.5 (Qua4, 4 IPs, E = 1000, nu = 0), Qn = -200 on the top edges with x <= 2.5."""
+    mesh = Mesh.read(os.path.join(GOLDEN, "meshes", "test_durand_farias_example4_qua4.msh"))
+    f = Features(mesh)
+    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.LinearElastic(1000.0, 0.0), 4)))])
+    ess = Essential()
+    ess.edges(f.search_edges(At.X(0.0)), Dof.Ux, 0.0).edges(f.search_edges(At.X(DF_W)), Dof.Ux, 0.0)
+    ess.edges(f.search_edges(At.Y(0.0)), Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges(f.search_edges(At.Y(DF_H), lambda x: x[0] <= DF_B), Nbc.Qn, -DF_QN)
+    return mesh, base, Config(mesh), ess, nat
+
+
+def strip_load_stress(x, y, b=DF_B, h=DF_H, q=DF_QN):
+    """Stresses (sxx, syy, sxy; compression negative) under a uniform strip load q on |x| <= b over an
+    elastic half-space, at depth h - y (Poulos & Davis 1974; Timoshenko & Goodier): with t1, t2 the
+    angles from the vertical to the strip edges, a = t2 - t1, d = t1,
+    syy = -(q/pi) [a + sin a cos(a + 2d)], sxx = -(q/pi) [a - sin a cos(a + 2d)],
+    sxy = -(q/pi) sin a sin(a + 2d) = -(q/pi) (sin^2 t2 - sin^2 t1)."""
+    z = h - y
+    if z == 0.0:
+        t1 = math.copysign(math.pi / 2.0, x - b) if x != b else 0.0
+        t2 = math.pi / 2.0
+    else:
+        t1 = math.atan((x - b) / z)
+        t2 = math.atan((x + b) / z)
+    a = t2 - t1
+    m = -q / math.pi
+    syy = m * (a + math.sin(a) * math.cos(a + 2.0 * t1))
+    sxx = m * (a - math.sin(a) * math.cos(a + 2.0 * t1))
+    sxy = m * (math.sin(t2) ** 2 - math.sin(t1) ** 2)
+    return sxx, syy, sxy
+
+
 P_751 = [0.0, 0.1, 0.14, 0.18, 0.19, 0.192]  # inner pressure per station
 
 

@@ -11,9 +11,12 @@ Outputs (data only: inputs and expected outputs):
 * *.tsv           -- SPO curves digitised from the book, shipped with the reference (data/spo/*.tsv):
                      plate deflection-load and deflection profiles (753), footing load-displacement (754)
 * sgm_5d17_ref.json -- Smith-Griffiths-Margetts Example 5.17 reference data (data/sgm/)
-* smith_meshes.json -- point coordinates and cell connectivity of the Smith sample meshes
-                     (src/base/sample_meshes.rs:261-735)
-* smith_u.json    -- expected displacement vectors of tests/test_solid_smith_*.rs
+* smith_meshes.json -- point coordinates and cell connectivity of the Smith sample meshes and Bhatti's
+                     Example 1.6 bracket (src/base/sample_meshes.rs:195-735)
+* smith_u.json    -- expected displacement vectors of tests/test_solid_smith_*.rs and
+                     tests/test_solid_bhatti_1d6_plane_stress.rs
+* meshes/*.msh    -- meshes of tests/test_solid_felippa_thick_cylinder_axisym.rs and
+                     tests/test_durand_farias_example4.rs (data/meshes/)
 * artificial/     -- state, summary, base and mesh files written by pmsim itself for three small linear
                      elastic runs (data/results/artificial/): the on-disk formats' known answers
 """
@@ -43,7 +46,7 @@ def spo(name):
 def meshes():
     src = open(os.path.join(REF, "src/base/sample_meshes.rs")).read()
     out = {}
-    for m in re.finditer(r"pub fn (smith_example_\w+)\(\) -> Mesh \{(.*?)\n    \}\n", src, re.S):
+    for m in re.finditer(r"pub fn (smith_example_\w+|bhatti_example_1d6_bracket)\(\) -> Mesh \{(.*?)\n    \}\n", src, re.S):
         name, body = m.group(1), m.group(2)
         ndim = int(re.search(r"ndim:\s*(\d+)", body).group(1))
         pts = {}
@@ -61,7 +64,7 @@ def meshes():
 def smith_vectors():
     out = {}
     for fn in sorted(os.listdir(os.path.join(REF, "tests"))):
-        if not fn.startswith("test_solid_smith_"):
+        if not (fn.startswith("test_solid_smith_") or fn.startswith("test_solid_bhatti_")):
             continue
         src = open(os.path.join(REF, "tests", fn)).read()
         m = re.search(r"let uu_correct = &\[(.*?)\];", src, re.S)
@@ -83,6 +86,10 @@ if __name__ == "__main__":
     shutil.copy(os.path.join(REF, "data/sgm/sgm_5d17_ref.json"), os.path.join(OUT, "sgm_5d17_ref.json"))
     meshes()
     smith_vectors()
+    os.makedirs(os.path.join(OUT, "meshes"), exist_ok=True)
+    for n in ["test_durand_farias_example4_qua4.msh", "test_solid_felippa_thick_cylinder_axisym.msh"]:
+        shutil.copy(os.path.join(REF, "data/meshes", n), os.path.join(OUT, "meshes", n))
+        os.chmod(os.path.join(OUT, "meshes", n), 0o644)
     art = os.path.join(OUT, "artificial")
     os.makedirs(art, exist_ok=True)
     for n in sorted(os.listdir(os.path.join(REF, "data/results/artificial"))):

@@ -46,3 +46,28 @@ def test_footing_load_settlement_from_solver_files(tmp_path):
     book = np.loadtxt(os.path.join(GOLDEN, "spo_754_footing_load_displacement.tsv"), skiprows=1)
     assert (np.abs(p[1:] - book[1:, 1]) / book[1:, 1]).max() < 0.02
     assert abs(p[-4:].mean() - (2.0 + math.pi)) / (2.0 + math.pi) < 0.005
+
+
+def test_durand_farias_example4_from_solver_files(tmp_path):
+    """tests/test_durand_farias_example4.rs: the HIP solver's files post-processed into nodal stresses
+    along the axis under the flexible footing, against the half-space solution within pmsim's
+    tolerances and against the same post-processing of the oracle's solution (1e-10 relative)."""
+    from pmsim_amd import At, Features
+    mesh, base, cfg, ess, nat = cases.durand_farias_example4()
+    io = fio.FileIo(str(tmp_path / "gpu"), "df4")
+    SolverImplicit(mesh, base, cfg, ess, nat).solve(FemState(mesh, base, ess, cfg), file_io=io)
+    write_oracle_run(cases.durand_farias_example4(), tmp_path / "orc", "df4")
+    feat = Features(mesh)
+    cells = feat.cells_via_edges(feat.search_edges(At.X(0.0)))
+    res = {}
+    for k in ("gpu", "orc"):
+        pp = post.PostProc(str(tmp_path / k), "df4")
+        res[k] = pp.nodal_stresses(pp.read_state(pp.n_state() - 1), cells, lambda x, y, z: x < 0.3)
+    g, o = res["gpu"], res["orc"]
+    assert g.k2id == o.k2id
+    for name in ("txx", "tyy", "txy"):
+        assert np.abs(getattr(g, name) - getattr(o, name)).max() <= 1e-10 * cases.DF_QN
+    for x, y, sxx, syy, sxy in zip(g.xx, g.yy, g.txx, g.tyy, g.txy):
+        a = cases.strip_load_stress(x, y)
+        assert abs(sxx - a[0]) / cases.DF_QN <= 0.25 and abs(syy - a[1]) / cases.DF_QN <= 0.23
+        assert abs(sxy - a[2]) / cases.DF_QN <= 0.09

@@ -154,6 +154,27 @@ def test_spo_753_circ_plate(lin_sol):
     _check_spo(cases.spo_753_circ_plate(), "spo_753_circ_plate_ref", 2e-9, 1e-6, lin_sol)
 
 
+def test_bhatti_1d6_plane_stress():
+    """Plane stress with thickness 0.25 and Qn on two Tri3 edges: u at pmsim's 1e-15
+    (tests/test_solid_bhatti_1d6_plane_stress.rs:42-51); host direct solve."""
+    import json as _json
+    import os as _os
+    d, f, hist = _solve_pair(cases.bhatti_1d6(), "host")
+    ref = np.asarray(_json.load(open(_os.path.join(cases.GOLDEN, "smith_u.json")))["test_solid_bhatti_1d6_plane_stress"])
+    assert np.abs(hist[-1][1][: len(ref)] - ref).max() <= 1e-15
+
+
+def test_felippa_thick_cylinder_axisym():
+    """Axisymmetric Qua8 tube (4 IPs) under internal pressure: u_r at the bottom points against Lame at
+    pmsim's 1e-15 (tests/test_solid_felippa_thick_cylinder_axisym.rs:88-105); host direct solve."""
+    case = cases.felippa_thick_cylinder()
+    mesh = case[0]
+    d, f, hist = _solve_pair(case, "host")
+    u = hist[-1][1]
+    for p in cases.Features(mesh).search_point_ids(cases.At.Y(0.0)):
+        assert abs(u[2 * p] - cases.lame_tube_ur(mesh.coords[p][0])) < 1e-15
+
+
 def test_spo_755_tensile():
     _check_spo(cases.spo_755_tensile(), "spo_755_tensile_perf_plast_ref", 1e-7, 1.2e-5, "pcg")
 

@@ -117,6 +117,24 @@ def test_smith_displacements(name):
     assert np.abs(u[-1][: len(ref)] - ref).max() <= tol
 
 
+def test_bhatti_1d6_plane_stress():
+    """Plane stress with thickness (tests/test_solid_bhatti_1d6_plane_stress.rs:42-51, 1e-15)."""
+    u, _, _ = orc.Oracle(cases.flat(cases.bhatti_1d6())).solve()
+    ref = np.asarray(json.load(open(os.path.join(cases.GOLDEN, "smith_u.json")))["test_solid_bhatti_1d6_plane_stress"])
+    assert np.abs(u[-1][: len(ref)] - ref).max() <= 1e-15
+
+
+def test_felippa_thick_cylinder_axisym():
+    """Axisymmetric Qua8 tube against Lame's u_r at the bottom points
+    (tests/test_solid_felippa_thick_cylinder_axisym.rs:88-105, 1e-15)."""
+    case = cases.felippa_thick_cylinder()
+    mesh = case[0]
+    u, _, _ = orc.Oracle(cases.flat(case)).solve()
+    for p in cases.Features(mesh).search_point_ids(cases.At.Y(0.0)):
+        r = mesh.coords[p][0]
+        assert abs(u[-1][2 * p] - cases.lame_tube_ur(r)) < 1e-15
+
+
 def test_sgm_5d17_ip_stresses():
     """9-point Gauss ordering pinned by per-IP stresses of the axisymmetric SGM example."""
     o = orc.Oracle(cases.flat(cases.smith("5d17_qua4_axisym")))

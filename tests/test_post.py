@@ -166,6 +166,28 @@ def test_footing_load_settlement(tmp_path):
     assert abs(p[-4:].mean() - (2.0 + math.pi)) / (2.0 + math.pi) < 0.005
 
 
+def test_durand_farias_example4(tmp_path):
+    """tests/test_durand_farias_example4.rs:69-113: nodal stresses (4-IP Qua4, extrapolated and averaged)
+    along the axis under a flexible strip footing against the half-space solution, within pmsim's
+    tolerances relative to the load (0.25 sxx, 0.23 syy, 0.09 sxy; the domain is finite)."""
+    mesh = write_oracle_run(cases.durand_farias_example4(), tmp_path, "df4")
+    pp = post.PostProc(str(tmp_path), "df4")
+    feat = Features(mesh)
+    left_cells = feat.cells_via_edges(feat.search_edges(At.X(0.0)))
+    pts = mesh.coords[mesh.cell_points(left_cells[0])]
+    hdx = (pts[:, 0].max() - pts[:, 0].min()) / 2.0
+    st = pp.read_state(pp.n_state() - 1)
+    nodal = pp.nodal_stresses(st, left_cells, lambda x, y, z: x < hdx)
+    assert len(nodal.xx) >= 8 and np.all(np.diff(nodal.yy) > 0)
+    for x, y, sxx, syy, sxy in zip(nodal.xx, nodal.yy, nodal.txx, nodal.tyy, nodal.txy):
+        a = cases.strip_load_stress(x, y)
+        assert abs(sxx - a[0]) / cases.DF_QN <= 0.25
+        assert abs(syy - a[1]) / cases.DF_QN <= 0.23
+        assert abs(sxy - a[2]) / cases.DF_QN <= 0.09
+    g = pp.gauss_stresses(st, left_cells, lambda x, y, z: x < hdx)
+    assert len(g.yy) == 2 * len(left_cells)  # the IP column nearest the axis
+
+
 def test_values_along_edges_plate(tmp_path):
     """values_along_edges over the plate's bottom edges starts at the centre (smaller y, then x) and
     its first value is the central deflection (tests/test_spo_753_circ_plate.rs:129-141)."""
