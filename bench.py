@@ -67,8 +67,27 @@ def measured_traffic(nx, ny, nz, precision, kernel):
     return None, None
 
 
-def cpu_baseline(seconds=12.0):
-    """Oracle element pass (update + f_int + K per element) on one host core over a bounded sample."""
+def _host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """Cores this process may use: the affinity mask, capped by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else n
+
+
+def cpu_baseline(seconds=12.0, seconds_all=6.0):
+    """Oracle element pass (update + f_int + K per element) over a bounded sample: on one host core (the
+    reported value) and, as `all_cores`, with its OpenMP element loops on every core this process may use."""
     from oracle import oracle as orc
     mesh, base, cfg, d = build_problem(8, 8, 4)
     o = orc.Oracle(d)
@@ -78,8 +97,23 @@ def cpu_baseline(seconds=12.0):
     reps = max(1, int(seconds / max(t1, 1e-6)))
     t = o.time_element_pass(ddu, reps=reps, nthreads=1)
     nip = mesh.ncell * 27
-    return dict(value=nip * reps / t, unit="IP-updates/s", cores=1, kind="port",
-                sample=f"8x8x4 Hex20 (256 elements, {nip} IPs) x {reps} passes, serial C++ oracle (update+f_int+K)")
+    out = dict(value=nip * reps / t, unit="IP-updates/s", cores=1, kind="port",
+               sample=f"8x8x4 Hex20 (256 elements, {nip} IPs) x {reps} passes, serial C++ oracle (update+f_int+K)",
+               host=_host_cpu_model())
+    nt = _host_threads()
+    if nt > 1:
+        mesh2, _, _, d2 = build_problem(16, 16, 8)
+        o2 = orc.Oracle(d2)
+        o2.init_state()
+        ddu2 = increment(mesh2)
+        t1 = o2.time_element_pass(ddu2, reps=1, nthreads=nt)
+        reps2 = max(1, int(seconds_all / max(t1, 1e-6)))
+        t2 = o2.time_element_pass(ddu2, reps=reps2, nthreads=nt)
+        nip2 = mesh2.ncell * 27
+        out["all_cores"] = dict(value=nip2 * reps2 / t2, unit="IP-updates/s", cores=nt,
+                                sample=f"16x16x8 Hex20 ({mesh2.ncell} elements, {nip2} IPs) x {reps2} passes, "
+                                       f"C++ oracle with OpenMP element loops on {nt} threads")
+    return out
 
 
 def run_mixed2d(a):
