@@ -54,6 +54,10 @@ typedef struct pm_config {
   int32_t pcg_small_max;            /* systems with at most this many DOFs run the whole PCG in one workgroup; < 0: default */
   int32_t verbose;                  /* to stderr; bit 0: a line per converged step, bit 1: a line per Newton iteration
                                        (pmsim's verbose_timesteps / verbose_iterations, config.rs:166-173; default 0) */
+  int32_t arc_length_method;        /* config.rs:395 (default 0): path following by the arc-length method
+                                       (src/fem/arc_length_control.rs); the loads are scaled by the loading factor */
+  double arc_length_psi;            /* config.rs:406 (default 1; 0 <= psi <= 1): weight of the load term in the constraint */
+  double first_trial_loading_factor;/* config.rs:411 (default 1; |l0| >= 1e-12): loading factor of the first step */
 } pm_config;
 
 /* Statistics of the last pm_solve / element pass. */
@@ -151,6 +155,9 @@ int pm_get_csr(const pm_fem* fem, int64_t* rowptr, int32_t* cols, double* vals);
 /* History of output steps kept with keep_history: count, then u (n x ndof) and IP states. */
 int pm_history_count(const pm_fem* fem, int32_t* n);
 int pm_get_history(const pm_fem* fem, int32_t index, double* u, double* lambda, double* ip_state, double* t);
+/* Loading factor of an output step (FemState::ell, src/fem/fem_state.rs:17): 0 for the initial state, 1 after
+ * a step without the arc-length method, the path's loading factor with it. */
+int pm_get_history_loading_factor(const pm_fem* fem, int32_t index, double* ell);
 /* Times `reps` element passes with HIP events on the handle's stream: per kernel class with events
  * between the launches, then `reps` passes as the solver runs them (one captured graph per pass when
  * use_graphs is set), in ms per pass. */

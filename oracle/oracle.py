@@ -44,7 +44,8 @@ def lib():
         L.orc_add_facet_qn.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int]
         L.orc_setup.argtypes = [C.c_void_p]
         L.orc_sizes.argtypes = [C.c_void_p, _ip]
-        L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, C.c_int]
+        L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, C.c_int, _dp]
+        L.orc_set_arc_length.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_init_state.argtypes = [C.c_void_p]
         L.orc_set_ip_state.argtypes = [C.c_void_p, _dp]
         L.orc_get_ip_state.argtypes = [C.c_void_p, _dp]
@@ -82,6 +83,8 @@ class Oracle:
         L.orc_set_ideal(self.h, int(d["axisym"]), int(d["plane_stress"]), float(d["thickness"]))
         L.orc_set_config(self.h, int(d["lmm"]), int(d["linear_problem"]), d["t_ini"], d["t_fin"], d["dt"],
                          int(d["n_max_iterations"]), d["tol_rr_abs"], d["tol_mdu_rel"])
+        if d.get("arc_length_method"):
+            L.orc_set_arc_length(self.h, 1, float(d["arc_length_psi"]), float(d["first_trial_loading_factor"]))
         tab = np.ascontiguousarray(d["ess_table"], dtype=np.float64)
         n = tab.shape[0]
         L.orc_set_essential(self.h, n, np.ascontiguousarray(d["ess_points"], dtype=np.int32),
@@ -108,7 +111,9 @@ class Oracle:
             raise OracleError(lib().orc_last_error(self.h).decode())
 
     def solve(self, max_out=None):
-        """Runs the Newton driver; returns (u_hist [nout×neq_total], ip_hist [nout×nip×(ns+4)], iters)."""
+        """Runs the Newton driver; returns (u_hist [nout×neq_total], ip_hist [nout×nip×(ns+4)], iters).
+        The loading factor of each output is kept in ``last_ell``; with the arc-length method a step that
+        did not converge reports minus its iteration count."""
         L = lib()
         nst = len(np.arange(self.d["t_ini"], self.d["t_fin"] + 1e-9 * self.d["dt"], self.d["dt"])) + 1
         max_out = max_out or nst + 1
@@ -116,9 +121,11 @@ class Oracle:
         ip = np.zeros(max_out * self.nip * (self.ns + 4))
         nout = C.c_int(0)
         iters = np.zeros(max(nst, 1) + 1, dtype=np.int32)
-        rc = L.orc_solve(self.h, u, ip, max_out, C.byref(nout), iters, len(iters))
+        ell = np.zeros(max_out)
+        rc = L.orc_solve(self.h, u, ip, max_out, C.byref(nout), iters, len(iters), ell)
         n = min(nout.value, max_out)
         self.last_nout = nout.value
+        self.last_ell = ell[:n]
         self.last_u = u.reshape(max_out, self.neq_total)[:n]
         self.last_ip = ip.reshape(max_out, self.nip, self.ns + 4)[:n]
         self.last_iters = iters

@@ -398,6 +398,9 @@ struct Problem {
   Ideal ideal;
   // config (config.rs:184-231 defaults)
   bool lmm = false, linear_problem = false;
+  bool arc = false;                   // arc-length method (config.rs:41-54; defaults ψ = 1, ℓ₀ = 1)
+  double arc_psi = 1.0, ell0 = 1.0;
+  double ell = 0.0;                   // loading factor ℓ (FemState::ell, 0 before the first step)
   double t_ini = 0, t_fin = 1, dt = 1, dt_out = 1, tol_rr_abs = 1e-10, tol_mdu_rel = 1e-8;
   int n_max_iterations = 10, n_max_time_steps = 1000, allowed_step_n_failure = 100;
   // prescribed: value(t_k) tables, k = 0..nstation-1 with t_k = t_ini + k*dt
@@ -734,10 +737,8 @@ void band_lu_apply(Band& B, std::vector<double>& x) {
 
 // direct solve with two steps of iterative refinement against the unfactored matrix A (as UMFPACK's
 // default, UMFPACK_IRSTEP = 2, used by pmsim's default Genie; src/fem/linear_system.rs:185-201)
-bool band_lu_solve(Band& B, std::vector<double>& x) {
-  const Band A = B;
+void band_lu_refined(const Band& A, Band& B, std::vector<double>& x) {  // B: factors of A
   const std::vector<double> b = x;
-  if (!band_lu_factor(B)) return false;
   band_lu_apply(B, x);
   std::vector<double> r(B.n);
   for (int step = 0; step < 2; step++) {
@@ -750,6 +751,12 @@ bool band_lu_solve(Band& B, std::vector<double>& x) {
     band_lu_apply(B, r);
     for (int i = 0; i < A.n; i++) x[i] += r[i];
   }
+}
+
+bool band_lu_solve(Band& B, std::vector<double>& x) {
+  const Band A = B;
+  if (!band_lu_factor(B)) return false;
+  band_lu_refined(A, B, x);
   return true;
 }
 
@@ -783,10 +790,11 @@ std::vector<int> rcm_points(const Problem& P) {
 // ---------------------------------------------------------------------------------------------
 // Newton driver (solver_implicit.rs:56-272)
 // ---------------------------------------------------------------------------------------------
-struct Out { double* u; double* ipv; int max_out; int n_out; };
+struct Out { double* u; double* ipv; int max_out; int n_out; double* ell = nullptr; };
 
 void write_output(Problem& P, Out& o) {
   if (o.n_out >= o.max_out) { o.n_out++; return; }
+  if (o.ell) o.ell[o.n_out] = P.ell;
   if (o.u) std::memcpy(o.u + (size_t)o.n_out * P.neq_total, P.u.data(), sizeof(double) * P.neq_total);
   if (o.ipv) {
     int nv = P.ns + 4;
@@ -842,6 +850,10 @@ const char* solve(Problem& P, Out& o) {
   int nt = P.neq_total, ne_max = 60;
   (void)ne_max;
   P.iters.clear(); P.norm_rr_hist.clear();
+  // config.validate (config.rs:255-265), reported as solver_data.rs:45-47 does
+  if (P.arc_psi < 0.0 || P.arc_psi > 1.0 || std::fabs(P.ell0) < 1e-12)
+    return "cannot allocate simulation because config.validate() failed";
+  P.ell = 0.0;
   // time_control.initialize
   P.t = P.t_ini;
   if (P.dt < 1e-10) return "Δt is smaller than the allowed minimum";
@@ -860,12 +872,35 @@ const char* solve(Problem& P, Out& o) {
   int nd = P.ndim;
   int step_n_failure = 0;
   bool converged = false;
+  // arc-length control (arc_length_control.rs:7-48): Δs, its bounds, ℓ and u of the two previous
+  // converged steps, the constraint g and its derivatives, the iterative δℓ and the total Δℓ
+  double dds = 0, dds_old = 0, dds_min = 0, dds_max = 0, ell_anc = 0, ell_old = 0, arc_g = 0, dg_dl = 0, ddl = 0, arc_dl = 0;
+  std::vector<double> u_anc(nt, 0.0), u_old(nt, 0.0), ddu1(nt), ddu2(nt);
+  bool converged_old = false;
+  auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+    double r = 0; for (int i = 0; i < nt; i++) r += a[i] * b[i]; return r;
+  };
   for (int timestep = 0; timestep < P.n_max_time_steps; timestep++) {
     // time_control.update
     if (P.t + P.dt > P.t_fin) { write_output(P, o); break; }
     P.t += P.dt;
-    std::fill(P.ddu.begin(), P.ddu.end(), 0.0);
-    double ell = 1.0;
+    if (P.arc) {
+      // trial_increments (arc_length_control.rs:56-83): extrapolate u and ℓ from the two previous
+      // converged steps scaled by Δs/Δs_old; Δu = u - u_old, Δℓ = ℓ - ℓ_old
+      if (timestep == 0) P.ell = P.ell0;
+      if (timestep > 0) {
+        if (std::fabs(dds_old) < 1e-12) return "Δs_old is too small";
+        const double alpha = dds / dds_old;
+        for (int i = 0; i < nt; i++) P.u[i] = (1.0 + alpha) * u_old[i] - alpha * u_anc[i];
+        P.ell = (1.0 + alpha) * ell_old - alpha * ell_anc;
+      }
+      for (int i = 0; i < nt; i++) P.ddu[i] = P.u[i] - u_old[i];
+      ddl = P.ell - ell_old;
+      converged_old = converged;
+    } else {
+      std::fill(P.ddu.begin(), P.ddu.end(), 0.0);
+      P.ell = 1.0;
+    }
     converged = false;
     if (!P.linear_problem) for (auto& s : P.st) s.lam = 0.0;  // reset_algorithmic_variables
     int its = 0;
@@ -890,13 +925,25 @@ const char* solve(Problem& P, Out& o) {
         for (size_t k = 0; k < P.pl_point.size(); k++)
           ff_ext[nd * P.pl_point[k] + P.pl_dof[k]] += load_value(P, &P.pl_table[k * P.pl_nstation], P.pl_nstation, P.t);
       }
-      for (int i = 0; i < nt; i++) rr[i] = ff_int[i] - ell * ff_ext[i];
+      for (int i = 0; i < nt; i++) rr[i] = ff_int[i] - P.ell * ff_ext[i];
       if (P.lmm) for (size_t p = 0; p < P.prescribed_eq.size(); p++) {
         int i = P.prescribed_eq[p], j = P.ndof + (int)p;
         rr[i] += P.u[j];
         rr[j] = P.u[i] - table_value(P, (int)p, P.t);
       }
-      double norm_rr = 0; bool bad = false;
+      // arc-length constraint g = Δu·Δu + ψ Δℓ² F·F - Δs² and its derivatives (arc_length_control.rs:86-105)
+      arc_g = 0.0;
+      if (P.arc) {
+        if (timestep > 0) {
+          const double ftf = dot(ff_ext, ff_ext);
+          arc_g = dot(P.ddu, P.ddu) + P.arc_psi * ddl * ddl * ftf - dds * dds;
+          dg_dl = 2.0 * P.arc_psi * ddl * ftf;
+        } else {
+          dg_dl = 1.0;
+        }
+      }
+      // analyze_rr: max(|R|inf, |g|) (convergence_control.rs:52-67)
+      double norm_rr = std::fabs(arc_g); bool bad = !std::isfinite(arc_g);
       for (double v : rr) { if (!std::isfinite(v)) bad = true; norm_rr = std::max(norm_rr, std::fabs(v)); }
       if (bad) return "Found NaN or Inf";
       P.norm_rr_hist.push_back(norm_rr);
@@ -928,9 +975,26 @@ const char* solve(Problem& P, Out& o) {
         for (int eq : P.prescribed_eq) B.at(S.iperm[eq], S.iperm[eq]) += 1.0;
       }
       std::vector<double> x(nt);
-      for (int i = 0; i < nt; i++) x[S.iperm[i]] = rr[i];
-      if (!band_lu_solve(B, x)) return "linear solver failed (singular matrix)";
-      for (int i = 0; i < nt; i++) mdu[i] = x[S.iperm[i]];
+      if (P.arc) {
+        // arc-length solve (arc_length_control.rs:108-126): Δu1 = K⁻¹ F_ext, Δu2 = K⁻¹ R,
+        // δℓ = (dg/du·Δu2 - g) / (dg/dℓ + dg/du·Δu1), -δu = Δu2 - δℓ Δu1, with dg/du = 2Δu (0 at step 0)
+        const Band A = B;
+        if (!band_lu_factor(B)) return "linear solver failed (singular matrix)";
+        for (int i = 0; i < nt; i++) x[S.iperm[i]] = ff_ext[i];
+        band_lu_refined(A, B, x);
+        for (int i = 0; i < nt; i++) ddu1[i] = x[S.iperm[i]];
+        for (int i = 0; i < nt; i++) x[S.iperm[i]] = rr[i];
+        band_lu_refined(A, B, x);
+        for (int i = 0; i < nt; i++) ddu2[i] = x[S.iperm[i]];
+        double c1 = 0, c2 = 0;
+        if (timestep > 0) for (int i = 0; i < nt; i++) { c1 += 2.0 * P.ddu[i] * ddu1[i]; c2 += 2.0 * P.ddu[i] * ddu2[i]; }
+        arc_dl = (c2 - arc_g) / (dg_dl + c1);
+        for (int i = 0; i < nt; i++) mdu[i] = ddu2[i] - arc_dl * ddu1[i];
+      } else {
+        for (int i = 0; i < nt; i++) x[S.iperm[i]] = rr[i];
+        if (!band_lu_solve(B, x)) return "linear solver failed (singular matrix)";
+        for (int i = 0; i < nt; i++) mdu[i] = x[S.iperm[i]];
+      }
       // analyze_mdu
       double norm_mdu = 0; bad = false;
       for (double v : mdu) { if (!std::isfinite(v)) bad = true; norm_mdu = std::max(norm_mdu, std::fabs(v)); }
@@ -948,6 +1012,7 @@ const char* solve(Problem& P, Out& o) {
         if (i < P.ndof && P.ignore[i]) continue;
         P.u[i] -= mdu[i]; P.ddu[i] -= mdu[i];
       }
+      if (P.arc) { P.ell += arc_dl; ddl += arc_dl; }  // update_load_factor: ℓ += δℓ, Δℓ += δℓ
       if (!P.linear_problem) {
         if (iteration == 0) P.backup = P.st; else P.st = P.backup;
       }
@@ -955,10 +1020,26 @@ const char* solve(Problem& P, Out& o) {
       if (const char* ue = update_secondary(P, P.ddu.data())) return ue;
       for (int i = 0; i < P.nip_total; i++) P.st[i].epbar = P.backup[i].epbar + P.st[i].lam;
       if (P.linear_problem) { converged = true; break; }
-      if (iteration == P.n_max_iterations - 1) return "Newton-Raphson did not converge";
+      if (!P.arc && iteration == P.n_max_iterations - 1) return "Newton-Raphson did not converge";
     }
-    if (!converged) return "Newton-Raphson did not converge";
-    P.iters.push_back(its);
+    if (!converged && !P.arc) return "Newton-Raphson did not converge";
+    P.iters.push_back(converged ? its : -its);
+    if (P.arc) {
+      // step_adaptation (arc_length_control.rs:136-168)
+      if (converged) {
+        if (timestep == 0) {
+          dds = std::sqrt(dot(P.ddu, P.ddu) + P.arc_psi * ddl * ddl * dot(ff_ext, ff_ext));
+          dds_max = dds;
+          dds_min = dds / 1024.0;
+        }
+        dds_old = dds;
+        if (converged_old) dds = std::min(std::max(2.0 * dds, dds_min), dds_max);
+        u_anc = u_old; u_old = P.u;
+        ell_anc = ell_old; ell_old = P.ell;
+      } else {
+        dds = converged_old ? std::max(dds / 2.0, dds_min) : std::max(dds / 4.0, dds_min);
+      }
+    }
     bool last = timestep == P.n_max_time_steps - 1;
     if ((converged && P.t >= t_out) || last) { write_output(P, o); t_out += P.dt_out; }
     step_n_failure++;
@@ -1045,9 +1126,16 @@ int orc_sizes(void* h, int* out) {  // [ndof, neq_total, nip_total, nsig]
 }
 
 // full Newton solve; u_out: (max_out × neq_total), ipv_out: (max_out × nip × (ns+4))
-int orc_solve(void* h, double* u_out, double* ipv_out, int max_out, int* n_out, int* iters_out, int max_steps) {
+int orc_set_arc_length(void* h, int enable, double psi, double ell0) {
   auto* P = (Problem*)h;
-  Out o{u_out, ipv_out, max_out, 0};
+  P->arc = enable != 0; P->arc_psi = psi; P->ell0 = ell0;
+  return 0;
+}
+
+// ell_out (nullable): the loading factor of each output
+int orc_solve(void* h, double* u_out, double* ipv_out, int max_out, int* n_out, int* iters_out, int max_steps, double* ell_out) {
+  auto* P = (Problem*)h;
+  Out o{u_out, ipv_out, max_out, 0, ell_out};
   P->ready = false;
   const char* e = solve(*P, o);
   *n_out = o.n_out;

@@ -42,6 +42,9 @@ void pm_default_config(pm_config* c) {
   c->tangent_kernel = 0;
   c->use_graphs = 1;
   c->pcg_small_max = -1;
+  c->arc_length_method = 0;
+  c->arc_length_psi = 1.0;
+  c->first_trial_loading_factor = 1.0;
 }
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
@@ -306,6 +309,12 @@ int pm_get_history(const pm_fem* f, int32_t index, double* u, double* lambda, do
   if (lambda && f->nprescribed) std::memcpy(lambda, f->hist_lam[index].data(), sizeof(double) * f->nprescribed);
   if (ip_state) std::memcpy(ip_state, f->hist_ip[index].data(), sizeof(double) * f->hist_ip[index].size());
   if (t) *t = f->hist_t[index];
+  return 0;
+}
+
+int pm_get_history_loading_factor(const pm_fem* f, int32_t index, double* ell) {
+  if (index < 0 || index >= (int32_t)f->hist_ell.size() || !ell) return 1;
+  *ell = f->hist_ell[index];
   return 0;
 }
 

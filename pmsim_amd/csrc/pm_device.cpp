@@ -116,6 +116,11 @@ int device_alloc(pm_fem* f) {
   return 0;
 }
 
+int arc_alloc(pm_fem* f) {
+  if (f->d_arc) return 0;
+  return dalloc(f, &f->d_arc, 4 * f->ndof + 4 * std::max<int64_t>(f->nprescribed, 1));
+}
+
 void device_free(pm_fem* f) {
   release_graphs(f);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
@@ -124,6 +129,7 @@ void device_free(pm_fem* f) {
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
   fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
+  fr(f->d_arc); f->d_arc = nullptr;
   if (f->stream) (void)hipStreamDestroy(f->stream);
   f->stream = nullptr;
   f->d_X = nullptr; f->d_browptr = f->d_bcols = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;

@@ -218,10 +218,14 @@ int build_patterns(pm_fem* f) {
   return 0;
 }
 
-// External forces at time station k: concentrated loads (bc_concentrated.rs:46-55) and normal
-// distributed loads (bc_distributed.rs:166-174: f = int N qn n dGamma with n the outward normal), each
-// with its value at that station (pmsim evaluates value = f(t) at the step's time, bc_distributed.rs:161-164).
+// External forces at time station k: concentrated loads on every equation (bc_concentrated.rs:46-54)
+// and normal distributed loads (bc_distributed.rs:166-174: f = int N qn n dGamma with n the outward
+// normal) on the equations that are not ignored, i.e. not prescribed under the reduced system method
+// (solver_data.rs:97-107), each with its value at that station (pmsim evaluates value = f(t) at the
+// step's time, bc_distributed.rs:161-164).  The prescribed rows of the reduced system are overwritten
+// anyway, but F.F of the arc-length constraint sees them.
 void assemble_f_ext(pm_fem* f, int64_t k) {
+  const bool rsm = !f->cfg.lagrange_mult_method && f->is_prescribed.size() == (size_t)f->ndof;
   f->f_ext.assign(f->ndof, 0.0);
   const int64_t kp = std::min<int64_t>(std::max<int64_t>(k, 0), f->pl_nstation - 1);
   for (size_t i = 0; i < f->pl_point.size(); i++)
@@ -265,9 +269,16 @@ void assemble_f_ext(pm_fem* f, int64_t k) {
         for (int m = 0; m < nn; m++) r += N[m] * f->X[(int64_t)F.pts[m] * 2];
         w *= r;
       }
-      for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) f->f_ext[(int64_t)F.pts[m] * nd + i] += N[m] * value * n[i] * w;
+      for (int m = 0; m < nn; m++)
+        for (int i = 0; i < nd; i++) {
+          const int64_t eq = (int64_t)F.pts[m] * nd + i;
+          if (rsm && f->is_prescribed[eq]) continue;  // ignored equations (solver_data.rs:103)
+          f->f_ext[eq] += N[m] * value * n[i] * w;
+        }
     }
   }
+  f->ftf = 0.0;
+  for (double v : f->f_ext) f->ftf += v * v;
 }
 
 // ----------------------------------------------------------------------------------------------

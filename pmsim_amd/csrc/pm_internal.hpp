@@ -119,6 +119,7 @@ struct pm_fem {
   std::vector<int64_t> prescribed_eq;
   std::vector<uint8_t> is_prescribed;  // per DOF
   std::vector<double> f_ext;           // host-assembled external forces at the current station
+  double ftf = 0.0;                    // F_ext . F_ext (serial, DOF order)
   // node-block CSR of K
   int bs = 2;                          // block size = ndim
   std::vector<int32_t> h_browptr;      // npoint+1
@@ -166,6 +167,8 @@ struct pm_fem {
   int64_t* d_peq = nullptr;            // prescribed equations
   double* d_free = nullptr;            // 1 for free DOFs, 0 for prescribed (ndof)
   double* d_pcg = nullptr;             // PCG work vectors (6*ndof)
+  double* d_arc = nullptr;             // arc-length vectors: u_old, u_anc, du1, F_free (ndof each), then
+                                       // lambda_old, lambda_anc, dlambda, y1 (nprescribed each); on first use
   double* d_binv = nullptr;            // block-Jacobi inverses (npoint*bs*bs)
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
@@ -178,10 +181,11 @@ struct pm_fem {
   int64_t device_bytes = 0;
   // state
   double t = 0;
+  double ell = 0;                      // loading factor (FemState::ell; 0 before the first step)
   std::vector<double> h_lam;           // host mirror of multipliers
   // history
   std::vector<std::vector<double>> hist_u, hist_lam, hist_ip;
-  std::vector<double> hist_t;
+  std::vector<double> hist_t, hist_ell;
   // stats
   pm_stats stats{};
   pm::Timers timers;
@@ -223,11 +227,16 @@ size_t tangent_shared_bytes(const Group& g);
 double dev_max_abs(pm_fem* f, const double* x, int64_t n, const double* mask);
 double dev_max_rel(pm_fem* f, const double* x, const double* x0, int64_t n);
 double dev_dot(pm_fem* f, const double* a, const double* b, int64_t n);
-void dev_residual(pm_fem* f, double* g);
+void dev_residual(pm_fem* f, double* g, double ell = 1.0);
 void dev_rhs(pm_fem* f, const double* g, double* t, double* kt, double* b);
-void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y);
+void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y, const double* rhs = nullptr);
+void dev_arc_trial(pm_fem* f, int64_t n, bool extrapolate, double alpha, const double* u_old, const double* u_anc, double* u,
+                   double* du);
+void dev_axpy(pm_fem* f, int64_t n, double a, const double* x, double* y);
+void dev_masked(pm_fem* f, const double* x, double* out);
 void dev_update_primary(pm_fem* f, const double* x, const double* y);
 void dev_fill(pm_fem* f, double* x, double v, int64_t n);
+int arc_alloc(pm_fem* f);
 // linear solvers
 int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters);
 int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x);

@@ -189,10 +189,11 @@ __global__ __launch_bounds__(TPB) void k_pcg_p(int64_t n, const double* __restri
 }
 
 // R = F_int - F_ext ; LMM rows (bc_prescribed.rs:99-108) or RSM (ignored rows = 0)
+// R = F_int - ell F_ext (solver_data.rs:117-120); ell = 1 gives the bits of F_int - F_ext
 __global__ __launch_bounds__(TPB) void k_residual(int64_t n, const double* __restrict__ fint, const double* __restrict__ fext,
-                                                  double* __restrict__ rr) {
+                                                  double ell, double* __restrict__ rr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) rr[i] = fint[i] - fext[i];
+  if (i < n) rr[i] = fint[i] - ell * fext[i];
 }
 
 __global__ __launch_bounds__(TPB) void k_prescribed_rows(int64_t np_, int lmm, const int64_t* __restrict__ peq, const double* __restrict__ lam,
@@ -234,6 +235,23 @@ __global__ __launch_bounds__(TPB) void k_update_u(int64_t n, const double* __res
 __global__ __launch_bounds__(TPB) void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] += a * x[i];
+}
+
+// arc-length predictor (arc_length_control.rs:69-79): u = (1 + alpha) u_old - alpha u_anc (alpha = 0 keeps
+// u), du = u - u_old
+__global__ __launch_bounds__(TPB) void k_arc_trial(int64_t n, int extrapolate, double alpha, const double* __restrict__ u_old,
+                                                   const double* __restrict__ u_anc, double* __restrict__ u, double* __restrict__ du) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (extrapolate) u[i] = (1.0 + alpha) * u_old[i] - alpha * u_anc[i];
+  du[i] = u[i] - u_old[i];
+}
+
+// out = x * free
+__global__ __launch_bounds__(TPB) void k_masked(int64_t n, const double* __restrict__ x, const double* __restrict__ free_,
+                                                double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = x[i] * free_[i];
 }
 
 // ---- fused PCG iteration (3 kernels, fixed grid; every block reduces the previous kernel's partials
@@ -599,8 +617,8 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
 }
 
 // ---- Newton vector steps ------------------------------------------------------------------------
-void dev_residual(pm_fem* f, double* g) {
-  hipLaunchKernelGGL(k_residual, dim3(nblk(f->ndof)), dim3(TPB), 0, f->stream, f->ndof, f->d_fint, f->d_fext, f->d_rr);
+void dev_residual(pm_fem* f, double* g, double ell) {
+  hipLaunchKernelGGL(k_residual, dim3(nblk(f->ndof)), dim3(TPB), 0, f->stream, f->ndof, f->d_fint, f->d_fext, ell, f->d_rr);
   if (f->nprescribed)
     hipLaunchKernelGGL(k_prescribed_rows, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed,
                        f->cfg.lagrange_mult_method, f->d_peq, f->d_lam, f->d_u, f->d_cval, f->d_rr, g);
@@ -614,12 +632,28 @@ void dev_rhs(pm_fem* f, const double* g, double* t, double* kt, double* b) {
   hipLaunchKernelGGL(k_rhs, dim3(nblk(f->ndof)), dim3(TPB), 0, f->stream, f->ndof, f->d_rr, kt, f->d_free, b);
 }
 
-// x[eq_p] = g_p ; y_p = rr[eq_p] - (K x)[eq_p]
-void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y) {
+// x[eq_p] = g_p (g = nullptr: x_p stays as solved, zero) ; y_p = rhs[eq_p] - (K x)[eq_p], rhs = R by default
+void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y, const double* rhs) {
   if (!f->nprescribed) return;
-  hipLaunchKernelGGL(k_scatter_g, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed, f->d_peq, g, x);
+  if (g) hipLaunchKernelGGL(k_scatter_g, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed, f->d_peq, g, x);
   dev_spmv(f, x, kx, false);
-  hipLaunchKernelGGL(k_mult_corr, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed, f->d_peq, f->d_rr, kx, y);
+  hipLaunchKernelGGL(k_mult_corr, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed, f->d_peq,
+                     rhs ? rhs : f->d_rr, kx, y);
+}
+
+void dev_arc_trial(pm_fem* f, int64_t n, bool extrapolate, double alpha, const double* u_old, const double* u_anc, double* u,
+                   double* du) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_arc_trial, dim3(nblk(n)), dim3(TPB), 0, f->stream, n, extrapolate ? 1 : 0, alpha, u_old, u_anc, u, du);
+}
+
+void dev_axpy(pm_fem* f, int64_t n, double a, const double* x, double* y) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, f->stream, n, a, x, y);
+}
+
+void dev_masked(pm_fem* f, const double* x, double* out) {
+  hipLaunchKernelGGL(k_masked, dim3(nblk(f->ndof)), dim3(TPB), 0, f->stream, f->ndof, x, f->d_free, out);
 }
 
 void dev_update_primary(pm_fem* f, const double* x, const double* y) {

@@ -7,6 +7,11 @@
 //
 // The convergence norms are taken over the same augmented vectors as pmsim's, so iteration
 // counts follow the reference's (convergence_control.rs:52-127).
+//
+// With the arc-length method (arc_length_control.rs, solver_implicit.rs:117-150,200-251) every
+// iteration also solves K du1 = F_ext (augmented: the multiplier rows' right-hand side is zero), and
+// the correction becomes -du = du2 - dl du1 with dl = (dg/du . du2 - g) / (dg/dl + dg/du . du1) for the
+// constraint g = du.du + psi dl^2 F.F - ds^2 over the augmented vectors [u; lambda].
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -130,12 +135,13 @@ static int save_output(pm_fem* f) {
   f->hist_lam.push_back(std::move(lam));
   f->hist_ip.push_back(std::move(ip));
   f->hist_t.push_back(f->t);
+  f->hist_ell.push_back(f->ell);
   return 0;
 }
 
-// external forces at the step's time when a natural load varies with time
-static int set_external_forces(pm_fem* f) {
-  if (!f->loads_vary) return 0;
+// external forces at the step's time when a natural load varies with time (always: at any rate)
+static int set_external_forces(pm_fem* f, bool always = false) {
+  if (!f->loads_vary && !always) return 0;
   assemble_f_ext(f, (int64_t)std::llround((f->t - f->cfg.t_ini) / f->cfg.dt));
   if (hipMemcpyAsync(f->d_fext, f->f_ext.data(), sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream) != hipSuccess ||
       hipStreamSynchronize(f->stream) != hipSuccess) {
@@ -162,14 +168,24 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
   *n_converged = 0;
   f->stats.n_steps = 0; f->stats.n_iterations = 0; f->stats.n_pcg_iterations = 0; f->stats.n_host_fallbacks = 0;
   f->stats.ms_linear = 0.0;
-  f->hist_u.clear(); f->hist_lam.clear(); f->hist_ip.clear(); f->hist_t.clear();
+  f->hist_u.clear(); f->hist_lam.clear(); f->hist_ip.clear(); f->hist_t.clear(); f->hist_ell.clear();
+  // config.validate (config.rs:255-265), reported as solver_data.rs:45-47 does
+  if (cfg.arc_length_psi < 0.0 || cfg.arc_length_psi > 1.0 || std::fabs(cfg.first_trial_loading_factor) < 1e-12) {
+    f->err = "cannot allocate simulation because config.validate() failed";
+    return 1;
+  }
   if (!cfg.lagrange_mult_method)
     for (double v : f->ess_table) if (v != 0.0) { f->err = "the Lagrange multiplier method is required for non-zero prescribed values"; return 1; }
   const int64_t n = f->ndof, npr = f->nprescribed;
   const bool lmm = cfg.lagrange_mult_method != 0;
+  const bool arc = cfg.arc_length_method != 0;
+  const double psi = cfg.arc_length_psi;
   // time_control.initialize
   f->t = cfg.t_ini;
+  f->ell = 0.0;
   if (cfg.dt < 1e-10) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
+  if (set_external_forces(f, true)) return 1;  // F_ext (and F.F) under the current configuration
+  if (arc && arc_alloc(f)) return 1;
   // initialize_internal_values and fresh primary variables
   for (auto& g : f->groups) launch_init_state(f, g);
   dev_fill(f, f->d_u, 0.0, n);
@@ -184,21 +200,68 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
   double* kt = tv + n;
   double* b = kt + n;
   double* kx = b + n;                // reuses b region only after b is consumed
+  // arc-length control: du1 = K^-1 F_ext, the previous two converged [u; lambda], the step's dlambda
+  const int64_t P = std::max<int64_t>(npr, 1);
+  double* u_old = f->d_arc;
+  double* u_anc = arc ? u_old + n : nullptr;
+  double* du1 = arc ? u_old + 2 * n : nullptr;
+  double* fb = arc ? u_old + 3 * n : nullptr;
+  double* lam_old = arc ? u_old + 4 * n : nullptr;
+  double* lam_anc = arc ? lam_old + P : nullptr;
+  double* dlam = arc ? lam_old + 2 * P : nullptr;
+  double* y1 = arc ? lam_old + 3 * P : nullptr;
+  double dds = 0, dds_old = 0, dds_min = 0, dds_max = 0, ell_anc = 0, ell_old = 0, ddl = 0, dg_dl = 0;
+  bool converged_old = false;
+  if (arc) { dev_fill(f, u_old, 0.0, 2 * n); dev_fill(f, lam_old, 0.0, 4 * P); }
+  auto aug_dot = [&](const double* a, const double* b_, const double* al, const double* bl) {
+    double v = dev_dot(f, a, b_, n);
+    if (lmm && npr) v += dev_dot(f, al, bl, npr);
+    return v;
+  };
   int step_n_failure = 0;
+  bool converged = false;
   for (int timestep = 0; timestep < cfg.n_max_time_steps; timestep++) {
     if (f->t + cfg.dt > cfg.t_fin) { if (save_output(f)) return 1; break; }
     f->t += cfg.dt;
     set_prescribed_values(f);
     if (set_external_forces(f)) return 1;
-    dev_fill(f, f->d_ddu, 0.0, n);
+    if (arc) {
+      // trial_increments (arc_length_control.rs:56-83)
+      double alpha = 0.0;
+      if (timestep == 0) f->ell = cfg.first_trial_loading_factor;
+      if (timestep > 0) {
+        if (std::fabs(dds_old) < 1e-12) { f->err = "Δs_old is too small"; return 1; }
+        alpha = dds / dds_old;
+        f->ell = (1.0 + alpha) * ell_old - alpha * ell_anc;
+      }
+      dev_arc_trial(f, n, timestep > 0, alpha, u_old, u_anc, f->d_u, f->d_ddu);
+      if (lmm) dev_arc_trial(f, npr, timestep > 0, alpha, lam_old, lam_anc, f->d_lam, dlam);
+      ddl = f->ell - ell_old;
+      converged_old = converged;
+    } else {
+      dev_fill(f, f->d_ddu, 0.0, n);
+      f->ell = 1.0;
+    }
     if (!cfg.linear_problem) for (auto& gr : f->groups) launch_reset_lambda(f, gr);
-    bool converged = false;
+    converged = false;
     for (int it = 0; it < cfg.n_max_iterations; it++) {
       f->stats.n_iterations++;
       launch_residual_gather(f);
-      dev_residual(f, g);
+      dev_residual(f, g, f->ell);
       double norm_rr = dev_max_abs(f, f->d_rr, n, nullptr);
       if (lmm && npr) norm_rr = std::max(norm_rr, dev_max_abs(f, g, npr, nullptr));
+      // arc-length constraint and derivatives (arc_length_control.rs:86-105); |g| joins the norm
+      double arc_g = 0.0;
+      if (arc) {
+        if (timestep > 0) {
+          arc_g = aug_dot(f->d_ddu, f->d_ddu, dlam, dlam) + psi * ddl * ddl * f->ftf - dds * dds;
+          dg_dl = 2.0 * psi * ddl * f->ftf;
+        } else {
+          dg_dl = 1.0;
+        }
+        if (!std::isfinite(arc_g)) { f->err = "Found NaN or Inf"; return 1; }
+        norm_rr = std::max(norm_rr, std::fabs(arc_g));
+      }
       if (!std::isfinite(norm_rr)) { f->err = "Found NaN or Inf"; return 1; }
       bool conv_rr = norm_rr < cfg.tol_rr_abs;
       if (!conv_rr && cfg.tol_rr_scaled > 0.0) conv_rr = norm_rr <= cfg.tol_rr_scaled * dev_max_abs(f, f->d_fint, n, nullptr);
@@ -213,10 +276,29 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
         if (check_device_error(f)) return 1;
         if (cfg.lin_sol == 0) build_block_jacobi(f);
       }
-      dev_rhs(f, g, tv, kt, b);
       int pits = 0;
-      if (linear_solve_free(f, b, f->d_mdu, &pits)) return 1;
+      if (arc) {
+        // linear system I: K du1 = F_ext (solved into the correction buffer, whose PCG graph is kept)
+        dev_masked(f, f->d_fext, fb);
+        if (linear_solve_free(f, fb, f->d_mdu, &pits)) return 1;
+        launch_copy(f, du1, f->d_mdu, n);
+        if (lmm) dev_complete_correction(f, nullptr, du1, kx, y1, f->d_fext);
+      }
+      dev_rhs(f, g, tv, kt, b);
+      int pits2 = 0;
+      if (linear_solve_free(f, b, f->d_mdu, &pits2)) return 1;
+      pits += pits2;
       if (lmm) dev_complete_correction(f, g, f->d_mdu, kx, f->d_y);
+      double arc_dl = 0.0;
+      if (arc) {
+        // dl = (dg/du . du2 - g) / (dg/dl + dg/du . du1), dg/du = 2 du (zero at the first step);
+        // -du = du2 - dl du1 (arc_length_control.rs:108-126)
+        const double c1 = timestep > 0 ? 2.0 * aug_dot(f->d_ddu, du1, dlam, y1) : 0.0;
+        const double c2 = timestep > 0 ? 2.0 * aug_dot(f->d_ddu, f->d_mdu, dlam, f->d_y) : 0.0;
+        arc_dl = (c2 - arc_g) / (dg_dl + c1);
+        dev_axpy(f, n, -arc_dl, du1, f->d_mdu);
+        if (lmm) dev_axpy(f, npr, -arc_dl, y1, f->d_y);
+      }
       // analyze_mdu over the augmented correction [x; y]
       bool conv_mdu = false;
       if (it == 0) {
@@ -237,17 +319,40 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
       }
       if (conv_mdu) { converged = true; break; }
       dev_update_primary(f, f->d_mdu, f->d_y);
+      if (arc) {
+        if (lmm) dev_axpy(f, npr, -1.0, f->d_y, dlam);  // the multipliers' part of du
+        f->ell += arc_dl;                              // update_load_factor (arc_length_control.rs:129-133)
+        ddl += arc_dl;
+      }
       // backup at the first iteration; later iterations update from the backup (= restore)
       if (it == 0 || cfg.linear_problem) for (auto& gr : f->groups) launch_copy(f, gr.d_backup, gr.d_state, gr.nel * gr.nip * (f->nsig + 4));
       if (element_pass(f, true, false, false)) return 1;
       if (check_device_error(f)) return 1;
       if (cfg.linear_problem) { converged = true; break; }
-      if (it == cfg.n_max_iterations - 1) { f->err = "Newton-Raphson did not converge"; return 1; }
+      if (!arc && it == cfg.n_max_iterations - 1) { f->err = "Newton-Raphson did not converge"; return 1; }
     }
-    if (!converged) { f->err = "Newton-Raphson did not converge"; return 1; }
-    (*n_converged)++;
-    f->stats.n_steps++;
-    if (cfg.verbose & 1) {
+    if (!converged && !arc) { f->err = "Newton-Raphson did not converge"; return 1; }
+    if (converged) { (*n_converged)++; f->stats.n_steps++; }
+    if (arc) {
+      // step_adaptation (arc_length_control.rs:136-168)
+      if (converged) {
+        if (timestep == 0) {
+          dds = std::sqrt(aug_dot(f->d_ddu, f->d_ddu, dlam, dlam) + psi * ddl * ddl * f->ftf);
+          dds_max = dds;
+          dds_min = dds / 1024.0;
+        }
+        dds_old = dds;
+        if (converged_old) dds = std::min(std::max(2.0 * dds, dds_min), dds_max);
+        launch_copy(f, u_anc, u_old, n);
+        launch_copy(f, u_old, f->d_u, n);
+        if (lmm && npr) { launch_copy(f, lam_anc, lam_old, npr); launch_copy(f, lam_old, f->d_lam, npr); }
+        ell_anc = ell_old;
+        ell_old = f->ell;
+      } else {
+        dds = converged_old ? std::max(dds / 2.0, dds_min) : std::max(dds / 4.0, dds_min);
+      }
+    }
+    if ((cfg.verbose & 1) && converged) {
       std::fprintf(stderr, "step %d t=%g converged: %d Newton iterations in all, %lld PCG iterations in all\n", timestep, f->t,
                    f->stats.n_iterations, (long long)f->stats.n_pcg_iterations);
       std::fflush(stderr);

@@ -225,6 +225,9 @@ class Config:
         self.n_max_iterations = 10
         self.tol_rr_abs = 1e-10
         self.tol_mdu_rel = 1e-8
+        self.arc_length_method = False
+        self.arc_length_psi = 1.0
+        self.first_trial_loading_factor = 1.0
         # MI355X path selection (not in pmsim): linear solver and tangent precision
         self.lin_sol = "pcg"          # "pcg" (device FP64 block-Jacobi PCG) or "host" (host direct)
         self.tangent_precision = "f64"  # "f64", "f32" or "bf16" (inexact-Newton tangent)
@@ -261,6 +264,18 @@ class Config:
     def set_n_max_iterations(self, n): self.n_max_iterations = int(n); return self
     def set_tol_rr_abs(self, v): self.tol_rr_abs = float(v); return self
     def set_tol_mdu_rel(self, v): self.tol_mdu_rel = float(v); return self
+    def set_arc_length_method(self, v=True): self.arc_length_method = bool(v); return self
+    def set_arc_length_psi(self, psi): self.arc_length_psi = float(psi); return self
+    def set_first_trial_loading_factor(self, ell0): self.first_trial_loading_factor = float(ell0); return self
+
+    def validate(self) -> Optional[str]:
+        """The arc-length checks of ``config.rs:255-265`` (None when valid)."""
+        if self.arc_length_psi < 0.0 or self.arc_length_psi > 1.0:
+            return f"arc_length_psi = {self.arc_length_psi!r} is incorrect; it must be 0.0 ≤ ψ ≤ 1.0"
+        if abs(self.first_trial_loading_factor) < 1e-12:
+            return (f"absolute first trial loading factor |ℓ₀| = {self.first_trial_loading_factor!r} is incorrect; "
+                    "it must be ≥ 1e-12")
+        return None
 
     def set_incremental(self, n_station: int):
         """``config.rs:538-546``: t_ini = 0, Δt = 1, t_fin = n_station - 1."""
@@ -333,6 +348,8 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
         lmm=config.lagrange_mult_method, linear_problem=config.linear_problem,
         t_ini=config.t_ini, t_fin=config.t_fin, dt=dt, n_max_iterations=config.n_max_iterations,
         tol_rr_abs=config.tol_rr_abs, tol_mdu_rel=config.tol_mdu_rel,
+        arc_length_method=config.arc_length_method, arc_length_psi=config.arc_length_psi,
+        first_trial_loading_factor=config.first_trial_loading_factor,
         ess_points=pts, ess_dofs=dofs, ess_table=tab,
         pl_points=np.asarray([x[0] for x in pl], dtype=np.int32),
         pl_dofs=np.asarray([x[1] for x in pl], dtype=np.int32),

@@ -28,7 +28,7 @@ EXPORTED = [
     "pm_solve", "pm_get_stats", "pm_sizes", "pm_set_ddu", "pm_get_u", "pm_get_f_int", "pm_get_ip_state", "pm_set_ip_state",
     "pm_get_csr", "pm_history_count", "pm_get_history", "pm_time_element_pass", "pm_linear_solve",
     "pm_estimate_device_bytes", "pm_release_device", "pm_serve", "pm_gauss_rule", "pm_extrapolation_matrix",
-    "pm_shape_values",
+    "pm_shape_values", "pm_get_history_loading_factor",
 ]
 
 
@@ -41,6 +41,7 @@ class PmConfig(C.Structure):
         ("lin_sol", C.c_int32), ("tangent_precision", C.c_int32), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int32),
         ("keep_history", C.c_int32), ("tol_rr_scaled", C.c_double), ("tangent_kernel", C.c_int32),
         ("use_graphs", C.c_int32), ("pcg_small_max", C.c_int32), ("verbose", C.c_int32),
+        ("arc_length_method", C.c_int32), ("arc_length_psi", C.c_double), ("first_trial_loading_factor", C.c_double),
     ]
 
 
@@ -102,6 +103,7 @@ def lib():
     L.pm_get_csr.argtypes = [C.c_void_p, _lp, _ip, _dp]
     L.pm_history_count.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.pm_get_history.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_void_p, _dp, C.POINTER(C.c_double)]
+    L.pm_get_history_loading_factor.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
     L.pm_time_element_pass.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.pm_gauss_rule.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p]
     L.pm_extrapolation_matrix.argtypes = [C.c_int32, C.c_int32, _dp]
@@ -131,6 +133,9 @@ def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmC
     c.t_ini, c.t_fin, c.dt, c.dt_out = d["t_ini"], d["t_fin"], d["dt"], d["dt"]
     c.tol_rr_abs, c.tol_mdu_rel = d["tol_rr_abs"], d["tol_mdu_rel"]
     c.keep_history = 1 if keep_history else 0
+    c.arc_length_method = 1 if d.get("arc_length_method") else 0
+    c.arc_length_psi = float(d.get("arc_length_psi", 1.0))
+    c.first_trial_loading_factor = float(d.get("first_trial_loading_factor", 1.0))
     if cfg is not None:
         c.constant_tangent = int(cfg.constant_tangent)
         c.n_max_time_steps = int(cfg.n_max_time_steps)
@@ -278,6 +283,18 @@ class Fem:
             out.append((t.value, u, lam[: self.nprescribed], ip.reshape(self.nip, self.ns + 4)))
         return out
 
+    def loading_factors(self):
+        """Loading factor of every output step of the last solve (FemState::ell)."""
+        L = lib()
+        n = C.c_int32(0)
+        L.pm_history_count(self.h, C.byref(n))
+        out = np.zeros(n.value)
+        for i in range(n.value):
+            v = C.c_double(0)
+            self._check(L.pm_get_history_loading_factor(self.h, i, C.byref(v)))
+            out[i] = v.value
+        return out
+
     def time_element_pass(self, reps=10):
         ms = np.zeros(6)
         self._check(lib().pm_time_element_pass(self.h, int(reps), ms))
@@ -295,6 +312,8 @@ class FemState:
 
     def __init__(self, mesh: Mesh, base: FemBase, essential: Essential, config: Config):
         self.t = 0.0
+        self.ell = 0.0  # loading factor
+        self.ell_history = np.zeros(0)
         self.u: Optional[np.ndarray] = None
         self.gauss: Optional[np.ndarray] = None  # nip × (nsig + 4): sigma (Mandel), z, lambda, elastic, epbar
         self.history: List[tuple] = []
@@ -325,9 +344,9 @@ class SolverImplicit:
         file_io.write_setup(self.mesh, self.desc)
         nip, model = self.cell_layout()
         prev = None
-        for (t, u, lam, ip) in self.fem.history():
+        for (t, u, lam, ip), ell in zip(self.fem.history(), self.fem.loading_factors()):
             ddu = u - prev if prev is not None else np.zeros_like(u)
-            file_io.write_state(state_dict(self.mesh.ndim, t, float(self.desc["dt"]), u, ddu, ip, nip, model))
+            file_io.write_state(state_dict(self.mesh.ndim, t, float(self.desc["dt"]), u, ddu, ip, nip, model, ell=ell))
             prev = u
         file_io.write_summary()
 
@@ -337,11 +356,13 @@ class SolverImplicit:
             self.write_files(file_io)
         if state is not None:
             state.history = self.fem.history()
+            state.ell_history = self.fem.loading_factors()
             u, lam = self.fem.get_u()
             state.u = u
             state.gauss = self.fem.get_ip_state()
             if state.history:
                 state.t = state.history[-1][0]
+                state.ell = float(state.ell_history[-1])
         return n
 
 

@@ -335,6 +335,56 @@ def hex20_cube(nx, ny, nz, n_inc=10, seed=20261203, jitter=0.05, uz=-0.01, ngaus
     return mesh, base, cfg, ess, Natural()
 
 
+ARC_E, ARC_NU, ARC_H, ARC_Z0 = 1000.0, 0.25, 100.0, 10.0
+
+
+def uniaxial_hex8_arc(psi=0.5, ell0=3.0, lmm=False, nstep=12, q=1.0):
+    """Arc-length method (src/fem/arc_length_control.rs) on one unit Hex8 in uniaxial compression: symmetry
+    planes x = 0, y = 0, z = 0, force ell*q on the top face as four nodal forces, von Mises with linear
+    hardening.  The stress is uniaxial and homogeneous, so the radial return is exact for any increment
+    and every converged (ell, u) lies on the bilinear curve of ``uniaxial_hex8_u``."""
+    X = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+    mesh = Mesh(3, X, [(1, "hex8", list(range(8)))])
+    p1 = ParamSolid(1.0, StressStrain.VonMises(ARC_E, ARC_NU, ARC_H, ARC_Z0), None)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.points([0, 3, 4, 7], Dof.Ux, 0.0).points([0, 1, 4, 5], Dof.Uy, 0.0).points([0, 1, 2, 3], Dof.Uz, 0.0)
+    nat = Natural()
+    nat.points([4, 5, 6, 7], Pbc.Fz, -q / 4.0)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(lmm).set_incremental(nstep + 1).set_n_max_iterations(20).set_tol_mdu_rel(1e-13)
+    cfg.set_arc_length_method(True).set_arc_length_psi(psi).set_first_trial_loading_factor(ell0)
+    return mesh, base, cfg, ess, nat
+
+
+def uniaxial_hex8_u(s):
+    """Closed form of the unit cube under uniaxial compressive stress s (monotonic loading): top uz and
+    ux at x = 1, and the accumulated plastic strain: uz = -s/E - ep, ux = nu s/E + ep/2,
+    ep = max(0, (s - z0)/H) (von Mises flow is deviatoric, so the lateral plastic strain is -ep/2 of
+    the axial one)."""
+    ep = max(0.0, (s - ARC_Z0) / ARC_H)
+    return -s / ARC_E - ep, ARC_NU * s / ARC_E + ep / 2.0, ep
+
+
+def footing_arc(nx=10, ny=5, n_inc=10, psi=1.0, ell0=0.2, lmm=False):
+    """Plane-strain strip footing under a uniform pressure on the top-left 1 m (Qn = -ell*q), traced by the
+    arc-length method: von Mises E=1e4 nu=0.3 z0=200 H=500 (hardening, so the path has no limit point)."""
+    mesh = generators.qua8_block(nx, ny, 10.0, 5.0)
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0)); right = feat.search_edges(At.X(10.0)); bottom = feat.search_edges(At.Y(0.0))
+    foot = feat.search_edges(At.Y(5.0), lambda x: x[0] <= 1.0 + 1e-9)
+    p1 = ParamSolid(1.0, StressStrain.VonMises(1e4, 0.3, 500.0, 200.0), 4)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(right, Dof.Ux, 0.0).edges(bottom, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
+    nat = Natural()
+    nat.edges(foot, Nbc.Qn, -500.0)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(lmm).set_incremental(n_inc + 1).set_n_max_iterations(20)
+    cfg.set_arc_length_method(True).set_arc_length_psi(psi).set_first_trial_loading_factor(ell0)
+    return mesh, base, cfg, ess, nat
+
+
 def flat(case):
     mesh, base, cfg, ess, nat = case
     return describe(mesh, base, cfg, ess, nat)

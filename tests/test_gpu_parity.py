@@ -220,6 +220,62 @@ def test_hex20_cube_vs_oracle(lin_sol):
     _check_every_increment(d, f, hist)
 
 
+def _check_arc(d, f, hist, tol=1e-10):
+    """An arc-length solve against the oracle: the same outputs, the loading factor of each output, the
+    displacements, multipliers and stresses of each at `tol` relative (to the field's largest
+    magnitude), plastic flags equal, and the same total number of Newton iterations."""
+    o = orc.Oracle(d)
+    uo, ipo, _ = o.solve()
+    ell = f.loading_factors()
+    assert len(hist) == len(uo) == len(ell)
+    assert np.abs(ell - o.last_ell).max() <= tol * np.abs(o.last_ell).max()
+    ns = f.ns
+    for k in range(len(hist)):
+        t, u, lam, ip = hist[k]
+        print(f"output {k}: ell {ell[k]:.15g} (oracle {o.last_ell[k]:.15g}) du {rel_err(u, uo[k][: f.ndof]):.2e}", flush=True)
+        assert rel_err(u, uo[k][: f.ndof]) < tol
+        if d["lmm"] and k > 0:
+            assert rel_err(lam, uo[k][f.ndof:]) < tol
+        assert rel_err(ip[:, :ns], ipo[k][:, :ns]) < tol
+        assert np.array_equal(ip[:, ns + 2], ipo[k][:, ns + 2])
+    assert f.stats()["n_iterations"] == int(np.abs(o.last_iters).sum())
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+@pytest.mark.parametrize("psi,lmm", [(0.5, True), (1e-4, False)])
+def test_arc_length_uniaxial_hex8(lin_sol, psi, lmm):
+    """Arc-length method on the uniaxial Hex8: every converged (ell, u) on the exact bilinear curve
+    (1e-11) and the path identical to the oracle's (1e-10)."""
+    d, f, hist = _solve_pair(cases.uniaxial_hex8_arc(psi=psi, lmm=lmm), lin_sol)
+    ell = f.loading_factors()
+    assert len(hist) == 13 and ell[-1] > 1.3 * cases.ARC_Z0
+    for k in range(1, len(hist)):
+        uz, ux, ep = cases.uniaxial_hex8_u(ell[k])
+        u, ip = hist[k][1], hist[k][3]
+        assert abs(u[3 * 6 + 2] - uz) < 1e-11 and abs(u[3 * 6 + 0] - ux) < 1e-11
+        assert np.abs(ip[:, 6 + 3] - ep).max() < 1e-11
+    _check_arc(d, f, hist)
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+@pytest.mark.parametrize("lmm", [False, True])
+def test_arc_length_footing(lin_sol, lmm):
+    """Strip footing under a pressure traced by the arc-length method (10x5 Qua8, 2x2 IP, hardening),
+    reduced system and Lagrange multipliers: the path against the oracle's at 1e-10."""
+    d, f, hist = _solve_pair(cases.footing_arc(lmm=lmm), lin_sol)
+    _check_arc(d, f, hist)
+    assert hist[-1][3][:, f.ns + 2].min() == 0.0  # plasticity developed
+
+
+def test_arc_length_config_validation():
+    """An invalid psi is refused at solve time with pmsim's error text (config.rs:255-259,
+    solver_data.rs:45-47)."""
+    mesh, base, cfg, ess, nat = cases.uniaxial_hex8_arc(psi=1.5)
+    f = Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    with pytest.raises(Exception, match="config.validate"):
+        f.solve()
+
+
 def test_pcg_matches_host_direct():
     mesh, base, cfg, ess, nat = cases.footing_c1()
     d = describe(mesh, base, cfg, ess, nat)

@@ -135,6 +135,56 @@ def test_felippa_thick_cylinder_axisym():
         assert abs(u[-1][2 * p] - cases.lame_tube_ur(r)) < 1e-15
 
 
+@pytest.mark.parametrize("psi,lmm", [(0.5, True), (1e-4, False), (1.0, False)])
+def test_arc_length_uniaxial_closed_form(psi, lmm):
+    """Arc-length method (arc_length_control.rs, solver_implicit.rs:117-150,200-251) on a Hex8 in uniaxial
+    compression: every converged (ell, u) on the exact bilinear curve (u 1e-13, ep 1e-13), the elastic
+    steps advance ell by ell0 (the extrapolated predictor is exact on a linear path), and consecutive
+    converged states satisfy the constraint |du|^2 + psi dl^2 |F|^2 = ds0^2 over the augmented vector
+    (u and, with LMM, the multipliers) to the convergence bound on |g|."""
+    ell0 = 3.0
+    o = orc.Oracle(cases.flat(cases.uniaxial_hex8_arc(psi=psi, lmm=lmm, ell0=ell0)))
+    u, ip, _ = o.solve()
+    ell = o.last_ell
+    assert len(ell) == 13 and ell[0] == 0.0 and np.all(o.last_iters[:12] > 0)
+    assert ell[-1] > 1.3 * cases.ARC_Z0  # well into the plastic range
+    for k in range(1, len(ell)):
+        uz, ux, ep = cases.uniaxial_hex8_u(ell[k])
+        assert abs(u[k][3 * 6 + 2] - uz) < 1e-13 and abs(u[k][3 * 6 + 0] - ux) < 1e-13
+        assert np.abs(ip[k][:, 6 + 3] - ep).max() < 1e-13
+        if ell[k] < cases.ARC_Z0:
+            assert abs(ell[k] - k * ell0) < 1e-9
+    ftf = 4 * (1.0 / 4.0) ** 2
+    ds0 = u[1] @ u[1] + psi * ell0 ** 2 * ftf
+    for k in range(2, len(ell)):
+        du = u[k] - u[k - 1]
+        # converged means max(|R|inf, |g|) < tol_rr_abs (convergence_control.rs:57-67)
+        assert abs(du @ du + psi * (ell[k] - ell[k - 1]) ** 2 * ftf - ds0) < 1e-10
+
+
+def test_arc_length_psi_zero_keeps_pmsim_stall():
+    """psi = 0: at the second step the stress is still the first step's when the constraint is first
+    evaluated, so dl = -ell0 cancels the residual exactly and -du = 0; the next iteration meets the same
+    cancellation and converges on the zero correction before dl is applied (iterate() returns before
+    update_load_factor, solver_implicit.rs:241-251).  The output keeps ell = ell0 with u at 2 ell0."""
+    o = orc.Oracle(cases.flat(cases.uniaxial_hex8_arc(psi=0.0, ell0=3.0)))
+    u, _, _ = o.solve()
+    assert abs(o.last_ell[2] - 3.0) < 1e-12
+    assert abs(u[2][3 * 6 + 2] - 2 * cases.uniaxial_hex8_u(3.0)[0]) < 1e-15
+
+
+def test_arc_length_config_validation():
+    """config.rs:255-265 and the error of solver_data.rs:45-47."""
+    mesh, base, cfg, ess, nat = cases.uniaxial_hex8_arc(psi=1.5)
+    assert cfg.validate() == "arc_length_psi = 1.5 is incorrect; it must be 0.0 ≤ ψ ≤ 1.0"
+    with pytest.raises(orc.OracleError, match="config.validate"):
+        orc.Oracle(cases.flat((mesh, base, cfg, ess, nat))).solve()
+    cfg.set_arc_length_psi(1.0).set_first_trial_loading_factor(0.0)
+    assert cfg.validate().startswith("absolute first trial loading factor")
+    cfg.set_first_trial_loading_factor(1.0)
+    assert cfg.validate() is None
+
+
 def test_sgm_5d17_ip_stresses():
     """9-point Gauss ordering pinned by per-IP stresses of the axisymmetric SGM example."""
     o = orc.Oracle(cases.flat(cases.smith("5d17_qua4_axisym")))
