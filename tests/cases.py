@@ -32,6 +32,45 @@ def one_qua4():
     return Mesh(2, [[0, 0], [1, 0], [1, 1], [0, 1]], [(1, "qua4", [0, 1, 2, 3])])
 
 
+PD_YOUNG, PD_POISSON, PD_DY = 1500.0, 0.25, 0.1
+
+
+def prescribe_displacements_2d():
+    """tests/test_prescribe_displacements_2d.rs: one Qua4, linear elastic E = 1500, nu = 0.25, ux = 0 on the
+    left edge, uy = 0 at the bottom and uy = -0.1 at the top (LMM); plane strain with free lateral
+    expansion: u = [0, 0, ex, 0, ex, -dy, 0, -dy] with ex = -dy nu / (nu - 1)."""
+    mesh = one_qua4()
+    feat = Features(mesh)
+    left = feat.search_edges(At.X(0.0), any_x)
+    bottom = feat.search_edges(At.Y(0.0), any_x)
+    top = feat.search_edges(At.Y(1.0), any_x)
+    p1 = ParamSolid(1.0, StressStrain.LinearElastic(PD_YOUNG, PD_POISSON), None)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    ess = Essential()
+    ess.edges(left, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0).edges(top, Dof.Uy, -PD_DY)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True)
+    return mesh, base, cfg, ess, Natural()
+
+
+SP_YOUNG, SP_POISSON, SP_Q = 1.0, 0.25, -2.0
+
+
+def solid_prescribed_displacement():
+    """tests/test_solid_prescribed_displacement.rs: one Qua4, E = 1, nu = 0.25; ux(0) = 0, uy(0, 1) = 0 and
+    uy(2, 3) = -q (nu - 1)(nu + 1)/E, the vertical strain of a uniaxial plane-strain stress q = -2; every
+    IP then holds sigma = [0, q, q nu, 0]."""
+    mesh = one_qua4()
+    p1 = ParamSolid(1.0, StressStrain.LinearElastic(SP_YOUNG, SP_POISSON), None)
+    base = FemBase(mesh, [(1, Elem.Solid(p1))])
+    strain_y = -SP_Q * (SP_POISSON - 1.0) * (SP_POISSON + 1.0) / SP_YOUNG
+    ess = Essential()
+    ess.points([0], Dof.Ux, 0.0).points([0, 1], Dof.Uy, 0.0).points([2, 3], Dof.Uy, strain_y)
+    cfg = Config(mesh)
+    cfg.set_lagrange_mult_method(True)
+    return mesh, base, cfg, ess, Natural()
+
+
 def block_2d_four_qua8():
     X = [[0, 0], [1, 0], [1, 1], [0, 1], [0.5, 0], [1, 0.5], [0.5, 1], [0, 0.5], [2, 0], [2, 1], [1.5, 0], [2, 0.5],
          [1.5, 1], [1, 2], [0, 2], [1, 1.5], [0.5, 2], [0, 1.5], [2, 2], [2, 1.5], [1.5, 2]]

@@ -175,6 +175,23 @@ def test_felippa_thick_cylinder_axisym():
         assert abs(u[2 * p] - cases.lame_tube_ur(mesh.coords[p][0])) < 1e-15
 
 
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+def test_prescribe_displacements_2d(lin_sol):
+    """tests/test_prescribe_displacements_2d.rs:55-68: u at 1e-15."""
+    _, f, hist = _solve_pair(cases.prescribe_displacements_2d(), lin_sol)
+    ex = -cases.PD_DY * cases.PD_POISSON / (cases.PD_POISSON - 1.0)
+    ref = [0.0, 0.0, ex, 0.0, ex, -cases.PD_DY, 0.0, -cases.PD_DY]
+    assert np.abs(hist[-1][1][:8] - ref).max() <= 1e-15
+
+
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+def test_solid_prescribed_displacement(lin_sol):
+    """tests/test_solid_prescribed_displacement.rs: sigma = [0, q, q nu, 0] at every IP (1e-15)."""
+    _, f, hist = _solve_pair(cases.solid_prescribed_displacement(), lin_sol)
+    q, nu = cases.SP_Q, cases.SP_POISSON
+    assert np.abs(hist[-1][3][:, :4] - [0.0, q, q * nu, 0.0]).max() <= 1e-15
+
+
 def test_spo_755_tensile():
     _check_spo(cases.spo_755_tensile(), "spo_755_tensile_perf_plast_ref", 1e-7, 1.2e-5, "pcg")
 

@@ -185,6 +185,21 @@ def test_arc_length_config_validation():
     assert cfg.validate() is None
 
 
+def test_prescribe_displacements_2d():
+    """tests/test_prescribe_displacements_2d.rs:55-68: u at 1e-15."""
+    u, _, _ = orc.Oracle(cases.flat(cases.prescribe_displacements_2d())).solve()
+    ex = -cases.PD_DY * cases.PD_POISSON / (cases.PD_POISSON - 1.0)
+    ref = [0.0, 0.0, ex, 0.0, ex, -cases.PD_DY, 0.0, -cases.PD_DY]
+    assert np.abs(u[-1][:8] - ref).max() <= 1e-15
+
+
+def test_solid_prescribed_displacement():
+    """tests/test_solid_prescribed_displacement.rs: sigma = [0, q, q nu, 0] at every IP (1e-15)."""
+    _, ip, _ = orc.Oracle(cases.flat(cases.solid_prescribed_displacement())).solve()
+    q, nu = cases.SP_Q, cases.SP_POISSON
+    assert np.abs(ip[-1][:, :4] - [0.0, q, q * nu, 0.0]).max() <= 1e-15
+
+
 def test_sgm_5d17_ip_stresses():
     """9-point Gauss ordering pinned by per-IP stresses of the axisymmetric SGM example."""
     o = orc.Oracle(cases.flat(cases.smith("5d17_qua4_axisym")))
