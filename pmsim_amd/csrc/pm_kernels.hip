@@ -159,6 +159,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
     double r = 0.0;
     if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * Xe[m * ND];
     // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
+    // shear sums are accumulated as 2 e_ij (one FMA per term) and halved once
     double exx = 0, eyy = 0, ezz = 0, exy = 0, eyz = 0, exz = 0;
 #pragma unroll 4
     for (int m = 0; m < NN; m++) {
@@ -172,17 +173,17 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
       }
       if constexpr (ND == 2) {
         double ux = Ue[m * 2], uy = Ue[m * 2 + 1];
-        exx += ux * G[0]; eyy += uy * G[1]; exy += (ux * G[1] + uy * G[0]) * 0.5;
+        exx += ux * G[0]; eyy += uy * G[1]; exy += ux * G[1]; exy += uy * G[0];
       } else {
         double ux = Ue[m * 3], uy = Ue[m * 3 + 1], uz = Ue[m * 3 + 2];
         exx += ux * G[0]; eyy += uy * G[1]; ezz += uz * G[2];
-        exy += (ux * G[1] + uy * G[0]) * 0.5; eyz += (uy * G[2] + uz * G[1]) * 0.5; exz += (ux * G[2] + uz * G[0]) * 0.5;
+        exy += ux * G[1]; exy += uy * G[0]; eyz += uy * G[2]; eyz += uz * G[1]; exz += ux * G[2]; exz += uz * G[0];
       }
     }
     if (axisym) for (int m = 0; m < NN; m++) ezz += Ue[m * ND] * tN[q * NN + m] / r;
     double de[NS];
-    de[0] = exx; de[1] = eyy; de[2] = ezz; de[3] = exy * SQ2;
-    if constexpr (NS == 6) { de[4] = eyz * SQ2; de[5] = exz * SQ2; }
+    de[0] = exx; de[1] = eyy; de[2] = ezz; de[3] = (exy * 0.5) * SQ2;
+    if constexpr (NS == 6) { de[4] = (eyz * 0.5) * SQ2; de[5] = (exz * 0.5) * SQ2; }
     const int64_t ip = e * nip + q;
     double sig[NS];
 #pragma unroll
@@ -229,15 +230,17 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
       }
       if constexpr (ND == 2) {
         double sxy = S[3] * ISQ2;
-        double fx = S[0] * G[0] + sxy * G[1];
-        if (axisym) fx += S[2] * tN[q * NN + m] / S[NS + 4];
-        fm[0] += fx;
-        fm[1] += sxy * G[0] + S[1] * G[1];
+        fm[0] += S[0] * G[0];
+        fm[0] += sxy * G[1];
+        if (axisym) fm[0] += S[2] * tN[q * NN + m] / S[NS + 4];
+        fm[1] += sxy * G[0];
+        fm[1] += S[1] * G[1];
       } else {
         double sxy = S[3] * ISQ2, syz = S[4] * ISQ2, sxz = S[5] * ISQ2;
-        fm[0] += S[0] * G[0] + sxy * G[1] + sxz * G[2];
-        fm[1] += sxy * G[0] + S[1] * G[1] + syz * G[2];
-        fm[2] += sxz * G[0] + syz * G[1] + S[2] * G[2];
+        // one FMA per term into the accumulators
+        fm[0] += S[0] * G[0]; fm[0] += sxy * G[1]; fm[0] += sxz * G[2];
+        fm[1] += sxy * G[0]; fm[1] += S[1] * G[1]; fm[1] += syz * G[2];
+        fm[2] += sxz * G[0]; fm[2] += syz * G[1]; fm[2] += S[2] * G[2];
       }
     }
 #pragma unroll
@@ -432,8 +435,16 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 #pragma unroll
           for (int i = 0; i < ND; i++) {
             Dg[u][v] += Ga[u][i] * ub[i];
+            // three chained accumulations: one FMA each into the accumulator (a parenthesised sum
+            // would cost a multiply, two FMAs and an add)
 #pragma unroll
-            for (int j = 0; j < ND; j++) K[u][v][i][j] += ub[i] * Ga[u][j] + tIa[u][i] * vb[j] + ta[u][i] * wb[j];
+            for (int j = 0; j < ND; j++) {
+              double k = K[u][v][i][j];
+              k += ub[i] * Ga[u][j];
+              k += tIa[u][i] * vb[j];
+              k += ta[u][i] * wb[j];
+              K[u][v][i][j] = k;
+            }
           }
           if (AX) K[u][v][0][0] += hA[u] * hh;
         }

@@ -284,6 +284,20 @@ def test_arc_length_footing(lin_sol, lmm):
     assert hist[-1][3][:, f.ns + 2].min() == 0.0  # plasticity developed
 
 
+@pytest.mark.parametrize("lin_sol", ["host", "pcg"])
+@pytest.mark.parametrize("nmax", [3, 4])
+def test_arc_length_failed_steps(lin_sol, nmax):
+    """At most `nmax` Newton iterations per step: some steps do not converge, which is not an error with
+    the arc-length method; ds is halved after a converged step or quartered after a failed one and the
+    next step extrapolates from the last two converged states (arc_length_control.rs:136-168).  The
+    outputs, loading factors and Newton counts follow the oracle's."""
+    case = cases.footing_arc(n_inc=16)
+    case[2].set_n_max_iterations(nmax)
+    d, f, hist = _solve_pair(case, lin_sol)
+    _check_arc(d, f, hist)
+    assert 3 <= f.stats()["n_steps"] < 16
+
+
 def test_arc_length_config_validation():
     """An invalid psi is refused at solve time with pmsim's error text (config.rs:255-259,
     solver_data.rs:45-47)."""
