@@ -292,7 +292,10 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
   double* sN = sdN + nip * DS;           // nip * NN (axisymmetric only)
   double* sX = sN + (axisym ? nip * NN : 0);
   double* sQ = sX + epb * NN * ND;
-  // node vectors of one IP chunk; at the end the output staging (16-byte aligned for the copy out)
+  // node vectors of one IP chunk, ES doubles per element (odd, so that the reads of the two elements a
+  // wave straddles fall in disjoint LDS bank pairs); at the end the output staging (16-byte aligned
+  // for the copy out)
+  const int ES = (ch * NNP * PW) | 1;
   double* sP = sQ + ((epb * nip * QW + (int)(sQ - sh)) % 2 == 0 ? epb * nip * QW : epb * nip * QW + 1);
   __shared__ uint8_t srank[2 * TPB];  // epb * NN <= TPB * NN / NT <= 4/3 TPB
   const int tid = threadIdx.x;
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     for (int ql = 0; ql < nq; ql++) {
       const double* Q = sQ + (el * nip + q0 + ql) * QW;
       const double ha = Q[0], be = Q[1], ga = Q[2];
-      const double* Pq = sP + (el * ch + ql) * NNP * PW;
+      const double* Pq = sP + el * ES + ql * NNP * PW;
       double Ga[2][ND], tIa[2][ND], ta[2][ND], hA[2];
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -462,7 +465,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 #pragma unroll 1
     for (int w = tid; w < epb * nq * NNP; w += blockDim.x) {
       const int e = w / (nq * NNP), rem = w - e * nq * NNP, ql = rem / NNP, m = rem - ql * NNP;
-      double* P = sP + ((e * ch + ql) * NNP + m) * PW;
+      double* P = sP + e * ES + (ql * NNP + m) * PW;
       if (e0 + e >= nel || m >= NN) {
 #pragma unroll
         for (int k = 0; k < PW; k++) P[k] = 0.0;
@@ -661,7 +664,7 @@ static int epb_update(int nd, int nn, int nip) {
 static size_t shm_tangent(int nd, int nn, int nip, int epb, int ch, bool axisym) {
   int nh = (nn + 1) / 2, pw = 2 * nd + 1;
   size_t table = (size_t)nip * (1 + nn * nd + 1) + (axisym ? (size_t)nip * nn : 0);
-  size_t pblk = (size_t)epb * ch * 2 * nh * pw;                   // node vectors of one IP chunk
+  size_t pblk = (size_t)epb * ((ch * 2 * nh * pw) | 1);           // node vectors of one IP chunk (odd stride)
   size_t stg = (size_t)packed_ke_values(nn, nd);                  // output staging, one element (FP64 bound)
   size_t uni = std::max(pblk, stg);
   return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + 1 + uni);  // +1: alignment
