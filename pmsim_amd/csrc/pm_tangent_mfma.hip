@@ -99,8 +99,15 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   using S = MShape<ND, NN>;
   using T = typename MfmaT<PREC>::T;
   MLayout L;
-  L.nrow = axisym ? 4 : 2;  // g, ts  |  g, tI, ts, h
-  L.rsk = ipc + 1;          // ipc is a multiple of 4: odd stride
+  if (PREC == 2) {
+    // BF16 operands converted once when the rows are staged: g, ts and the weighted g wM, g wB, ts wG
+    // (axisymmetric: g, tI, ts, h and g wM, tI wB, ts wG, h wH); 4-value groups 8-byte aligned
+    L.nrow = axisym ? 8 : 5;
+    L.rsk = ipc + 4;
+  } else {
+    L.nrow = axisym ? 4 : 2;  // g, ts  |  g, tI, ts, h
+    L.rsk = ipc + 1;          // ipc is a multiple of 4: odd stride
+  }
   L.dsn = NN * ND + 1;
   L.tdn = (size_t)nip;
   L.tn = L.tdn + (size_t)nip * L.dsn;
@@ -109,7 +116,8 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   L.s = (size_t)NN * ND;
   L.q = L.s + (size_t)nip * S::SV;
   L.uni = (L.q + (size_t)nip * S::QW + 1) & ~(size_t)1;
-  const size_t rows = ((size_t)L.nrow * S::NCP * L.rsk + 4 * (size_t)ipc) * sizeof(T);
+  const size_t rows = PREC == 2 ? (size_t)L.nrow * S::NCP * L.rsk * sizeof(short)
+                                 : ((size_t)L.nrow * S::NCP * L.rsk + 4 * (size_t)ipc) * sizeof(T);
   const size_t mt = (size_t)S::NCP * (S::NCP + 1) * sizeof(T);  // dense M, both triangles
   const size_t kst = (size_t)packed_ke_values(NN, ND) * sizeof(T);
   size_t u = rows > mt ? rows : mt;
@@ -299,7 +307,71 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
     for (int ch = 0; ch < nchunk; ch++) {
       __syncthreads();  // per-IP data ready / previous chunk consumed
       const int nv = min(ipc, nip - ch * ipc);  // valid IPs of this chunk
-      if (active) {
+      if (active && PREC == 2) {
+        // BF16 rows of the valid IPs (nodes in rank order, column ND * rank(m) + i), each operand
+        // rounded once here: the unweighted A rows and the B rows times their IP weight (the weight
+        // product in FP32, then rounded, as the MFMA operand)
+        short* sB = reinterpret_cast<short*>(slot + L.uni);
+        const int KB = NCP * RSK;
+        for (int w = te; w < nv * NN; w += TPE) {
+          const int ql = w / NN, m = w - ql * NN, q = ch * ipc + ql;
+          const double* Q = sQ + q * QW;
+          const double* dN = sdN + q * DSN + m * ND;
+          double G[ND];
+#pragma unroll
+          for (int j = 0; j < ND; j++) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < ND; k++) sum += dN[k] * Q[3 + k * ND + j];
+            G[j] = sum;
+          }
+          const double* sv = Q + 3 + ND * ND;
+          const double h = axisym ? sN[q * NN + m] / Q[3 + ND * ND + 6] : 0.0;
+          double ts[ND];
+          if constexpr (ND == 2) {
+            const double sxy = sv[3] * ISQ2;
+            ts[0] = sv[0] * G[0] + sxy * G[1] + h * sv[2];
+            ts[1] = sxy * G[0] + sv[1] * G[1];
+          } else {
+            const double sxy = sv[3] * ISQ2, syz = sv[4] * ISQ2, sxz = sv[5] * ISQ2;
+            ts[0] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
+            ts[1] = sxy * G[0] + sv[1] * G[1] + syz * G[2];
+            ts[2] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
+          }
+          const float wM = (float)(0.5 * Q[0]), wB = (float)Q[1], wG = (float)Q[2], wH = (float)Q[0];
+          short* c = sB + (ND * srank[el * NN + m]) * RSK + ql;
+#pragma unroll
+          for (int i = 0; i < ND; i++) {
+            const float g = (float)G[i], t = (float)ts[i];
+            if (NR == 5) {
+              c[i * RSK] = to_bf16(g);
+              c[KB + i * RSK] = to_bf16(t);
+              c[2 * KB + i * RSK] = to_bf16(g * wM);
+              c[3 * KB + i * RSK] = to_bf16(g * wB);
+              c[4 * KB + i * RSK] = to_bf16(t * wG);
+            } else {
+              const float ti = (float)(G[i] + (i == 0 ? h : 0.0)), hh = i == 0 ? (float)h : 0.0f;
+              c[i * RSK] = to_bf16(g);
+              c[KB + i * RSK] = to_bf16(ti);
+              c[2 * KB + i * RSK] = to_bf16(t);
+              c[3 * KB + i * RSK] = to_bf16(hh);
+              c[4 * KB + i * RSK] = to_bf16(g * wM);
+              c[5 * KB + i * RSK] = to_bf16(ti * wB);
+              c[6 * KB + i * RSK] = to_bf16(t * wG);
+              c[7 * KB + i * RSK] = to_bf16(hh * wH);
+            }
+          }
+        }
+        // zero padding: slots of IPs beyond nip (all columns), and columns [NC, NCP) of the valid slots
+        const int npad_q = (ipc - nv) * NCP, npad = npad_q + (NCP - NC) * nv;
+        for (int w = te; w < npad; w += TPE) {
+          int col, ql;
+          constexpr int PADC = NCP > NC ? NCP - NC : 1;
+          if (w < npad_q) { ql = nv + w / NCP; col = w - (ql - nv) * NCP; }
+          else { const int w2 = w - npad_q; ql = w2 / PADC; col = NC + (w2 - ql * PADC); }
+          for (int k = 0; k < NR; k++) sB[k * KB + col * RSK + ql] = 0;
+        }
+      } else if (active) {
         // rows of the valid IPs, nodes in rank order: column ND * rank(m) + i
         for (int w = te; w < nv * NN; w += TPE) {
           const int ql = w / NN, m = w - ql * NN, q = ch * ipc + ql;
@@ -363,35 +435,29 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         const int tIk = NR == 2 ? 0 : KR;        // the tI rows (g itself unless axisymmetric)
         const int tsk = NR == 2 ? KR : 2 * KR;   // the ts rows
         if constexpr (PREC == 2) {
+          // operands as staged: one 8-byte read per operand and tile (kinds: A rows first, then B rows)
+          const short* sB = reinterpret_cast<const short*>(slot + L.uni);
+          const int KB = NCP * RSK;
+          const int kA_i = NR == 5 ? 0 : 1, kA_t = NR == 5 ? 1 : 2, kB = NR == 5 ? 2 : 4;
 #pragma unroll 1
           for (int k0 = 0; k0 < ipc; k0 += KS) {
             const int kq = k0 + 4 * (l >> 4);
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
-              const T* ca = sL + (16 * tIv[s] + col) * RSK;
-              const T* cb = sL + (16 * tJv[s] + col) * RSK;
-              s4_t ga, gm, ia, gb, ta, tb;
-#pragma unroll
-              for (int j = 0; j < 4; j++) {
-                const int k = kq + j;
-                ga[j] = to_bf16(ca[k]);
-                gm[j] = to_bf16(cb[k] * sW[k]);
-                ia[j] = to_bf16(ca[tIk + k]);
-                gb[j] = to_bf16(cb[tIk + k] * sW[ipc + k]);
-                ta[j] = to_bf16(ca[tsk + k]);
-                tb[j] = to_bf16(cb[tsk + k] * sW[2 * ipc + k]);
-              }
+              const short* ca = sB + (16 * tIv[s] + col) * RSK + kq;
+              const short* cb = sB + (16 * tJv[s] + col) * RSK + kq;
+              const s4_t ga = *reinterpret_cast<const s4_t*>(ca);
+              const s4_t gm = *reinterpret_cast<const s4_t*>(cb + kB * KB);
+              const s4_t ia = *reinterpret_cast<const s4_t*>(ca + kA_i * KB);
+              const s4_t gb = *reinterpret_cast<const s4_t*>(cb + (kB + 1) * KB);
+              const s4_t ta = *reinterpret_cast<const s4_t*>(ca + kA_t * KB);
+              const s4_t tb = *reinterpret_cast<const s4_t*>(cb + (kB + 2) * KB);
               accM[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga, gm, accM[s], 0, 0, 0);
               accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ia, gb, accC[s], 0, 0, 0);
               accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ta, tb, accC[s], 0, 0, 0);
-              if (NR == 4) {
-                s4_t ha, hb;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                  const int k = kq + j;
-                  ha[j] = to_bf16(ca[3 * KR + k]);
-                  hb[j] = to_bf16(cb[3 * KR + k] * sW[3 * ipc + k]);
-                }
+              if (NR == 8) {
+                const s4_t ha = *reinterpret_cast<const s4_t*>(ca + 3 * KB);
+                const s4_t hb = *reinterpret_cast<const s4_t*>(cb + 7 * KB);
                 accC[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ha, hb, accC[s], 0, 0, 0);
               }
             }
