@@ -226,21 +226,26 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
       tIv[s] = I; tJv[s] = I + t;
     }
 
-    // one lane per IP: J, J^-1, c times the D coefficients (von_mises.rs:124-167), deviator, radius
+    // J entries over all threads of the element (one per IP and entry, the sum over the nodes in
+    // ascending order), parked where J^-1 goes
+    if (active)
+      for (int w = te; w < nip * ND * ND; w += TPE) {
+        const int q = w / (ND * ND), ij = w - q * ND * ND, i = ij / ND, j = ij - i * ND;
+        const double* dN = sdN + q * DSN;
+        double sum = 0.0;
+#pragma unroll 4
+        for (int m = 0; m < NN; m++) sum += sX[m * ND + i] * dN[m * ND + j];
+        sQ[q * QW + 3 + ij] = sum;
+      }
+    __syncthreads();
+    // one lane per IP: J^-1, c times the D coefficients (von_mises.rs:124-167), deviator, radius
     if (active && wv == 0 && l < nip) {
       const int q = l;
       double J[ND][ND];
 #pragma unroll
       for (int i = 0; i < ND; i++)
 #pragma unroll
-        for (int j = 0; j < ND; j++) J[i][j] = 0.0;
-      const double* dN = sdN + q * DSN;
-#pragma unroll 2
-      for (int m = 0; m < NN; m++)
-#pragma unroll
-        for (int i = 0; i < ND; i++)
-#pragma unroll
-          for (int j = 0; j < ND; j++) J[i][j] += sX[m * ND + i] * dN[m * ND + j];
+        for (int j = 0; j < ND; j++) J[i][j] = sQ[q * QW + 3 + i * ND + j];
       double Ji[ND][ND], det;
       if constexpr (ND == 2) {
         det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
