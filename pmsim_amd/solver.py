@@ -186,9 +186,15 @@ class Fem:
             p = np.asarray(pts, dtype=np.int32)
             v = np.ascontiguousarray(v, dtype=np.float64)
             self._check(L.pm_add_facet_qn(h, int(fk), len(p), p, int(cell), v, len(v)))
+        self.device = device
         if initialize:
-            self._check(L.pm_initialize(h, device))
-            self.refresh_sizes()
+            self.initialize()
+
+    def initialize(self):
+        """Uploads the problem to the device (pm_initialize); done by the constructor unless it was
+        built with initialize=False (serving initializes at admission)."""
+        self._check(lib().pm_initialize(self.h, self.device))
+        self.refresh_sizes()
 
     def refresh_sizes(self):
         sz = np.zeros(6, dtype=np.int64)
