@@ -10,6 +10,10 @@ using namespace pm;
 
 static thread_local std::string g_last_error;
 
+#define CHECK_HANDLE(f)                                                        \
+  do {                                                                         \
+    if (!(f)) { g_last_error = "null handle"; return 1; }                      \
+  } while (0)
 #define CHECK_INIT(f)                                                          \
   do {                                                                         \
     if (!(f)) { g_last_error = "null handle"; return 1; }                      \
@@ -49,10 +53,15 @@ void pm_default_config(pm_config* c) {
 
 int pm_fem_create(pm_fem** out, int32_t ndim, int64_t npoint, const double* coords, int64_t ncell, const int32_t* kinds,
                   const int32_t* attributes, const int64_t* cell_ptr, const int32_t* cell_points) {
+  if (!out) { g_last_error = "null output handle"; return 1; }
   *out = nullptr;
   if (ndim != 2 && ndim != 3) { g_last_error = "ndim must be 2 or 3"; return 1; }
   if (npoint < 1) { g_last_error = "there must be at least one point in the mesh"; return 1; }
   if (ncell < 1) { g_last_error = "there are no cells in the mesh"; return 1; }
+  if (!coords || !kinds || !attributes || !cell_ptr || !cell_points) { g_last_error = "null mesh array"; return 1; }
+  if (cell_ptr[0] != 0) { g_last_error = "cell pointers must start at 0 and not decrease"; return 1; }
+  for (int64_t e = 0; e < ncell; e++)
+    if (cell_ptr[e + 1] < cell_ptr[e]) { g_last_error = "cell pointers must start at 0 and not decrease"; return 1; }
   auto* f = new pm_fem();
   f->ndim = ndim; f->npoint = npoint; f->ncell = ncell;
   f->X.assign(coords, coords + npoint * ndim);
@@ -73,6 +82,7 @@ void pm_fem_destroy(pm_fem* f) {
 
 int pm_set_param_solid(pm_fem* f, int32_t attribute, int32_t model, double young, double poisson, double hh, double z_ini,
                        int32_t ngauss) {
+  CHECK_HANDLE(f);
   if (model != 0 && model != 1) { f->err = "unknown stress-strain model"; return 1; }
   ParamSolid p;
   p.model = model; p.young = young; p.poisson = poisson; p.hh = hh; p.z_ini = z_ini; p.ngauss = ngauss; p.set = true;
@@ -81,6 +91,7 @@ int pm_set_param_solid(pm_fem* f, int32_t attribute, int32_t model, double young
 }
 
 int pm_set_idealization(pm_fem* f, int32_t axisymmetric, int32_t plane_stress, double thickness) {
+  CHECK_HANDLE(f);
   if (thickness <= 0.0) { f->err = "thickness must be > 0.0"; return 1; }
   if ((axisymmetric || plane_stress) && f->ndim != 2) { f->err = "axisymmetric or plane-stress requires ndim = 2"; return 1; }
   f->axisym = axisymmetric != 0; f->plane_stress = plane_stress != 0; f->thickness = thickness;
@@ -88,6 +99,8 @@ int pm_set_idealization(pm_fem* f, int32_t axisymmetric, int32_t plane_stress, d
 }
 
 int pm_set_config(pm_fem* f, const pm_config* c) {
+  CHECK_HANDLE(f);
+  if (!c) { f->err = "null configuration"; return 1; }
   if (c->dt <= 0.0) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
@@ -101,7 +114,9 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
 }
 
 int pm_set_essential(pm_fem* f, int64_t n, const int32_t* points, const int32_t* dofs, const double* table, int32_t nstation) {
+  CHECK_HANDLE(f);
   if (f->initialized) { f->err = "essential conditions must be set before pm_initialize"; return 1; }
+  if (n < 0 || (n > 0 && (nstation < 1 || !points || !dofs || !table))) { f->err = "an essential table needs at least one station"; return 1; }
   f->ess_point.assign(points, points + n);
   f->ess_dof.assign(dofs, dofs + n);
   f->ess_table.assign(table, table + n * (int64_t)nstation);
@@ -110,6 +125,8 @@ int pm_set_essential(pm_fem* f, int64_t n, const int32_t* points, const int32_t*
 }
 
 int pm_set_point_loads(pm_fem* f, int64_t n, const int32_t* points, const int32_t* dofs, const double* table, int32_t nstation) {
+  CHECK_HANDLE(f);
+  if (n < 0 || (n > 0 && (!points || !dofs || !table))) { f->err = "null load table"; return 1; }
   if (f->initialized) { f->err = "natural conditions must be set before pm_initialize"; return 1; }
   if (nstation < 1) { f->err = "a load table needs at least one station"; return 1; }
   for (int64_t i = 0; i < n; i++)
@@ -124,6 +141,8 @@ int pm_set_point_loads(pm_fem* f, int64_t n, const int32_t* points, const int32_
 
 int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell, const double* values,
                     int32_t nstation) {
+  CHECK_HANDLE(f);
+  if (!points || !values) { f->err = "null facet points or values"; return 1; }
   if (f->initialized) { f->err = "natural conditions must be set before pm_initialize"; return 1; }
   if (cell < 0 || cell >= f->ncell) { f->err = "facet owner cell out of range"; return 1; }
   if (kind_nnode(facet_kind) != npts) { f->err = "wrong number of points for facet kind"; return 1; }
@@ -226,11 +245,14 @@ int pm_element_pass(pm_fem* f) {
 
 int pm_solve(pm_fem* f, int32_t* n_converged) {
   CHECK_INIT(f);
+  if (!n_converged) { f->err = "null output"; return 1; }
   (void)hipSetDevice(f->device);
   return newton_solve(f, n_converged);
 }
 
 int pm_get_stats(const pm_fem* f, pm_stats* s) {
+  CHECK_HANDLE(f);
+  if (!s) return 1;
   *s = f->stats;
   s->n_ip = f->nip;
   s->n_block_nnz = f->nnzb;
@@ -239,6 +261,8 @@ int pm_get_stats(const pm_fem* f, pm_stats* s) {
 }
 
 int pm_sizes(const pm_fem* f, int64_t* out) {
+  CHECK_HANDLE(f);
+  if (!out) return 1;
   if (!f->initialized && !f->released) return 1;
   out[0] = f->ndof; out[1] = f->nprescribed; out[2] = f->nip; out[3] = f->nsig; out[4] = f->nnzb; out[5] = f->ndim;
   return 0;
@@ -246,12 +270,15 @@ int pm_sizes(const pm_fem* f, int64_t* out) {
 
 int pm_set_ddu(pm_fem* f, const double* ddu) {
   CHECK_INIT(f);
+  if (!ddu) { f->err = "null increment"; return 1; }
   PM_HIP(hipMemcpyAsync(f->d_ddu, ddu, sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream));
   PM_HIP(hipStreamSynchronize(f->stream));
   return 0;
 }
 
 int pm_get_u(const pm_fem* f, double* u, double* lambda) {
+  CHECK_HANDLE(f);
+  if (!u) return 1;
   if (f->released) {
     std::copy(f->rel_u.begin(), f->rel_u.end(), u);
     if (lambda) std::copy(f->rel_lam.begin(), f->rel_lam.end(), lambda);
@@ -265,12 +292,16 @@ int pm_get_u(const pm_fem* f, double* u, double* lambda) {
 }
 
 int pm_get_f_int(const pm_fem* f, double* out) {
+  CHECK_HANDLE(f);
+  if (!out) return 1;
   if (!f->initialized) return 1;
   if (hipMemcpyAsync(out, f->d_fint, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
   return hipStreamSynchronize(f->stream) == hipSuccess ? 0 : 1;
 }
 
 int pm_get_ip_state(const pm_fem* f, double* v) {
+  CHECK_HANDLE(f);
+  if (!v) return 1;
   if (f->released) { std::copy(f->rel_ip.begin(), f->rel_ip.end(), v); return 0; }
   if (!f->initialized) return 1;
   return copy_ip_state_to_host(f, v);
@@ -278,10 +309,13 @@ int pm_get_ip_state(const pm_fem* f, double* v) {
 
 int pm_set_ip_state(pm_fem* f, const double* v) {
   CHECK_INIT(f);
+  if (!v) { f->err = "null IP state"; return 1; }
   return copy_ip_state_to_device(f, v);
 }
 
 int pm_get_csr(const pm_fem* f, int64_t* rowptr, int32_t* cols, double* vals) {
+  CHECK_HANDLE(f);
+  if (!rowptr) return 1;
   if (!f->initialized) return 1;
   const int nd = f->ndim;
   std::vector<double> kv;
@@ -301,9 +335,15 @@ int pm_get_csr(const pm_fem* f, int64_t* rowptr, int32_t* cols, double* vals) {
   return 0;
 }
 
-int pm_history_count(const pm_fem* f, int32_t* n) { *n = (int32_t)f->hist_u.size(); return 0; }
+int pm_history_count(const pm_fem* f, int32_t* n) {
+  CHECK_HANDLE(f);
+  if (!n) return 1;
+  *n = (int32_t)f->hist_u.size();
+  return 0;
+}
 
 int pm_get_history(const pm_fem* f, int32_t index, double* u, double* lambda, double* ip_state, double* t) {
+  CHECK_HANDLE(f);
   if (index < 0 || index >= (int32_t)f->hist_u.size()) return 1;
   if (u) std::memcpy(u, f->hist_u[index].data(), sizeof(double) * f->ndof);
   if (lambda && f->nprescribed) std::memcpy(lambda, f->hist_lam[index].data(), sizeof(double) * f->nprescribed);
@@ -313,6 +353,7 @@ int pm_get_history(const pm_fem* f, int32_t index, double* u, double* lambda, do
 }
 
 int pm_get_history_loading_factor(const pm_fem* f, int32_t index, double* ell) {
+  CHECK_HANDLE(f);
   if (index < 0 || index >= (int32_t)f->hist_ell.size() || !ell) return 1;
   *ell = f->hist_ell[index];
   return 0;
@@ -320,6 +361,7 @@ int pm_get_history_loading_factor(const pm_fem* f, int32_t index, double* ell) {
 
 int pm_time_element_pass(pm_fem* f, int32_t reps, double* ms) {
   CHECK_INIT(f);
+  if (!ms || reps < 1) { f->err = "pm_time_element_pass needs reps >= 1 and an output array"; return 1; }
   hipEvent_t ev[5];
   for (auto& e : ev) PM_HIP(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};
@@ -362,6 +404,7 @@ int pm_time_element_pass(pm_fem* f, int32_t reps, double* ms) {
 
 int pm_linear_solve(pm_fem* f, const double* b, double* x, int32_t* iterations) {
   CHECK_INIT(f);
+  if (!b || !x || !iterations) { f->err = "null argument"; return 1; }
   double* db = f->d_pcg + 6 * f->ndof;  // scratch beyond the PCG work vectors
   double* dx = f->d_mdu;
   PM_HIP(hipMemcpyAsync(db, b, sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream));

@@ -42,6 +42,53 @@ def test_library_errors_without_device():
     assert rc != 0 and L.pm_global_last_error().decode() == "there are no cells in the mesh"
 
 
+def test_c_abi_rejects_null_and_bad_arguments():
+    """Every entry point checks its handle and required pointers instead of dereferencing them; bad
+    tables and cell pointers are refused with a message (all without a device)."""
+    import ctypes as C
+    from pmsim_amd import solver
+    solver.lib()
+    R = C.CDLL(solver.LIB_PATH)  # a second view of the same library, without argument declarations
+    R.pm_global_last_error.restype = C.c_char_p
+    R.pm_last_error.restype = C.c_char_p
+    R.pm_last_error.argtypes = [C.c_void_p]
+    gerr = lambda: R.pm_global_last_error().decode()
+    d = C.c_double
+    assert R.pm_set_param_solid(None, 1, 0, d(1.0), d(0.25), d(0.0), d(0.0), 0) == 1 and gerr() == "null handle"
+    for call in (lambda: R.pm_set_idealization(None, 0, 0, d(1.0)), lambda: R.pm_set_config(None, None),
+                 lambda: R.pm_set_essential(None, C.c_int64(0), None, None, None, 1),
+                 lambda: R.pm_set_point_loads(None, C.c_int64(0), None, None, None, 1),
+                 lambda: R.pm_add_facet_qn(None, 0, 0, None, C.c_int64(0), None, 1),
+                 lambda: R.pm_solve(None, None), lambda: R.pm_get_u(None, None, None),
+                 lambda: R.pm_get_ip_state(None, None), lambda: R.pm_get_stats(None, None),
+                 lambda: R.pm_history_count(None, None), lambda: R.pm_get_history(None, 0, None, None, None, None),
+                 lambda: R.pm_get_csr(None, None, None, None), lambda: R.pm_sizes(None, None),
+                 lambda: R.pm_time_element_pass(None, 1, None), lambda: R.pm_linear_solve(None, None, None, None)):
+        assert call() == 1 and gerr() == "null handle"
+    assert R.pm_fem_create(None, 2, C.c_int64(1), None, C.c_int64(1), None, None, None, None) == 1
+    assert gerr() == "null output handle"
+    h = C.c_void_p()
+    X = np.zeros(8)
+    kinds = np.full(1, M.QUA4, dtype=np.int32)
+    attrs = np.ones(1, dtype=np.int32)
+    pts = np.arange(4, dtype=np.int32)
+    bad_ptr = np.array([1, 4], dtype=np.int64)
+    assert R.pm_fem_create(C.byref(h), 2, C.c_int64(4), X.ctypes, C.c_int64(1), kinds.ctypes, attrs.ctypes,
+                           bad_ptr.ctypes, pts.ctypes) == 1
+    assert gerr() == "cell pointers must start at 0 and not decrease"
+    ptr = np.array([0, 4], dtype=np.int64)
+    assert R.pm_fem_create(C.byref(h), 2, C.c_int64(4), X.ctypes, C.c_int64(1), kinds.ctypes, attrs.ctypes,
+                           ptr.ctypes, pts.ctypes) == 0
+    try:
+        p1, d1, t1 = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32), np.zeros(1)
+        assert R.pm_set_essential(h, C.c_int64(1), p1.ctypes, d1.ctypes, t1.ctypes, 0) == 1
+        assert R.pm_last_error(h).decode() == "an essential table needs at least one station"
+        assert R.pm_set_config(h, None) == 1 and R.pm_last_error(h).decode() == "null configuration"
+        assert R.pm_solve(h, None) == 1 and R.pm_last_error(h).decode() == "pm_initialize has not been called"
+    finally:
+        R.pm_fem_destroy(h)
+
+
 def test_essential_override_and_tables():
     mesh, base, cfg, ess, nat = cases.spo_754_footing()
     d = describe(mesh, base, cfg, ess, nat)
