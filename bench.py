@@ -383,12 +383,13 @@ def main():
                      "executed_frac": EXECUTED_FLOP_PER_IP * nip / (ms["tangent"] * 1e-3) / 1e12 / PEAK_TFLOPS[a.precision]},
     }
     # the scatter is the HBM-bound half of the assembly: the stored element matrices in (upper node blocks,
-    # 210 of the 400 per Hex20 element, 1,892 values with padding), CSR blocks out (in the tangent's
-    # precision), one map entry per element block pair and the map pointers
+    # 210 of the 400 per Hex20 element, 1,892 values with padding), the stored upper CSR blocks out (in the
+    # tangent's precision), one map entry per element upper block and the map pointers
     nel = f.nip // 27
     ke_values = nel * (((20 * 21 // 2) * 9 + 3) // 4 * 4)
     vbytes = 8 if a.precision == "f64" else 4
-    scatter_bytes = ke_values * vbytes + f.nnzb * 9 * vbytes + nel * 400 * 4 + (f.nnzb + 1) * 4
+    nnzu = (f.nnzb + f.ndof // 3) // 2
+    scatter_bytes = ke_values * vbytes + nnzu * 9 * vbytes + nel * 210 * 4 + (nnzu + 1) * 4
     out["hbm"] = {"kernel": "k_scatter", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": scatter_bytes}
     if not a.no_cpu:

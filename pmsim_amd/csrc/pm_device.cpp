@@ -69,9 +69,14 @@ int device_alloc(pm_fem* f) {
     PM_HIP(hipStreamSynchronize(f->stream));  // host vectors go out of scope
   }
   // patterns and maps
-  if (dalloc(f, &f->d_browptr, f->npoint + 1) || upload(f, f->d_browptr, f->h_browptr.data(), f->npoint + 1)) return 1;
-  if (dalloc(f, &f->d_bcols, f->nnzb) || upload(f, f->d_bcols, f->h_bcols.data(), f->nnzb)) return 1;
-  if (dalloc(f, &f->d_kmap_ptr, f->nnzb + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzb + 1)) return 1;
+  // K is held as its upper node blocks; each row's lower blocks are listed with their stored mirrors
+  const int64_t nnzl = (int64_t)f->h_lcol.size();
+  if (dalloc(f, &f->d_urp, f->npoint + 1) || upload(f, f->d_urp, f->h_urp.data(), f->npoint + 1)) return 1;
+  if (dalloc(f, &f->d_ucol, f->nnzu) || upload(f, f->d_ucol, f->h_ucol.data(), f->nnzu)) return 1;
+  if (dalloc(f, &f->d_lrp, f->npoint + 1) || upload(f, f->d_lrp, f->h_lrp.data(), f->npoint + 1)) return 1;
+  if (dalloc(f, &f->d_lcol, nnzl) || upload(f, f->d_lcol, f->h_lcol.data(), nnzl)) return 1;
+  if (dalloc(f, &f->d_lidx, nnzl) || upload(f, f->d_lidx, f->h_lidx.data(), nnzl)) return 1;
+  if (dalloc(f, &f->d_kmap_ptr, f->nnzu + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzu + 1)) return 1;
   if (dalloc(f, &f->d_korder, (int64_t)f->h_korder.size()) || upload(f, f->d_korder, f->h_korder.data(), (int64_t)f->h_korder.size())) return 1;
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
@@ -79,10 +84,10 @@ int device_alloc(pm_fem* f) {
   // K values: FP64 for FP64 tangents; FP32 only for reduced-precision tangents (formed, stored and
   // multiplied in FP32; readers that need FP64 widen them)
   if (f->cfg.tangent_precision == 0) {
-    if (dalloc(f, &f->d_kval, f->nnzb * b2)) return 1;
+    if (dalloc(f, &f->d_kval, f->nnzu * b2)) return 1;
     if (dalloc(f, &f->d_ke, f->n_ke_values)) return 1;
   } else {
-    if (dalloc(f, &f->d_ke32, f->n_ke_values) || dalloc(f, &f->d_kval32, f->nnzb * b2)) return 1;
+    if (dalloc(f, &f->d_ke32, f->n_ke_values) || dalloc(f, &f->d_kval32, f->nnzu * b2)) return 1;
   }
   if (dalloc(f, &f->d_fe, f->n_fe)) return 1;
   // vectors
@@ -125,14 +130,14 @@ void device_free(pm_fem* f) {
   release_graphs(f);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
-  fr(f->d_X); fr(f->d_browptr); fr(f->d_bcols); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
+  fr(f->d_X); fr(f->d_urp); fr(f->d_ucol); fr(f->d_lrp); fr(f->d_lcol); fr(f->d_lidx); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
   fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
   fr(f->d_arc); f->d_arc = nullptr;
   if (f->stream) (void)hipStreamDestroy(f->stream);
   f->stream = nullptr;
-  f->d_X = nullptr; f->d_browptr = f->d_bcols = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;
+  f->d_X = nullptr; f->d_urp = f->d_ucol = f->d_lrp = f->d_lcol = f->d_lidx = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;
   f->d_kmap_ptr = f->d_kmap = nullptr; f->d_korder = nullptr; f->d_fe = nullptr; f->d_fmap_ptr = f->d_fmap = nullptr;
   f->d_u = f->d_ddu = f->d_fint = f->d_fext = f->d_rr = f->d_mdu = f->d_mdu0 = f->d_lam = nullptr;
   f->d_y = f->d_cval = nullptr; f->d_peq = nullptr; f->d_free = f->d_pcg = f->d_binv = f->d_red = f->d_scal = nullptr; f->d_err = nullptr;

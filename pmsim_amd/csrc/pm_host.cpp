@@ -139,45 +139,88 @@ int build_patterns(pm_fem* f) {
     std::copy(rows[a].begin(), rows[a].end(), f->h_bcols.begin() + f->h_browptr[a]);
     std::vector<int32_t>().swap(rows[a]);
   }
-  // K gather map: count contributions per block
-  std::vector<int32_t> kptr(nnzb + 1, 0);
-  auto block_index = [&](int32_t a, int32_t b) -> int64_t {
-    const int32_t* lo = f->h_bcols.data() + f->h_browptr[a];
-    const int32_t* hi = f->h_bcols.data() + f->h_browptr[a + 1];
-    return (int64_t)(std::lower_bound(lo, hi, b) - f->h_bcols.data());
+  // upper pattern (b >= a), full -> upper map and the lower blocks of each row (the pattern is symmetric)
+  f->h_urp.assign(np + 1, 0);
+  for (int64_t a = 0; a < np; a++) {
+    int32_t nu = 0;
+    for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++) nu += f->h_bcols[k] >= a ? 1 : 0;
+    f->h_urp[a + 1] = f->h_urp[a] + nu;
+  }
+  const int64_t nnzu = f->h_urp[np];
+  if (2 * nnzu != nnzb + np) { f->err = "internal error: the block pattern is not symmetric"; return 1; }
+  f->nnzu = nnzu;
+  f->h_ucol.resize(nnzu);
+  for (int64_t a = 0; a < np; a++) {
+    int32_t u = f->h_urp[a];
+    for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++)
+      if (f->h_bcols[k] >= a) f->h_ucol[u++] = f->h_bcols[k];
+  }
+  for (int64_t a = 0; a < np; a++)  // block Jacobi reads the diagonal as the first stored block of a row
+    if (f->h_urp[a] == f->h_urp[a + 1] || f->h_ucol[f->h_urp[a]] != a) { f->err = "internal error: a row without its diagonal block"; return 1; }
+  auto upper_index = [&](int32_t a, int32_t b) -> int64_t {  // stored block (a, b), a <= b
+    const int32_t* lo = f->h_ucol.data() + f->h_urp[a];
+    const int32_t* hi = f->h_ucol.data() + f->h_urp[a + 1];
+    const int32_t* it = std::lower_bound(lo, hi, b);
+    return (it != hi && *it == b) ? (int64_t)(it - f->h_ucol.data()) : -1;
   };
-  // per element, block index of every (la, lb) local pair
+  f->h_full2up.resize(nnzb);
+  f->h_lrp.assign(np + 1, 0);
+  f->h_lcol.clear();
+  f->h_lidx.clear();
+  f->h_lcol.reserve(nnzb - nnzu);
+  f->h_lidx.reserve(nnzb - nnzu);
+  for (int64_t a = 0; a < np; a++) {
+    for (int32_t k = f->h_browptr[a]; k < f->h_browptr[a + 1]; k++) {
+      const int32_t b = f->h_bcols[k];
+      const int64_t u = b >= a ? upper_index((int32_t)a, b) : upper_index(b, (int32_t)a);
+      if (u < 0) { f->err = "internal error: the block pattern is not symmetric"; return 1; }
+      f->h_full2up[k] = b >= a ? (int32_t)u : (int32_t)((uint32_t)u | 0x80000000u);
+      if (b < a) { f->h_lcol.push_back(b); f->h_lidx.push_back((int32_t)u); }
+    }
+    f->h_lrp[a + 1] = (int32_t)f->h_lcol.size();
+  }
+  // K gather map over the stored (upper) blocks: element node pairs (la, lb) with point(la) <= point(lb).
+  // Element matrices hold their nodes in rank order (ascending point id), so such a pair is the stored
+  // element block (rank[la], rank[lb]) with rank[la] <= rank[lb]: every read is direct.
+  std::vector<int32_t> kptr(nnzu + 1, 0);
   int64_t total = 0;
-  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; total += n * n; }
+  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; total += n * (n + 1) / 2; }
   if (f->n_ke_values >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) { f->err = "mesh too large for int32 scatter maps"; return 1; }
   std::vector<int32_t> pairblk(total);
   std::vector<int64_t> pbase(f->ncell + 1, 0);
-  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; pbase[e + 1] = pbase[e] + n * n; }
-#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < f->ncell; e++) { int64_t n = f->cptr[e + 1] - f->cptr[e]; pbase[e + 1] = pbase[e] + n * (n + 1) / 2; }
+  bool bad_pair = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad_pair)
   for (int64_t e = 0; e < f->ncell; e++) {
-    int64_t n = f->cptr[e + 1] - f->cptr[e];
+    int64_t n = f->cptr[e + 1] - f->cptr[e], w = pbase[e];
     const int32_t* pts = &f->cpts[f->cptr[e]];
     for (int64_t la = 0; la < n; la++)
-      for (int64_t lb = 0; lb < n; lb++) pairblk[pbase[e] + la * n + lb] = (int32_t)block_index(pts[la], pts[lb]);
+      for (int64_t lb = 0; lb < n; lb++) {
+        if (pts[la] > pts[lb]) continue;
+        const int64_t u = upper_index(pts[la], pts[lb]);
+        if (u < 0) bad_pair = true;
+        pairblk[w++] = (int32_t)u;
+      }
   }
+  if (bad_pair) { f->err = "internal error: element pair outside the block pattern"; return 1; }
   for (int64_t i = 0; i < total; i++) kptr[pairblk[i] + 1]++;
-  for (int64_t i = 0; i < nnzb; i++) kptr[i + 1] += kptr[i];
+  for (int64_t i = 0; i < nnzu; i++) kptr[i + 1] += kptr[i];
   std::vector<int32_t> kmap(total);
   {
     std::vector<int32_t> fill(kptr.begin(), kptr.end() - 1);
     for (int64_t e = 0; e < f->ncell; e++) {  // ascending cells => ascending contributions
       const Group& g = f->groups[f->cell_group[e]];
-      int64_t n = g.nnode, el = f->cell_local[e];
+      int64_t n = g.nnode, el = f->cell_local[e], w = pbase[e];
+      const int32_t* pts = &f->cpts[f->cptr[e]];
       int rk[32];
       cell_ranks(f, e, rk);
       for (int64_t la = 0; la < n; la++)
         for (int64_t lb = 0; lb < n; lb++) {
-          // node block (la, lb) is the stored upper block (rank[la], rank[lb]) or, below the diagonal,
-          // the transpose of (rank[lb], rank[la]): value offset, sign bit set for a transposed read
+          if (pts[la] > pts[lb]) continue;
           const int ra = rk[la], rb = rk[lb];
-          const int64_t off = g.ke_base + el * g.kew + (int64_t)packed_block((int)n, std::min(ra, rb), std::max(ra, rb)) * g.ndim * g.ndim;
-          int32_t pb = pairblk[pbase[e] + la * n + lb];
-          kmap[fill[pb]++] = ra <= rb ? (int32_t)off : (int32_t)((uint32_t)off | 0x80000000u);
+          if (ra > rb) { f->err = "internal error: element ranks out of order"; return 1; }
+          const int64_t off = g.ke_base + el * g.kew + (int64_t)packed_block((int)n, ra, rb) * g.ndim * g.ndim;
+          kmap[fill[pairblk[w++]]++] = (int32_t)off;
         }
     }
   }
@@ -190,10 +233,10 @@ int build_patterns(pm_fem* f) {
   // smallest element-matrix offset a chunk reads, so chunks of different node layers that read the same
   // elements run close together and find those matrices in L2.  The sums do not depend on the order.
   {
-    const int64_t nchunk = (nnzb + GATHER_CHUNK - 1) / GATHER_CHUNK;
+    const int64_t nchunk = (nnzu + GATHER_CHUNK - 1) / GATHER_CHUNK;
     std::vector<int32_t> key(nchunk, INT32_MAX);
     for (int64_t c = 0; c < nchunk; c++)
-      for (int64_t b = c * GATHER_CHUNK; b < std::min<int64_t>(nnzb, (c + 1) * GATHER_CHUNK); b++)
+      for (int64_t b = c * GATHER_CHUNK; b < std::min<int64_t>(nnzu, (c + 1) * GATHER_CHUNK); b++)
         for (int32_t k = kptr[b]; k < kptr[b + 1]; k++) key[c] = std::min(key[c], (int32_t)((uint32_t)kmap[k] & 0x7fffffffu));
     f->h_korder.resize(nchunk);
     std::iota(f->h_korder.begin(), f->h_korder.end(), 0);
@@ -284,17 +327,32 @@ void assemble_f_ext(pm_fem* f, int64_t k) {
 // ----------------------------------------------------------------------------------------------
 // host direct solver: envelope (skyline) Cholesky of K_ff after a reverse Cuthill-McKee ordering
 // ----------------------------------------------------------------------------------------------
+// K values in the full block pattern (h_browptr / h_bcols), as FP64 numbers: the stored upper blocks are
+// copied down (FP32 values widened) and each lower block is the transpose of its stored mirror.
 int copy_kval_to_host(const pm_fem* f, std::vector<double>& kv) {
-  const size_t n = (size_t)f->nnzb * f->ndim * f->ndim;
-  kv.resize(n);
-  if (f->d_kval)
-    return (hipMemcpyAsync(kv.data(), f->d_kval, n * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
-            hipStreamSynchronize(f->stream) != hipSuccess) ? 1 : 0;
-  std::vector<float> k32(n);
-  if (hipMemcpyAsync(k32.data(), f->d_kval32, n * sizeof(float), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
-      hipStreamSynchronize(f->stream) != hipSuccess)
-    return 1;
-  for (size_t i = 0; i < n; i++) kv[i] = (double)k32[i];
+  const int nd = f->ndim, b2 = nd * nd;
+  const size_t nu = (size_t)f->nnzu * b2;
+  std::vector<double> up(nu);
+  if (f->d_kval) {
+    if (hipMemcpyAsync(up.data(), f->d_kval, nu * sizeof(double), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
+        hipStreamSynchronize(f->stream) != hipSuccess)
+      return 1;
+  } else {
+    std::vector<float> k32(nu);
+    if (hipMemcpyAsync(k32.data(), f->d_kval32, nu * sizeof(float), hipMemcpyDeviceToHost, f->stream) != hipSuccess ||
+        hipStreamSynchronize(f->stream) != hipSuccess)
+      return 1;
+    for (size_t i = 0; i < nu; i++) up[i] = (double)k32[i];
+  }
+  kv.resize((size_t)f->nnzb * b2);
+  for (int64_t k = 0; k < f->nnzb; k++) {
+    const uint32_t m = (uint32_t)f->h_full2up[k];
+    const double* src = up.data() + (size_t)(m & 0x7fffffffu) * b2;
+    double* dst = kv.data() + (size_t)k * b2;
+    const bool tr = (m & 0x80000000u) != 0;
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++) dst[i * nd + j] = tr ? src[j * nd + i] : src[i * nd + j];
+  }
   return 0;
 }
 

@@ -122,9 +122,15 @@ struct pm_fem {
   double ftf = 0.0;                    // F_ext . F_ext (serial, DOF order)
   // node-block CSR of K
   int bs = 2;                          // block size = ndim
-  std::vector<int32_t> h_browptr;      // npoint+1
+  std::vector<int32_t> h_browptr;      // npoint+1  (full pattern: host direct solver, pm_get_csr)
   std::vector<int32_t> h_bcols;        // nnzb
-  int64_t nnzb = 0;
+  int64_t nnzb = 0;                    // node blocks of the full (symmetric) pattern
+  // K is held on the device as its upper node blocks (b >= a); rows read their lower blocks through
+  // the transposes of the stored mirrors (pm_spmv.hpp)
+  int64_t nnzu = 0;                    // upper blocks = (nnzb + npoint) / 2
+  std::vector<int32_t> h_urp, h_ucol;  // upper pattern (npoint+1, nnzu)
+  std::vector<int32_t> h_lrp, h_lcol, h_lidx;  // lower blocks of each row: column, stored upper block
+  std::vector<int32_t> h_full2up;      // full block -> stored upper block, sign bit: read transposed
   std::vector<int32_t> h_kmap_ptr, h_kmap;   // K gather map (host copy until upload)
   std::vector<int32_t> h_korder;             // gather chunks (TPB CSR blocks each) in processing order
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
@@ -139,10 +145,13 @@ struct pm_fem {
   bool released = false;               // device buffers freed after a solve; results kept below
   std::vector<double> rel_u, rel_lam, rel_ip;
   double* d_X = nullptr;               // point coordinates
-  int32_t* d_browptr = nullptr;
-  int32_t* d_bcols = nullptr;
-  double* d_kval = nullptr;            // nnzb*bs*bs (FP64), FP64 tangents only
-  float* d_kval32 = nullptr;           // nnzb*bs*bs (FP32), reduced-precision tangents only
+  int32_t* d_urp = nullptr;            // upper pattern
+  int32_t* d_ucol = nullptr;
+  int32_t* d_lrp = nullptr;            // lower blocks of each row
+  int32_t* d_lcol = nullptr;
+  int32_t* d_lidx = nullptr;
+  double* d_kval = nullptr;            // nnzu*bs*bs (FP64), FP64 tangents only
+  float* d_kval32 = nullptr;           // nnzu*bs*bs (FP32), reduced-precision tangents only
   double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order, packed (n_ke_values)
   float* d_ke32 = nullptr;             // FP32 element blocks
   int64_t n_ke_values = 0;

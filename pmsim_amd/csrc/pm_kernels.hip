@@ -544,9 +544,9 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 }
 
 // deterministic gather of element blocks into node-block CSR values (assembly.rs:124-135): one thread
-// per CSR block sums its contributing element blocks in ascending cell order.  A map entry is the value
-// offset of a stored (upper, rank-order) element block; with the sign bit set the block is read
-// transposed (the contribution lies below the element's diagonal).
+// per stored (upper) CSR block sums its contributing element blocks in ascending cell order.  A map
+// entry is the value offset of a stored (upper, rank-order) element block; with the sign bit set the
+// block would be read transposed (the maps for the upper triangle never set it).
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ korder, const int32_t* __restrict__ kptr,
                                                  const int32_t* __restrict__ kmap, const KT* __restrict__ ke, double* __restrict__ kval,
@@ -791,16 +791,17 @@ int tune_tangent(pm_fem* f) {
 }
 
 static_assert(TPB == GATHER_CHUNK, "the gather's chunk order assumes one workgroup of TPB blocks per chunk");
+// the gather forms the stored (upper) node blocks of K only
 void launch_scatter(pm_fem* f) {
-  const int64_t n = f->nnzb;
+  const int64_t n = f->nnzu;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;
   if (f->bs == 2) {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+    if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
+    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   } else {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, f->nnzb, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+    if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
+    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
   }
 }
 

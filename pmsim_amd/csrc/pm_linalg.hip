@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "pm_internal.hpp"
+#include "pm_spmv.hpp"
 
 namespace pm {
 
@@ -71,35 +72,27 @@ __global__ __launch_bounds__(TPB) void k_red_final(int mode, int npart, const do
   if (threadIdx.x == 0) scal[slot] = r;
 }
 
-// y = A x (node-block CSR, block size BS); masked: y *= free
+// y = A x (node blocks of size BS, upper triangle stored); masked: y *= free
 template <int BS, typename VT>
-__global__ __launch_bounds__(TPB) void k_spmv(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                              const VT* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
+__global__ __launch_bounds__(TPB) void k_spmv(int64_t np, KSym<VT> K, const double* __restrict__ x, double* __restrict__ y,
                                               const double* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= np * BS) return;
   const int64_t a = t / BS;
   const int i = (int)(t - a * BS);
-  double s = 0.0;
-  for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
-    const int64_t b = cols[k];
-    const VT* v = vals + (int64_t)k * BS * BS + i * BS;
-#pragma unroll
-    for (int j = 0; j < BS; j++) s += (double)v[j] * x[b * BS + j];
-  }
+  const double s = ksym_row<BS, VT>(K, a, i, x);
   y[t] = mask ? s * mask[t] : s;
 }
 
 // block-Jacobi inverse of the diagonal node blocks (from the K values in the tangent's precision),
 // identity on prescribed rows/columns
 template <int BS, typename KT>
-__global__ __launch_bounds__(TPB) void k_bjacobi(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                 const KT* __restrict__ vals, const double* __restrict__ free_, double* __restrict__ binv) {
+__global__ __launch_bounds__(TPB) void k_bjacobi(int64_t np, const int32_t* __restrict__ urp, const KT* __restrict__ vals,
+                                                 const double* __restrict__ free_, double* __restrict__ binv) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= np) return;
   double A[BS][BS];
-  int32_t k = rp[a];
-  while (k < rp[a + 1] && cols[k] != a) k++;
+  const int32_t k = urp[a];  // the diagonal block is the first stored block of its row
   for (int i = 0; i < BS; i++)
     for (int j = 0; j < BS; j++) {
       bool fi = free_[a * BS + i] != 0.0, fj = free_[a * BS + j] != 0.0;
@@ -278,21 +271,14 @@ __device__ __forceinline__ void bjacobi_apply(int64_t a, const double* __restric
 
 // K1: q = (A p) * free ; partial p.q
 template <int BS, typename VT>
-__global__ __launch_bounds__(PCG_TPB) void k_pcg_q(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                   const VT* __restrict__ vals, const double* __restrict__ free_,
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_q(int64_t np, KSym<VT> K, const double* __restrict__ free_,
                                                    const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
   __shared__ double sh[PCG_TPB];
   double acc = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * BS; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t a = t / BS;
     const int i = (int)(t - a * BS);
-    double s = 0.0;
-    for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
-      const int64_t b = cols[k];
-      const VT* v = vals + (int64_t)k * BS * BS + i * BS;
-#pragma unroll
-      for (int j = 0; j < BS; j++) s += (double)v[j] * p[b * BS + j];
-    }
+    double s = ksym_row<BS, VT>(K, a, i, p);
     s *= free_[t];
     q[t] = s;
     acc += p[t] * s;
@@ -387,8 +373,7 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
 }
 
 template <int BS, typename VT>
-__global__ __launch_bounds__(SMALL_TPB) void k_pcg_small(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                         const VT* __restrict__ vals, const double* __restrict__ free_,
+__global__ __launch_bounds__(SMALL_TPB) void k_pcg_small(int64_t np, KSym<VT> K, const double* __restrict__ free_,
                                                          const double* __restrict__ binv, const double* __restrict__ b,
                                                          double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
                                                          double* __restrict__ p, double* __restrict__ q, double rtol2, int maxit,
@@ -415,13 +400,7 @@ __global__ __launch_bounds__(SMALL_TPB) void k_pcg_small(int64_t np, const int32
       for (int64_t t = threadIdx.x; t < n; t += SMALL_TPB) {
         const int64_t a = t / BS;
         const int i = (int)(t - a * BS);
-        double s = 0.0;
-        for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
-          const int64_t c = cols[k];
-          const VT* v = vals + (int64_t)k * BS * BS + i * BS;
-#pragma unroll
-          for (int j = 0; j < BS; j++) s += (double)v[j] * p[c * BS + j];
-        }
+        double s = ksym_row<BS, VT>(K, a, i, p);
         s *= free_[t];
         q[t] = s;
         pq += p[t] * s;
@@ -462,6 +441,9 @@ inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB
 // ================================================================================================
 double* scal_ptr(pm_fem* f, int slot) { return f->d_scal + slot; }
 
+template <typename VT>
+static KSym<VT> ksym(const pm_fem* f, const VT* vals) { return KSym<VT>{f->d_urp, f->d_ucol, f->d_lrp, f->d_lcol, f->d_lidx, vals}; }
+
 static void reduce(pm_fem* f, int op, int64_t n, const double* a, const double* b, const double* mask, int slot) {
   unsigned grid = (unsigned)std::min<int64_t>(RED_BLOCKS, std::max<int64_t>(1, (n + TPB - 1) / TPB));
   hipLaunchKernelGGL(k_red_partial, dim3(grid), dim3(TPB), 0, f->stream, op, n, a, b, mask, f->d_red);
@@ -497,22 +479,22 @@ void dev_spmv(pm_fem* f, const double* x, double* y, bool masked) {
   const double* mask = masked ? f->d_free : nullptr;
   const bool lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
   if (f->bs == 2) {
-    if (lowp) hipLaunchKernelGGL((k_spmv<2, float>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, x, y, mask);
-    else hipLaunchKernelGGL((k_spmv<2, double>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, x, y, mask);
+    if (lowp) hipLaunchKernelGGL((k_spmv<2, float>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, ksym(f, f->d_kval32), x, y, mask);
+    else hipLaunchKernelGGL((k_spmv<2, double>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, ksym(f, f->d_kval), x, y, mask);
   } else {
-    if (lowp) hipLaunchKernelGGL((k_spmv<3, float>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, x, y, mask);
-    else hipLaunchKernelGGL((k_spmv<3, double>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, x, y, mask);
+    if (lowp) hipLaunchKernelGGL((k_spmv<3, float>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, ksym(f, f->d_kval32), x, y, mask);
+    else hipLaunchKernelGGL((k_spmv<3, double>), dim3(nblk(n)), dim3(TPB), 0, f->stream, f->npoint, ksym(f, f->d_kval), x, y, mask);
   }
 }
 
 void build_block_jacobi(pm_fem* f) {
   const unsigned g = nblk(f->npoint);
   if (f->d_kval) {
-    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
-    else hipLaunchKernelGGL((k_bjacobi<3, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, f->d_binv);
+    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_urp, f->d_kval, f->d_free, f->d_binv);
+    else hipLaunchKernelGGL((k_bjacobi<3, double>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_urp, f->d_kval, f->d_free, f->d_binv);
   } else {
-    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, f->d_binv);
-    else hipLaunchKernelGGL((k_bjacobi<3, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, f->d_binv);
+    if (f->bs == 2) hipLaunchKernelGGL((k_bjacobi<2, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_urp, f->d_kval32, f->d_free, f->d_binv);
+    else hipLaunchKernelGGL((k_bjacobi<3, float>), dim3(g), dim3(TPB), 0, f->stream, f->npoint, f->d_urp, f->d_kval32, f->d_free, f->d_binv);
   }
 }
 
@@ -530,7 +512,7 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   *iters = 0;
   if (n <= f->pcg_small_max) {
     double* out = f->d_scal + 20;
-#define PM_SMALL(BS, VT, VALS) hipLaunchKernelGGL((k_pcg_small<BS, VT>), dim3(1), dim3(SMALL_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, VALS, \
+#define PM_SMALL(BS, VT, VALS) hipLaunchKernelGGL((k_pcg_small<BS, VT>), dim3(1), dim3(SMALL_TPB), 0, f->stream, np, ksym(f, VALS), \
                                                   f->d_free, f->d_binv, d_b, d_x, r, z, p, q, rtol2, f->cfg.pcg_max_iter, out)
     if (f->bs == 2) { if (lowp) PM_SMALL(2, float, f->d_kval32); else PM_SMALL(2, double, f->d_kval); }
     else { if (lowp) PM_SMALL(3, float, f->d_kval32); else PM_SMALL(3, double, f->d_kval); }
@@ -570,12 +552,12 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
     const double* rz_old = rzs + (it & 1);
     double* rz_new = rzs + ((it + 1) & 1);
     if (f->bs == 2) {
-      if (lowp) hipLaunchKernelGGL((k_pcg_q<2, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, p, q, pa);
-      else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, p, q, pa);
+      if (lowp) hipLaunchKernelGGL((k_pcg_q<2, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
+      else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     } else {
-      if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval32, f->d_free, p, q, pa);
-      else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_browptr, f->d_bcols, f->d_kval, f->d_free, p, q, pa);
+      if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
+      else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     }
     hipLaunchKernelGGL(k_pcg_p2, dim3(rb), dim3(PCG_TPB), 0, f->stream, n, rb, pb, pc, rz_old, rz_new, rr_s, z, p);

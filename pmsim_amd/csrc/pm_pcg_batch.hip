@@ -18,6 +18,7 @@
 #include <thread>
 
 #include "pm_internal.hpp"
+#include "pm_spmv.hpp"
 
 namespace pm {
 
@@ -29,8 +30,11 @@ constexpr int MAX_SYS = 512;     // systems in one batch
 
 struct PcgSys {
   int64_t np;
-  const int32_t* rp;
-  const int32_t* cols;
+  const int32_t* urp;   // K as its upper node blocks, lower blocks of each row through their mirrors
+  const int32_t* ucol;
+  const int32_t* lrp;
+  const int32_t* lcol;
+  const int32_t* lidx;
   const void* vals;
   const double* freev;
   const double* binv;
@@ -80,18 +84,12 @@ __global__ __launch_bounds__(BTPB) void kb_q(const PcgSys* __restrict__ sys, con
   int si, w;
   unpack(map[blockIdx.x], si, w);
   const PcgSys& S = sys[si];
-  const VT* __restrict__ vals = static_cast<const VT*>(S.vals);
+  const KSym<VT> K{S.urp, S.ucol, S.lrp, S.lcol, S.lidx, static_cast<const VT*>(S.vals)};
   double acc = 0.0;
   for (int64_t t = (int64_t)w * BTPB + threadIdx.x; t < S.np * BS; t += (int64_t)S.nq * BTPB) {
     const int64_t a = t / BS;
     const int i = (int)(t - a * BS);
-    double s = 0.0;
-    for (int32_t k = S.rp[a]; k < S.rp[a + 1]; k++) {
-      const int64_t b = S.cols[k];
-      const VT* v = vals + (int64_t)k * BS * BS + i * BS;
-#pragma unroll
-      for (int j = 0; j < BS; j++) s += (double)v[j] * S.p[b * BS + j];
-    }
+    double s = ksym_row<BS, VT>(K, a, i, S.p);
     s *= S.freev[t];
     S.q[t] = s;
     acc += S.p[t] * s;
@@ -237,7 +235,7 @@ struct PcgEngine {
     job.lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
     job.max_it = (f->cfg.pcg_max_iter + CHUNK - 1) / CHUNK * CHUNK;
     PcgSys& S = job.desc;
-    S.np = np; S.rp = f->d_browptr; S.cols = f->d_bcols;
+    S.np = np; S.urp = f->d_urp; S.ucol = f->d_ucol; S.lrp = f->d_lrp; S.lcol = f->d_lcol; S.lidx = f->d_lidx;
     S.vals = job.lowp ? (const void*)f->d_kval32 : (const void*)f->d_kval;
     S.freev = f->d_free; S.binv = f->d_binv;
     S.r = f->d_pcg; S.z = S.r + n; S.p = S.z + n; S.q = S.p + n; S.x = d_x;

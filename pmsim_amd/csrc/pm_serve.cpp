@@ -39,7 +39,8 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   b += conn * 5;                                  // connectivity + ranks
   b += nip * (ns + 4) * 8 * 2;                    // IP state + backup
   b += kev * ts;                                  // element matrices (packed upper blocks)
-  b += blocks * (b2 * ts + 4 + 4);                 // K values in the tangent's precision (bound: nnzb <= element blocks), cols, map
+  b += blocks * (b2 * ts + 16);                   // K (upper blocks) in the tangent's precision, upper and lower index
+                                                  // lists and the gather map (bound: every count <= element blocks)
   b += nfe * (8 + 4);                             // element forces + R map
   b += (f->npoint + 1) * 4 * 3;                   // row pointers
   b += ndof * 8 * 16 + np * 8 * 8;                // vectors, PCG work, multipliers
