@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <type_traits>
 #include <vector>
 
@@ -779,6 +780,9 @@ int tune_tangent(pm_fem* f) {
       if (hipEventRecord(ev1, f->stream) != hipSuccess || hipEventSynchronize(ev1) != hipSuccess) { rc = 1; break; }
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, ev0, ev1);
+      if (f->cfg.verbose & 1)
+        std::fprintf(stderr, "tangent tuning: kind %d, %s kernel, %d elements per workgroup, %d IPs per chunk: %.3f ms\n", g.kind,
+                     c.kern == 2 ? "matrix-core" : "vector", c.epb, c.ch, ms / 3.0f);
       if (ms < best) { best = ms; pick = c; }
     }
     g.tune_kernel = pick.kern; g.tune_epb = pick.epb; g.tune_ch = pick.ch;
