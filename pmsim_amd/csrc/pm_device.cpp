@@ -77,12 +77,22 @@ int device_alloc(pm_fem* f) {
   if (dalloc(f, &f->d_lcol, nnzl) || upload(f, f->d_lcol, f->h_lcol.data(), nnzl)) return 1;
   if (dalloc(f, &f->d_lidx, nnzl) || upload(f, f->d_lidx, f->h_lidx.data(), nnzl)) return 1;
   if (dalloc(f, &f->d_kmap_ptr, f->nnzu + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzu + 1)) return 1;
+  // 3D: the long PCG solves stream a full-pattern copy (widened from the stored blocks at each solve);
+  // reading the lower blocks through their mirrors costs more there than the widening
+  if (f->ndim == 3) {
+    if (dalloc(f, &f->d_frp, f->npoint + 1) || upload(f, f->d_frp, f->h_browptr.data(), f->npoint + 1)) return 1;
+    if (dalloc(f, &f->d_fcol, f->nnzb) || upload(f, f->d_fcol, f->h_bcols.data(), f->nnzb)) return 1;
+    if (dalloc(f, &f->d_f2u, f->nnzb) || upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb)) return 1;
+  }
   if (dalloc(f, &f->d_korder, (int64_t)f->h_korder.size()) || upload(f, f->d_korder, f->h_korder.data(), (int64_t)f->h_korder.size())) return 1;
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
   if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
   // K values: FP64 for FP64 tangents; FP32 only for reduced-precision tangents (formed, stored and
   // multiplied in FP32; readers that need FP64 widen them)
+  if (f->ndim == 3) {
+    if (f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2)) return 1;
+  }
   if (f->cfg.tangent_precision == 0) {
     if (dalloc(f, &f->d_kval, f->nnzu * b2)) return 1;
     if (dalloc(f, &f->d_ke, f->n_ke_values)) return 1;
@@ -135,6 +145,8 @@ void device_free(pm_fem* f) {
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
   fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
   fr(f->d_arc); f->d_arc = nullptr;
+  fr(f->d_frp); fr(f->d_fcol); fr(f->d_f2u); fr(f->d_kfull); fr(f->d_kfull32);
+  f->d_frp = f->d_fcol = f->d_f2u = nullptr; f->d_kfull = nullptr; f->d_kfull32 = nullptr;
   if (f->stream) (void)hipStreamDestroy(f->stream);
   f->stream = nullptr;
   f->d_X = nullptr; f->d_urp = f->d_ucol = f->d_lrp = f->d_lcol = f->d_lidx = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;

@@ -150,6 +150,11 @@ struct pm_fem {
   int32_t* d_lrp = nullptr;            // lower blocks of each row
   int32_t* d_lcol = nullptr;
   int32_t* d_lidx = nullptr;
+  int32_t* d_frp = nullptr;            // 3D meshes: the full pattern and its map to the stored blocks, for
+  int32_t* d_fcol = nullptr;           // the full K that the multi-workgroup PCG widens at each solve
+  int32_t* d_f2u = nullptr;
+  double* d_kfull = nullptr;           // nnzb*bs*bs (FP64 tangents) or
+  float* d_kfull32 = nullptr;          // nnzb*bs*bs (FP32 / BF16 tangents)
   double* d_kval = nullptr;            // nnzu*bs*bs (FP64), FP64 tangents only
   float* d_kval32 = nullptr;           // nnzu*bs*bs (FP32), reduced-precision tangents only
   double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order, packed (n_ke_values)

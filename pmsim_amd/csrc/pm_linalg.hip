@@ -269,6 +269,51 @@ __device__ __forceinline__ void bjacobi_apply(int64_t a, const double* __restric
   }
 }
 
+// full-pattern K from the stored blocks: block k of the full pattern is the stored block f2u[k], read
+// transposed when the sign bit is set (the lower blocks)
+template <int BS, typename VT>
+__global__ __launch_bounds__(TPB) void k_widen(int64_t nnzb, const int32_t* __restrict__ f2u, const VT* __restrict__ up,
+                                               VT* __restrict__ full) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnzb) return;
+  const uint32_t m = (uint32_t)f2u[k];
+  const VT* s = up + (int64_t)(m & 0x7fffffffu) * BS * BS;
+  VT v[BS * BS];
+#pragma unroll
+  for (int q = 0; q < BS * BS; q++) v[q] = s[q];
+  VT* d = full + k * BS * BS;
+  const bool tr = (m & 0x80000000u) != 0;
+#pragma unroll
+  for (int i = 0; i < BS; i++)
+#pragma unroll
+    for (int j = 0; j < BS; j++) d[i * BS + j] = tr ? v[j * BS + i] : v[i * BS + j];
+}
+
+// K1 with the full-pattern copy: the same columns in the same order as ksym_row, so the same sums
+template <int BS, typename VT>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_q_full(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                        const VT* __restrict__ vals, const double* __restrict__ free_,
+                                                        const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
+  __shared__ double sh[PCG_TPB];
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * BS; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = t / BS;
+    const int i = (int)(t - a * BS);
+    double s = 0.0;
+    for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
+      const int64_t b = cols[k];
+      const VT* v = vals + (int64_t)k * BS * BS + i * BS;
+#pragma unroll
+      for (int j = 0; j < BS; j++) s += (double)v[j] * p[b * BS + j];
+    }
+    s *= free_[t];
+    q[t] = s;
+    acc += p[t] * s;
+  }
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // K1: q = (A p) * free ; partial p.q
 template <int BS, typename VT>
 __global__ __launch_bounds__(PCG_TPB) void k_pcg_q(int64_t np, KSym<VT> K, const double* __restrict__ free_,
@@ -528,6 +573,13 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
     return 2;
   }
   if (f->engine) return engine_solve(f->engine, f, d_b, d_x, iters);  // batched with the other served requests
+  // 3D: widen K to the full pattern once per solve, then stream rows of it (same values and column order)
+  const bool full = lowp ? f->d_kfull32 != nullptr : f->d_kfull != nullptr;
+  if (full) {
+    const unsigned gw = (unsigned)((f->nnzb + TPB - 1) / TPB);
+    if (lowp) hipLaunchKernelGGL((k_widen<3, float>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval32, f->d_kfull32);
+    else hipLaunchKernelGGL((k_widen<3, double>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval, f->d_kfull);
+  }
   // grids: rows for the SpMV (rbq), nodes for the vector updates (rb); partial arrays sized accordingly
   const int rb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + PCG_TPB - 1) / PCG_TPB));
   const int rbq = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + PCG_TPB - 1) / PCG_TPB));
@@ -556,7 +608,9 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
       else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     } else {
-      if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
+      if (full && lowp) hipLaunchKernelGGL((k_pcg_q_full<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa);
+      else if (full) hipLaunchKernelGGL((k_pcg_q_full<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa);
+      else if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
       else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     }
