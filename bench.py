@@ -303,6 +303,32 @@ def run_precision(a):
 M_NAMES = {0: "Tri3", 1: "Tri6", 2: "Tri10", 3: "Tri15", 4: "Qua4", 5: "Qua8", 6: "Qua9", 7: "Tet4", 8: "Tet10", 9: "Hex8", 10: "Hex20"}
 
 
+def run_pcg(a):
+    """Time of one PCG iteration on the Hex20 tangent: one element pass assembles K, then linear solves
+    with an unreachable tolerance run exactly --pcg-iters iterations each (the last one timed)."""
+    from pmsim_amd.solver import Fem, PmError
+    mesh, base, cfg, d = build_problem(a.nx, a.ny, a.nz, a.precision, a.kernel)
+    cfg.set_pcg(1e-30, a.pcg_iters)
+    f = Fem(d, cfg, keep_history=False)
+    f.set_ddu(increment(mesh))
+    f.backup_secondary_values()
+    f.element_pass()
+    b = np.random.default_rng(1).uniform(-1.0, 1.0, f.ndof)
+    times = []
+    for _ in range(max(1, a.warmup) + 1):
+        t0 = time.time()
+        try:
+            f.linear_solve(b)
+        except PmError:
+            pass  # stops at the iteration limit by design
+        times.append(time.time() - t0)
+    ms = times[-1] / a.pcg_iters * 1e3
+    print(json.dumps({"metric": "PCG ms per iteration (block Jacobi, K in the tangent's precision)", "value": ms,
+                      "unit": "ms", "n_gpus": 1, "steps": a.pcg_iters, "warmup": max(1, a.warmup),
+                      "higher_is_better": False, "dtype": a.precision, "data": "synthetic (Hex20 tangent after one pass)",
+                      "config": {"workload": f"Hex20 {a.nx}x{a.ny}x{a.nz}, {f.ndof} DOFs, PCG on the free DOFs"}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,7 +340,8 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision"])
+    ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision", "pcg"])
+    ap.add_argument("--pcg-iters", type=int, default=1600, help="pcg: PCG iterations per timed solve")
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
     ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first")
     ap.add_argument("--uz-stations", default="", help="precision: top uz after each increment (comma-separated) instead of --n-inc")
@@ -336,6 +363,8 @@ def main():
         return run_mixed2d(a)
     if a.workload == "precision":
         return run_precision(a)
+    if a.workload == "pcg":
+        return run_pcg(a)
     if a.workload == "serve":
         # more hardware queues than the default 4 so the concurrent requests' streams do not serialise
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.max_active)))
