@@ -289,26 +289,48 @@ __global__ __launch_bounds__(TPB) void k_widen(int64_t nnzb, const int32_t* __re
     for (int j = 0; j < BS; j++) d[i * BS + j] = tr ? v[j * BS + i] : v[i * BS + j];
 }
 
-// K1 with the full-pattern copy: the same columns in the same order as ksym_row, so the same sums
-template <int BS, typename VT>
-__global__ __launch_bounds__(PCG_TPB) void k_pcg_q_full(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+// K1 with the full-pattern copy: LPR lanes per block row, lane l taking the row's blocks l, l + LPR, ...
+// (consecutive lanes read consecutive blocks, so a row's run of blocks is read in whole cache lines),
+// then a butterfly sum over the group; p.q partials per workgroup as in k_pcg_q.
+template <int BS, typename VT, int LPR>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_q_rows(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
                                                         const VT* __restrict__ vals, const double* __restrict__ free_,
                                                         const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
+  static_assert(LPR >= 1 && LPR <= 64 && (LPR & (LPR - 1)) == 0, "lanes per row: a power of two within a wave");
   __shared__ double sh[PCG_TPB];
+  const int lane = (int)(threadIdx.x & (LPR - 1));
+  const int64_t ngroups = (int64_t)gridDim.x * blockDim.x / LPR;
   double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * BS; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t a = t / BS;
-    const int i = (int)(t - a * BS);
-    double s = 0.0;
-    for (int32_t k = rp[a]; k < rp[a + 1]; k++) {
-      const int64_t b = cols[k];
-      const VT* v = vals + (int64_t)k * BS * BS + i * BS;
+  // every lane of a group runs the same rows (a depends on the group only), so the group's lanes reach
+  // the shuffles together
+  for (int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR; a < np; a += ngroups) {
+    double y[BS];
 #pragma unroll
-      for (int j = 0; j < BS; j++) s += (double)v[j] * p[b * BS + j];
+    for (int i = 0; i < BS; i++) y[i] = 0.0;
+    for (int32_t k = rp[a] + lane; k < rp[a + 1]; k += LPR) {
+      const int64_t b = cols[k];
+      const VT* v = vals + (int64_t)k * BS * BS;
+      double pb[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) pb[j] = p[b * BS + j];
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) y[i] += (double)v[i * BS + j] * pb[j];
     }
-    s *= free_[t];
-    q[t] = s;
-    acc += p[t] * s;
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int i = 0; i < BS; i++) y[i] += __shfl_xor(y[i], off, LPR);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < BS; i++) {
+        const int64_t t = a * BS + i;
+        const double sv = y[i] * free_[t];
+        q[t] = sv;
+        acc += p[t] * sv;
+      }
+    }
   }
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -599,6 +621,18 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   if (!std::isfinite(bb)) { f->err = "Found NaN or Inf"; return 1; }
   const double tol2 = rtol2 * bb;
   const int check_every = 16;  // even, so a chunk starts on the same rz slot every time
+  // 3D full-pattern SpMV: lanes per block row from the mean row length (a power of two, 8 to 64)
+  const int64_t mean_row = (f->nnzb + np - 1) / std::max<int64_t>(np, 1);
+  const int lpr = mean_row > 32 ? 64 : mean_row > 16 ? 32 : mean_row > 8 ? 16 : 8;
+  auto launch_q_rows = [&](bool lp, int grid) {
+#define PM_QROWS(L) do { if (lp) hipLaunchKernelGGL((k_pcg_q_rows<3, float, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa); \
+                         else hipLaunchKernelGGL((k_pcg_q_rows<3, double, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa); } while (0)
+    if (lpr == 64) PM_QROWS(64);
+    else if (lpr == 32) PM_QROWS(32);
+    else if (lpr == 16) PM_QROWS(16);
+    else PM_QROWS(8);
+#undef PM_QROWS
+  };
   // one PCG iteration = 3 launches; iteration `it` reads rz from slot it&1 and writes slot (it+1)&1
   auto iterate = [&](int it) {
     const double* rz_old = rzs + (it & 1);
@@ -608,8 +642,7 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
       else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     } else {
-      if (full && lowp) hipLaunchKernelGGL((k_pcg_q_full<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa);
-      else if (full) hipLaunchKernelGGL((k_pcg_q_full<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa);
+      if (full) launch_q_rows(lowp, rbq);
       else if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
       else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
       hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
