@@ -77,22 +77,12 @@ int device_alloc(pm_fem* f) {
   if (dalloc(f, &f->d_lcol, nnzl) || upload(f, f->d_lcol, f->h_lcol.data(), nnzl)) return 1;
   if (dalloc(f, &f->d_lidx, nnzl) || upload(f, f->d_lidx, f->h_lidx.data(), nnzl)) return 1;
   if (dalloc(f, &f->d_kmap_ptr, f->nnzu + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzu + 1)) return 1;
-  // 3D: the long PCG solves stream a full-pattern copy (widened from the stored blocks at each solve);
-  // reading the lower blocks through their mirrors costs more there than the widening
-  if (f->ndim == 3) {
-    if (dalloc(f, &f->d_frp, f->npoint + 1) || upload(f, f->d_frp, f->h_browptr.data(), f->npoint + 1)) return 1;
-    if (dalloc(f, &f->d_fcol, f->nnzb) || upload(f, f->d_fcol, f->h_bcols.data(), f->nnzb)) return 1;
-    if (dalloc(f, &f->d_f2u, f->nnzb) || upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb)) return 1;
-  }
   if (dalloc(f, &f->d_korder, (int64_t)f->h_korder.size()) || upload(f, f->d_korder, f->h_korder.data(), (int64_t)f->h_korder.size())) return 1;
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
   if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
   // K values: FP64 for FP64 tangents; FP32 only for reduced-precision tangents (formed, stored and
   // multiplied in FP32; readers that need FP64 widen them)
-  if (f->ndim == 3) {
-    if (f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2)) return 1;
-  }
   if (f->cfg.tangent_precision == 0) {
     if (dalloc(f, &f->d_kval, f->nnzu * b2)) return 1;
     if (dalloc(f, &f->d_ke, f->n_ke_values)) return 1;
@@ -128,6 +118,19 @@ int device_alloc(pm_fem* f) {
   std::vector<int32_t>().swap(f->h_korder);
   std::vector<int32_t>().swap(f->h_fmap);
   std::vector<int32_t>().swap(f->h_fmap_ptr);
+  return 0;
+}
+
+// The multi-workgroup PCG streams a full-pattern copy of K, widened from the stored upper blocks at each
+// solve: reading each lower block through its mirror costs more over a long solve than one widening.
+// Allocated at the first such solve (systems solved by the single-workgroup PCG never need it).
+int full_pattern_alloc(pm_fem* f) {
+  if (f->d_frp) return 0;
+  const int64_t b2 = (int64_t)f->bs * f->bs;
+  if (dalloc(f, &f->d_frp, f->npoint + 1) || upload(f, f->d_frp, f->h_browptr.data(), f->npoint + 1)) return 1;
+  if (dalloc(f, &f->d_fcol, f->nnzb) || upload(f, f->d_fcol, f->h_bcols.data(), f->nnzb)) return 1;
+  if (dalloc(f, &f->d_f2u, f->nnzb) || upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb)) return 1;
+  if (f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2)) return 1;
   return 0;
 }
 

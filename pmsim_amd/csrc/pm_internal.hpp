@@ -150,8 +150,8 @@ struct pm_fem {
   int32_t* d_lrp = nullptr;            // lower blocks of each row
   int32_t* d_lcol = nullptr;
   int32_t* d_lidx = nullptr;
-  int32_t* d_frp = nullptr;            // 3D meshes: the full pattern and its map to the stored blocks, for
-  int32_t* d_fcol = nullptr;           // the full K that the multi-workgroup PCG widens at each solve
+  int32_t* d_frp = nullptr;            // the full pattern and its map to the stored blocks, for the full K
+  int32_t* d_fcol = nullptr;           // that the multi-workgroup PCG widens at each solve (full_pattern_alloc)
   int32_t* d_f2u = nullptr;
   double* d_kfull = nullptr;           // nnzb*bs*bs (FP64 tangents) or
   float* d_kfull32 = nullptr;          // nnzb*bs*bs (FP32 / BF16 tangents)
@@ -251,6 +251,7 @@ void dev_masked(pm_fem* f, const double* x, double* out);
 void dev_update_primary(pm_fem* f, const double* x, const double* y);
 void dev_fill(pm_fem* f, double* x, double v, int64_t n);
 int arc_alloc(pm_fem* f);
+int full_pattern_alloc(pm_fem* f);
 // linear solvers
 int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters);
 int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x);

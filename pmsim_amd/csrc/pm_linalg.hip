@@ -289,9 +289,8 @@ __global__ __launch_bounds__(TPB) void k_widen(int64_t nnzb, const int32_t* __re
     for (int j = 0; j < BS; j++) d[i * BS + j] = tr ? v[j * BS + i] : v[i * BS + j];
 }
 
-// K1 with the full-pattern copy: LPR lanes per block row, lane l taking the row's blocks l, l + LPR, ...
-// (consecutive lanes read consecutive blocks, so a row's run of blocks is read in whole cache lines),
-// then a butterfly sum over the group; p.q partials per workgroup as in k_pcg_q.
+// K1: q = (A p) * free over the full-pattern copy of K, LPR lanes per block row (full_row_lanes);
+// partial p.q per workgroup
 template <int BS, typename VT, int LPR>
 __global__ __launch_bounds__(PCG_TPB) void k_pcg_q_rows(int64_t np, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
                                                         const VT* __restrict__ vals, const double* __restrict__ free_,
@@ -301,27 +300,10 @@ __global__ __launch_bounds__(PCG_TPB) void k_pcg_q_rows(int64_t np, const int32_
   const int lane = (int)(threadIdx.x & (LPR - 1));
   const int64_t ngroups = (int64_t)gridDim.x * blockDim.x / LPR;
   double acc = 0.0;
-  // every lane of a group runs the same rows (a depends on the group only), so the group's lanes reach
-  // the shuffles together
+  // a depends on the group only: the lanes of a group run the same rows and reach the shuffles together
   for (int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR; a < np; a += ngroups) {
     double y[BS];
-#pragma unroll
-    for (int i = 0; i < BS; i++) y[i] = 0.0;
-    for (int32_t k = rp[a] + lane; k < rp[a + 1]; k += LPR) {
-      const int64_t b = cols[k];
-      const VT* v = vals + (int64_t)k * BS * BS;
-      double pb[BS];
-#pragma unroll
-      for (int j = 0; j < BS; j++) pb[j] = p[b * BS + j];
-#pragma unroll
-      for (int i = 0; i < BS; i++)
-#pragma unroll
-        for (int j = 0; j < BS; j++) y[i] += (double)v[i * BS + j] * pb[j];
-    }
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int i = 0; i < BS; i++) y[i] += __shfl_xor(y[i], off, LPR);
+    full_row_lanes<BS, VT>(rp, cols, vals, a, lane, LPR, p, y);
     if (lane == 0) {
 #pragma unroll
       for (int i = 0; i < BS; i++) {
@@ -331,24 +313,6 @@ __global__ __launch_bounds__(PCG_TPB) void k_pcg_q_rows(int64_t np, const int32_
         acc += p[t] * sv;
       }
     }
-  }
-  const double r = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
-// K1: q = (A p) * free ; partial p.q
-template <int BS, typename VT>
-__global__ __launch_bounds__(PCG_TPB) void k_pcg_q(int64_t np, KSym<VT> K, const double* __restrict__ free_,
-                                                   const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
-  __shared__ double sh[PCG_TPB];
-  double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * BS; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t a = t / BS;
-    const int i = (int)(t - a * BS);
-    double s = ksym_row<BS, VT>(K, a, i, p);
-    s *= free_[t];
-    q[t] = s;
-    acc += p[t] * s;
   }
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -594,17 +558,22 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
     f->err = "PCG did not converge";
     return 2;
   }
-  if (f->engine) return engine_solve(f->engine, f, d_b, d_x, iters);  // batched with the other served requests
-  // 3D: widen K to the full pattern once per solve, then stream rows of it (same values and column order)
-  const bool full = lowp ? f->d_kfull32 != nullptr : f->d_kfull != nullptr;
-  if (full) {
+  // K widened to the full pattern once per solve; the iterations stream rows of it (same values and
+  // column order as the stored blocks give)
+  if (full_pattern_alloc(f)) return 1;
+  {
     const unsigned gw = (unsigned)((f->nnzb + TPB - 1) / TPB);
-    if (lowp) hipLaunchKernelGGL((k_widen<3, float>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval32, f->d_kfull32);
-    else hipLaunchKernelGGL((k_widen<3, double>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval, f->d_kfull);
+#define PM_WIDEN(BS) do { if (lowp) hipLaunchKernelGGL((k_widen<BS, float>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval32, f->d_kfull32); \
+                          else hipLaunchKernelGGL((k_widen<BS, double>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval, f->d_kfull); } while (0)
+    if (f->bs == 2) PM_WIDEN(2);
+    else PM_WIDEN(3);
+#undef PM_WIDEN
   }
+  if (f->engine) return engine_solve(f->engine, f, d_b, d_x, iters);  // batched with the other served requests
   // grids: rows for the SpMV (rbq), nodes for the vector updates (rb); partial arrays sized accordingly
   const int rb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + PCG_TPB - 1) / PCG_TPB));
-  const int rbq = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + PCG_TPB - 1) / PCG_TPB));
+  const int lpr = lanes_per_row(f->nnzb, np);
+  const int rbq = spmv_workgroups(np, lpr, PCG_TPB);
   double* pa = f->d_red;            // partials: pa (rbq), pb (rb), pc (rb)
   double* pb = pa + rbq;
   double* pc = pb + rb;
@@ -621,32 +590,22 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   if (!std::isfinite(bb)) { f->err = "Found NaN or Inf"; return 1; }
   const double tol2 = rtol2 * bb;
   const int check_every = 16;  // even, so a chunk starts on the same rz slot every time
-  // 3D full-pattern SpMV: lanes per block row from the mean row length (a power of two, 8 to 64)
-  const int64_t mean_row = (f->nnzb + np - 1) / std::max<int64_t>(np, 1);
-  const int lpr = mean_row > 32 ? 64 : mean_row > 16 ? 32 : mean_row > 8 ? 16 : 8;
-  auto launch_q_rows = [&](bool lp, int grid) {
-#define PM_QROWS(L) do { if (lp) hipLaunchKernelGGL((k_pcg_q_rows<3, float, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa); \
-                         else hipLaunchKernelGGL((k_pcg_q_rows<3, double, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa); } while (0)
-    if (lpr == 64) PM_QROWS(64);
-    else if (lpr == 32) PM_QROWS(32);
-    else if (lpr == 16) PM_QROWS(16);
-    else PM_QROWS(8);
+  auto launch_q_rows = [&](int grid) {
+#define PM_QROWS(BS, L) do { if (lowp) hipLaunchKernelGGL((k_pcg_q_rows<BS, float, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa); \
+                             else hipLaunchKernelGGL((k_pcg_q_rows<BS, double, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa); } while (0)
+#define PM_QROWS_L(BS) do { if (lpr == 64) PM_QROWS(BS, 64); else if (lpr == 32) PM_QROWS(BS, 32); else if (lpr == 16) PM_QROWS(BS, 16); else PM_QROWS(BS, 8); } while (0)
+    if (f->bs == 2) PM_QROWS_L(2);
+    else PM_QROWS_L(3);
+#undef PM_QROWS_L
 #undef PM_QROWS
   };
   // one PCG iteration = 3 launches; iteration `it` reads rz from slot it&1 and writes slot (it+1)&1
   auto iterate = [&](int it) {
     const double* rz_old = rzs + (it & 1);
     double* rz_new = rzs + ((it + 1) & 1);
-    if (f->bs == 2) {
-      if (lowp) hipLaunchKernelGGL((k_pcg_q<2, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
-      else hipLaunchKernelGGL((k_pcg_q<2, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
-      hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
-    } else {
-      if (full) launch_q_rows(lowp, rbq);
-      else if (lowp) hipLaunchKernelGGL((k_pcg_q<3, float>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa);
-      else hipLaunchKernelGGL((k_pcg_q<3, double>), dim3(rbq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa);
-      hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
-    }
+    launch_q_rows(rbq);
+    if (f->bs == 2) hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
+    else hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rbq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
     hipLaunchKernelGGL(k_pcg_p2, dim3(rb), dim3(PCG_TPB), 0, f->stream, n, rb, pb, pc, rz_old, rz_new, rr_s, z, p);
   };
   // a whole chunk of iterations between two convergence checks is captured once per handle (and

@@ -1,8 +1,8 @@
 // Batched PCG for serving: one engine per pm_serve call runs the block-Jacobi PCG iterations of every
 // served request that is currently in its linear solve as the same three kernel launches, instead of
 // three launches per request per iteration.  Each request keeps its own matrix, vectors and partial-sum
-// buffers; inside the batched kernels every request sees exactly the work split (workgroups, rows per
-// thread, partial-sum order) of the single-request multi-workgroup PCG, so its iterates are bitwise the
+// buffers; inside the batched kernels every request sees exactly the work split (workgroups, lanes per
+// row, partial-sum order) of the single-request multi-workgroup PCG, so its iterates are bitwise the
 // same whether it is solved alone or in a batch, and the same as pm_solve's.
 //
 // Requests hand a solve to the engine from their own worker thread (after synchronizing their stream)
@@ -30,12 +30,10 @@ constexpr int MAX_SYS = 512;     // systems in one batch
 
 struct PcgSys {
   int64_t np;
-  const int32_t* urp;   // K as its upper node blocks, lower blocks of each row through their mirrors
-  const int32_t* ucol;
-  const int32_t* lrp;
-  const int32_t* lcol;
-  const int32_t* lidx;
-  const void* vals;
+  const int32_t* frp;   // full-pattern copy of K (widened by the request before it hands the solve over)
+  const int32_t* fcol;
+  const void* kfull;
+  int lpr;              // lanes per block row of the SpMV
   const double* freev;
   const double* binv;
   double *x, *r, *z, *p, *q;
@@ -84,15 +82,22 @@ __global__ __launch_bounds__(BTPB) void kb_q(const PcgSys* __restrict__ sys, con
   int si, w;
   unpack(map[blockIdx.x], si, w);
   const PcgSys& S = sys[si];
-  const KSym<VT> K{S.urp, S.ucol, S.lrp, S.lcol, S.lidx, static_cast<const VT*>(S.vals)};
+  const int lpr = S.lpr;
+  const int lane = (int)(threadIdx.x & (lpr - 1));
+  const int64_t ngroups = (int64_t)S.nq * BTPB / lpr;
   double acc = 0.0;
-  for (int64_t t = (int64_t)w * BTPB + threadIdx.x; t < S.np * BS; t += (int64_t)S.nq * BTPB) {
-    const int64_t a = t / BS;
-    const int i = (int)(t - a * BS);
-    double s = ksym_row<BS, VT>(K, a, i, S.p);
-    s *= S.freev[t];
-    S.q[t] = s;
-    acc += S.p[t] * s;
+  for (int64_t a = ((int64_t)w * BTPB + threadIdx.x) / lpr; a < S.np; a += ngroups) {
+    double y[BS];
+    full_row_lanes<BS, VT>(S.frp, S.fcol, static_cast<const VT*>(S.kfull), a, lane, lpr, S.p, y);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < BS; i++) {
+        const int64_t t = a * BS + i;
+        const double sv = y[i] * S.freev[t];
+        S.q[t] = sv;
+        acc += S.p[t] * sv;
+      }
+    }
   }
   const double r = bsum(acc, sh);
   if (threadIdx.x == 0) S.pa[w] = r;
@@ -235,12 +240,13 @@ struct PcgEngine {
     job.lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
     job.max_it = (f->cfg.pcg_max_iter + CHUNK - 1) / CHUNK * CHUNK;
     PcgSys& S = job.desc;
-    S.np = np; S.urp = f->d_urp; S.ucol = f->d_ucol; S.lrp = f->d_lrp; S.lcol = f->d_lcol; S.lidx = f->d_lidx;
-    S.vals = job.lowp ? (const void*)f->d_kval32 : (const void*)f->d_kval;
+    S.np = np; S.frp = f->d_frp; S.fcol = f->d_fcol;
+    S.kfull = job.lowp ? (const void*)f->d_kfull32 : (const void*)f->d_kfull;
+    S.lpr = lanes_per_row(f->nnzb, np);
     S.freev = f->d_free; S.binv = f->d_binv;
     S.r = f->d_pcg; S.z = S.r + n; S.p = S.z + n; S.q = S.p + n; S.x = d_x;
     S.nv = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + BTPB - 1) / BTPB));
-    S.nq = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + BTPB - 1) / BTPB));
+    S.nq = spmv_workgroups(np, S.lpr, BTPB);
     S.pa = f->d_red; S.pb = S.pa + S.nq; S.pc = S.pb + S.nv;
     S.rzs = f->d_scal + 16; S.rr = f->d_scal + 18;
     if (hipStreamSynchronize(f->stream) != hipSuccess) { f->err = "stream synchronization failed"; return 1; }

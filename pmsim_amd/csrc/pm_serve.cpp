@@ -45,7 +45,7 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   b += (f->npoint + 1) * 4 * 3;                   // row pointers
   b += ndof * 8 * 16 + np * 8 * 8;                // vectors, PCG work, multipliers
   if (f->cfg.arc_length_method) b += (4 * ndof + 4 * np) * 8;  // arc-length vectors
-  if (nd == 3) b += blocks * (b2 * ts + 8) + (f->npoint + 1) * 4;  // full-pattern K copy for the PCG
+  if (ndof > f->pcg_small_max) b += blocks * (b2 * ts + 8) + (f->npoint + 1) * 4;  // full-pattern K copy (multi-workgroup PCG)
   b += f->npoint * b2 * 8;                        // block-Jacobi inverses
   b += 1 << 20;                                   // reductions, scalars, shape tables
   *bytes = b;

@@ -41,4 +41,46 @@ __device__ __forceinline__ double ksym_row(const KSym<VT>& K, int64_t a, int i, 
   return s;
 }
 
+// Row a of K x over the full-pattern copy of K (row pointers rp, columns cols, BS x BS row-major blocks),
+// taken by a group of lpr lanes (lpr a power of two <= 64, the group aligned in the wave): lane l
+// multiplies the row's blocks l, l + lpr, ... (consecutive lanes read consecutive blocks), then a
+// butterfly over the group leaves the row's sums y in every lane of the group.  Every lane of the group
+// must call it for the same row.  The same arithmetic whatever the caller, so the single-request and the
+// batched PCG give the same bits.
+template <int BS, typename VT>
+__device__ __forceinline__ void full_row_lanes(const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                               const VT* __restrict__ vals, int64_t a, int lane, int lpr,
+                                               const double* __restrict__ x, double* y) {
+#pragma unroll
+  for (int i = 0; i < BS; i++) y[i] = 0.0;
+  for (int32_t k = rp[a] + lane; k < rp[a + 1]; k += lpr) {
+    const int64_t b = cols[k];
+    const VT* v = vals + (int64_t)k * BS * BS;
+    double xb[BS];
+#pragma unroll
+    for (int j = 0; j < BS; j++) xb[j] = x[b * BS + j];
+#pragma unroll
+    for (int i = 0; i < BS; i++)
+#pragma unroll
+      for (int j = 0; j < BS; j++) y[i] += (double)v[i * BS + j] * xb[j];
+  }
+  // lane ^ off stays inside the aligned group for off < lpr
+  for (int off = lpr >> 1; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < BS; i++) y[i] += __shfl_xor(y[i], off);
+}
+
+// lanes per block row for full_row_lanes, from the mean row length: a power of two, 8 to 64
+inline int lanes_per_row(int64_t nnzb, int64_t np) {
+  const int64_t mean_row = (nnzb + np - 1) / (np > 0 ? np : 1);
+  return mean_row > 32 ? 64 : mean_row > 16 ? 32 : mean_row > 8 ? 16 : 8;
+}
+
+// workgroups of tpb threads for an SpMV of np block rows at lpr lanes per row: one row per lane group
+// where that fits in 4096 workgroups (short latency chains on small systems), else a grid-stride loop
+inline int spmv_workgroups(int64_t np, int lpr, int tpb) {
+  const int64_t w = (np * lpr + tpb - 1) / tpb;
+  return (int)(w < 1 ? 1 : w > 4096 ? 4096 : w);
+}
+
 }  // namespace pm
