@@ -125,13 +125,21 @@ int device_alloc(pm_fem* f) {
 // solve: reading each lower block through its mirror costs more over a long solve than one widening.
 // Allocated at the first such solve (systems solved by the single-workgroup PCG never need it).
 int full_pattern_alloc(pm_fem* f) {
-  if (f->d_frp) return 0;
+  if (f->d_kfull || f->d_kfull32) return 0;  // allocated last: set only once every piece is in place
   const int64_t b2 = (int64_t)f->bs * f->bs;
-  if (dalloc(f, &f->d_frp, f->npoint + 1) || upload(f, f->d_frp, f->h_browptr.data(), f->npoint + 1)) return 1;
-  if (dalloc(f, &f->d_fcol, f->nnzb) || upload(f, f->d_fcol, f->h_bcols.data(), f->nnzb)) return 1;
-  if (dalloc(f, &f->d_f2u, f->nnzb) || upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb)) return 1;
-  if (f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2)) return 1;
-  return 0;
+  const bool ok = !dalloc(f, &f->d_frp, f->npoint + 1) && !upload(f, f->d_frp, f->h_browptr.data(), f->npoint + 1) &&
+                  !dalloc(f, &f->d_fcol, f->nnzb) && !upload(f, f->d_fcol, f->h_bcols.data(), f->nnzb) &&
+                  !dalloc(f, &f->d_f2u, f->nnzb) && !upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb) &&
+                  !(f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2));
+  if (ok) return 0;
+  // a failed allocation leaves nothing behind, so a later solve tries again from scratch
+  (void)hipStreamSynchronize(f->stream);  // uploads into these buffers end before they are freed
+  for (void* q : {(void*)f->d_frp, (void*)f->d_fcol, (void*)f->d_f2u, (void*)f->d_kfull, (void*)f->d_kfull32})
+    if (q) (void)hipFree(q);
+  f->d_frp = f->d_fcol = f->d_f2u = nullptr;
+  f->d_kfull = nullptr;
+  f->d_kfull32 = nullptr;
+  return 1;
 }
 
 int arc_alloc(pm_fem* f) {
