@@ -101,19 +101,26 @@ def test_batched_pcg_matches_standalone_solve():
         assert np.array_equal(f.get_ip_state(), h.get_ip_state())
 
 
-@pytest.mark.parametrize("name", ["footing_f64", "footing_f32", "footing_bf16", "hex20_f64", "arc_lmm"])
+@pytest.mark.parametrize("name", ["footing_f64", "footing_f32", "footing_bf16", "hex20_f64", "arc_lmm",
+                                  "footing_f64_multi", "footing_f32_multi"])
 def test_footprint_estimate_bounds_the_allocation(name):
     """Admission relies on pm_estimate_device_bytes being an upper bound of what the handle allocates
-    (including the arc-length vectors allocated at the first arc-length solve)."""
+    (including the arc-length vectors allocated at the first arc-length solve, and the full-pattern K
+    copy allocated at the first multi-workgroup PCG solve: the *_multi cases force that PCG)."""
     import cases
     case = {"footing_f64": cases.footing_c1(n_inc=2), "footing_f32": cases.footing_c1(n_inc=2),
             "footing_bf16": cases.footing_c1(n_inc=2), "hex20_f64": cases.hex20_cube(3, 3, 2, n_inc=2),
-            "arc_lmm": cases.footing_arc(lmm=True, n_inc=3)}[name]
+            "arc_lmm": cases.footing_arc(lmm=True, n_inc=3), "footing_f64_multi": cases.footing_c1(n_inc=2),
+            "footing_f32_multi": cases.footing_c1(n_inc=2)}[name]
     mesh, base, cfg, ess, nat = case
     cfg.set_tangent_precision(name.split("_")[1] if name.startswith("footing") else "f64")
+    if name.endswith("_multi"):
+        cfg.set_pcg_small_max(0)
     f = Fem(describe(mesh, base, cfg, ess, nat), cfg, keep_history=False, initialize=False)
     est = f.estimate_device_bytes()
     f.initialize()
     f.solve()
     used = f.stats()["device_bytes"]
     assert 0 < used <= est, (used, est)
+    if name.endswith("_multi"):
+        assert f.stats()["n_pcg_iterations"] > 0
