@@ -593,7 +593,7 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   auto launch_q_rows = [&](int grid) {
 #define PM_QROWS(BS, L) do { if (lowp) hipLaunchKernelGGL((k_pcg_q_rows<BS, float, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull32, f->d_free, p, q, pa); \
                              else hipLaunchKernelGGL((k_pcg_q_rows<BS, double, L>), dim3(grid), dim3(PCG_TPB), 0, f->stream, np, f->d_frp, f->d_fcol, f->d_kfull, f->d_free, p, q, pa); } while (0)
-#define PM_QROWS_L(BS) do { if (lpr == 64) PM_QROWS(BS, 64); else if (lpr == 32) PM_QROWS(BS, 32); else PM_QROWS(BS, 8); } while (0)
+#define PM_QROWS_L(BS) do { if (lpr == 32) PM_QROWS(BS, 32); else PM_QROWS(BS, 8); } while (0)
     if (f->bs == 2) PM_QROWS_L(2);
     else PM_QROWS_L(3);
 #undef PM_QROWS_L

@@ -70,12 +70,12 @@ __device__ __forceinline__ void full_row_lanes(const int32_t* __restrict__ rp, c
     for (int i = 0; i < BS; i++) y[i] += __shfl_xor(y[i], off);
 }
 
-// lanes per block row for full_row_lanes, from the mean row length: 64 for Hex20 (57 blocks per row),
-// 8 for 2D quadratic meshes (about 16: two blocks per lane measured faster than one, with a shorter
-// butterfly, on small and large systems alike)
+// lanes per block row for full_row_lanes, from the mean row length: 32 for Hex20 (57 blocks per row; 64
+// lanes, one block each, measured 8 % slower: a longer butterfly), 8 for 2D quadratic meshes (about 16:
+// two blocks per lane measured faster than one on small and large systems alike)
 inline int lanes_per_row(int64_t nnzb, int64_t np) {
   const int64_t mean_row = (nnzb + np - 1) / (np > 0 ? np : 1);
-  return mean_row > 32 ? 64 : mean_row > 16 ? 32 : 8;
+  return mean_row > 16 ? 32 : 8;
 }
 
 // workgroups of tpb threads for an SpMV of np block rows at lpr lanes per row: one row per lane group
