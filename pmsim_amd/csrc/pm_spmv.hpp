@@ -70,6 +70,50 @@ __device__ __forceinline__ void full_row_lanes(const int32_t* __restrict__ rp, c
     for (int i = 0; i < BS; i++) y[i] += __shfl_xor(y[i], off);
 }
 
+// full_row_lanes for two rows a0 and a1 at once (twice the loads in flight); each row's sums are formed
+// exactly as full_row_lanes forms them
+template <int BS, typename VT>
+__device__ __forceinline__ void full_row_lanes2(const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                const VT* __restrict__ vals, int64_t a0, int64_t a1, int lane, int lpr,
+                                                const double* __restrict__ x, double* y0, double* y1) {
+#pragma unroll
+  for (int i = 0; i < BS; i++) { y0[i] = 0.0; y1[i] = 0.0; }
+  int32_t k0 = rp[a0] + lane, k1 = rp[a1] + lane;
+  const int32_t e0 = rp[a0 + 1], e1 = rp[a1 + 1];
+  while (k0 < e0 || k1 < e1) {
+    if (k0 < e0) {
+      const int64_t b = cols[k0];
+      const VT* v = vals + (int64_t)k0 * BS * BS;
+      double xb[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) xb[j] = x[b * BS + j];
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) y0[i] += (double)v[i * BS + j] * xb[j];
+      k0 += lpr;
+    }
+    if (k1 < e1) {
+      const int64_t b = cols[k1];
+      const VT* v = vals + (int64_t)k1 * BS * BS;
+      double xb[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) xb[j] = x[b * BS + j];
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) y1[i] += (double)v[i * BS + j] * xb[j];
+      k1 += lpr;
+    }
+  }
+  for (int off = lpr >> 1; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < BS; i++) {
+      y0[i] += __shfl_xor(y0[i], off);
+      y1[i] += __shfl_xor(y1[i], off);
+    }
+}
+
 // lanes per block row for full_row_lanes, from the mean row length: 32 for Hex20 (57 blocks per row; 64
 // lanes, one block each, measured 8 % slower: a longer butterfly), 8 for 2D quadratic meshes (about 16:
 // two blocks per lane measured faster than one on small and large systems alike)
