@@ -124,3 +124,28 @@ def test_footprint_estimate_bounds_the_allocation(name):
     assert 0 < used <= est, (used, est)
     if name.endswith("_multi"):
         assert f.stats()["n_pcg_iterations"] > 0
+
+
+def test_batched_pcg_matches_standalone_solve_hex20():
+    """The same on Hex20 meshes, where the single-request SpMV takes two rows per lane-group pass and the
+    batched one a row at a time: each row's sums and the p.q order are the same, so are the bits."""
+    import cases
+    hs, built = [], []
+    # 20x20x20 has 35,721 nodes: more block rows than the SpMV's 32,768 lane groups, so its groups pair rows
+    for k, (nx, ny, nz) in enumerate([(4, 4, 3), (20, 20, 20)]):
+        # 0.1 % compression, below first yield: a few Newton iterations (the bits, not plasticity, are tested)
+        mesh, base, cfg, ess, nat = cases.hex20_cube(nx, ny, nz, n_inc=2, seed=100 + k, uz=-0.001)
+        cfg.set_pcg_small_max(0)  # every system through the multi-workgroup PCG
+        d = describe(mesh, base, cfg, ess, nat)
+        built.append((d, cfg))
+        hs.append(Fem(d, cfg, keep_history=False, initialize=False))
+    res = serve(hs, np.zeros(len(hs)), max_active=2)
+    assert all(r["status"] == 0 for r in res), [r.get("error") for r in res]
+    assert hs[1].ndof == 3 * 35721
+    for (d, cfg), h, r in zip(built, hs, res):
+        f = Fem(d, cfg, keep_history=False)
+        f.solve()
+        assert r["n_pcg_iterations"] > 0
+        assert f.stats()["n_pcg_iterations"] == r["n_pcg_iterations"]
+        assert np.array_equal(f.get_u()[0], h.get_u()[0])
+        assert np.array_equal(f.get_ip_state(), h.get_ip_state())
