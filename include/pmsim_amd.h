@@ -180,8 +180,9 @@ typedef struct pm_request_stats {
   int64_t device_bytes;             /* footprint estimate used for admission */
 } pm_request_stats;
 
-/* Upper bound of the device memory pm_initialize will allocate for this handle (mesh, batches, state,
- * element matrices, K pattern bound, vectors). */
+/* Upper bound of the device memory this handle allocates: at pm_initialize (mesh, batches, state,
+ * element matrices, K pattern bound, vectors) and later in pm_solve (arc-length vectors, the
+ * full-pattern K copy of the multi-workgroup PCG). */
 int pm_estimate_device_bytes(const pm_fem* fem, int64_t* bytes);
 /* Copies u, lambda and the IP state to host memory and frees every device buffer of the handle; the
  * getters pm_get_u / pm_get_ip_state / pm_get_history / pm_get_stats keep working from the copies. */
