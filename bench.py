@@ -8,7 +8,12 @@ sigma, z, lambda, loading flag, epbar), B^T sigma into the element forces and th
 gather into R, B^T D B node blocks and their deterministic gather into the node-block CSR tangent.
 The increment is a seeded field large enough that part of the IPs return plastically.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 64 --ny 64 --nz 32] [--no-cpu]
+The same line carries `served`: the served half of the headline metric on a bounded seeded stream of
+parametric strip-footing solves (Poisson arrivals near the service rate), as solves/s and p50/p99 latency
+for the plain FP64 GPU form (one request at a time), the served FP64 form and the served mixed-precision
+form (FP32 tangents), with the largest relative displacement difference of each against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 64 --ny 64 --nz 32] [--no-cpu] [--no-serve]
 """
 import argparse
 import json
@@ -23,11 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense FP64 / FP32 / BF16 matrix peaks
-FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520
-# what the closed-form tangent executes per Hex20 IP: 210 upper node blocks x 30 FMA (three rank-1
-# updates and the diagonal dot product), the per-tile scalings (55 tiles x 18) and the per-IP set-up
-# (J, J^-1, D coefficients, node vectors: about 1,200)
-EXECUTED_FLOP_PER_IP = 210 * 30 * 2 + 55 * 18 + 1200
+FLOP_PER_IP = 2 * 60 * 60 * 6 + 2 * 6 * 6 * 60  # dense B^T D B per Hex20 IP = 47,520 (not what the kernel executes)
 
 
 def build_problem(nx, ny, nz, precision="f64", kernel="default"):
@@ -50,21 +51,39 @@ def increment(mesh, seed=7):
     return u.ravel()
 
 
-def measured_traffic(nx, ny, nz, precision, kernel):
-    """HBM bytes per launch of the tangent kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary of this workload (scaled by the copy-kernel calibration stored with it), or None."""
+def _profile(nx, ny, nz, precision, kernel, what):
+    """A committed rocprofv3 summary of the tangent kernel on this workload (JSON beside its .md), or None."""
     if (nx, ny, nz) != (64, 64, 32):
-        return None, None
+        return None
     tag = {("f64", "default"): "f64_vector", ("f64", "vector"): "f64_vector", ("f64", "mfma"): "f64_mfma",
+           ("f32", "default"): "f32_vector", ("f32", "vector"): "f32_vector", ("f32", "mfma"): "f32_mfma",
            ("bf16", "default"): "bf16_mfma", ("bf16", "mfma"): "bf16_mfma"}.get((precision, kernel))
-    path = os.path.join(ROOT, "profiles", f"element_pass_hex20_64x64x32_{tag}.json")
+    path = os.path.join(ROOT, "profiles", f"{what}_hex20_64x64x32_{tag}.json")
     if tag is None or not os.path.exists(path):
+        return None
+    return json.load(open(path))
+
+
+def measured_traffic(nx, ny, nz, precision, kernel):
+    """HBM bytes per launch of the tangent kernel from the committed FETCH_SIZE / WRITE_SIZE summary of this
+    workload (scaled by the copy-kernel calibration stored with it), or None."""
+    prof = _profile(nx, ny, nz, precision, kernel, "element_pass")
+    if prof is None:
         return None, None
-    prof = json.load(open(path))
     for name, k in prof["kernels"].items():
         if name.startswith("k_tangent"):
             return k["read_bytes"] + k["write_bytes"], name
     return None, None
+
+
+def measured_ops(nx, ny, nz, precision, kernel):
+    """Operations the tangent kernel executes per launch, from the committed SQ counter summary of this
+    workload: FP64 VALU flops (SQ_INSTS_VALU_FLOPS_FP64: an FMA counts 2, per active lane) for the vector
+    kernel, MFMA operations (SQ_INSTS_VALU_MFMA_MOPS_* x 512) for the matrix-core kernels.  None if absent."""
+    prof = _profile(nx, ny, nz, precision, kernel, "k_tangent_counters")
+    if prof is None:
+        return None
+    return prof
 
 
 def _host_cpu_model():
@@ -85,35 +104,58 @@ def _host_threads():
     return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else n
 
 
-def cpu_baseline(seconds=12.0, seconds_all=6.0):
-    """Oracle element pass (update + f_int + K per element) over a bounded sample: on one host core (the
-    reported value) and, as `all_cores`, with its OpenMP element loops on every core this process may use."""
+def cpu_baseline(seconds=12.0, seconds_all=6.0, n=(32, 32, 16)):
+    """Oracle element pass (update + f_int + K per element) over a bounded sample of the same workload: on one
+    host core (the reported value) and, as `all_cores`, with its OpenMP element loops on every core this process
+    may use.  The sample (32x32x16 Hex20: 16,384 elements, 442,368 IPs, 35 MB of IP state and its backup) is
+    larger than one CCD's L3; the rate does not depend on the sample size (compute-bound).  `full_solve` times
+    the 16x8 Qua8 strip footing (20 increments) in the oracle beside the GPU solve of the same problem."""
     from oracle import oracle as orc
-    mesh, base, cfg, d = build_problem(8, 8, 4)
+    mesh, base, cfg, d = build_problem(*n)
     o = orc.Oracle(d)
     o.init_state()
     ddu = increment(mesh)
     t1 = o.time_element_pass(ddu, reps=1, nthreads=1)
     reps = max(1, int(seconds / max(t1, 1e-6)))
-    t = o.time_element_pass(ddu, reps=reps, nthreads=1)
+    t = o.time_element_pass(ddu, reps=reps, nthreads=1) if reps > 1 else t1
     nip = mesh.ncell * 27
+    tag = f"{n[0]}x{n[1]}x{n[2]} Hex20 ({mesh.ncell} elements, {nip} IPs)"
     out = dict(value=nip * reps / t, unit="IP-updates/s", cores=1, kind="port",
-               sample=f"8x8x4 Hex20 (256 elements, {nip} IPs) x {reps} passes, serial C++ oracle (update+f_int+K)",
-               host=_host_cpu_model())
+               sample=f"{tag} x {reps} passes, serial C++ oracle (update+f_int+K)", host=_host_cpu_model())
     nt = _host_threads()
     if nt > 1:
-        mesh2, _, _, d2 = build_problem(16, 16, 8)
-        o2 = orc.Oracle(d2)
-        o2.init_state()
-        ddu2 = increment(mesh2)
-        t1 = o2.time_element_pass(ddu2, reps=1, nthreads=nt)
+        t1 = o.time_element_pass(ddu, reps=1, nthreads=nt)
         reps2 = max(1, int(seconds_all / max(t1, 1e-6)))
-        t2 = o2.time_element_pass(ddu2, reps=reps2, nthreads=nt)
-        nip2 = mesh2.ncell * 27
-        out["all_cores"] = dict(value=nip2 * reps2 / t2, unit="IP-updates/s", cores=nt,
-                                sample=f"16x16x8 Hex20 ({mesh2.ncell} elements, {nip2} IPs) x {reps2} passes, "
-                                       f"C++ oracle with OpenMP element loops on {nt} threads")
+        t2 = o.time_element_pass(ddu, reps=reps2, nthreads=nt)
+        out["all_cores"] = dict(value=nip * reps2 / t2, unit="IP-updates/s", cores=nt,
+                                sample=f"{tag} x {reps2} passes, C++ oracle with OpenMP element loops on {nt} threads")
+    out["full_solve"] = full_solve_baseline()
     return out
+
+
+def full_solve_baseline():
+    """The 16x8 Qua8 strip footing on von Mises material (20 increments, LMM, Newton with the consistent tangent):
+    the serial oracle (banded LU on the literal LMM system) against the GPU solve (PCG) of the same problem."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from oracle import oracle as orc
+    from pmsim_amd import describe
+    from pmsim_amd.solver import Fem
+    mesh, base, cfg, ess, nat = cases.footing_c1()
+    d = describe(mesh, base, cfg, ess, nat)
+    t0 = time.time()
+    uo, _, its = orc.Oracle(d).solve()
+    cpu_s = time.time() - t0
+    f = Fem(d, cfg, keep_history=False)
+    f.solve()  # warm-up (code objects, graphs)
+    g = Fem(d, cfg, keep_history=False)
+    t0 = time.time()
+    g.solve()
+    gpu_s = time.time() - t0
+    u = g.get_u()[0]
+    return {"workload": "16x8 Qua8 strip footing, 3x3 IP, von Mises, 20 increments", "cpu_s": cpu_s, "cpu_cores": 1,
+            "gpu_s": gpu_s, "newton_cpu": int(np.abs(its).sum()), "newton_gpu": g.stats()["n_iterations"],
+            "u_rel_diff": float(np.abs(u - uo[-1][: g.ndof]).max() / np.abs(uo[-1][: g.ndof]).max())}
 
 
 def run_mixed2d(a):
@@ -155,22 +197,23 @@ def run_mixed2d(a):
     print(json.dumps(out), flush=True)
 
 
-def run_serve(a):
+def serve_comparison(n_req, rate, seed, max_active=16, hbm_gb=200.0, plain=True, verbose=True):
     """Serving: Poisson arrivals of parametric strip-footing solves, admitted by HBM footprint and solved
-    concurrently (pm_serve); solves/s and latency percentiles, FP64 and FP32-tangent runs of the same
-    trace, and the largest relative difference of their final displacements."""
+    concurrently (pm_serve): solves/s and latency percentiles of the served FP64 and served FP32-tangent runs of
+    the same trace, optionally the plain FP64 GPU form (the same requests one at a time in arrival order), and
+    the largest relative differences of the final displacements against the served FP64 ones."""
     from pmsim_amd import describe
     from pmsim_amd.solver import Fem, serve
     from pmsim_amd.workloads import footing_request, poisson_arrivals
-    rng = np.random.default_rng(a.seed)
+    rng = np.random.default_rng(seed)
     reqs = []
-    t0 = time.time()
-    for _ in range(a.requests):
+    for _ in range(n_req):
         (mesh, base, cfg, ess, nat), p = footing_request(rng)
         reqs.append((mesh, base, cfg, ess, nat, p))
-    arrivals = poisson_arrivals(np.random.default_rng(a.seed + 1), a.requests, a.rate)
-    print(f"setup: {a.requests} requests, {sum(r[0].ncell for r in reqs)} elements in all, {time.time() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    arrivals = poisson_arrivals(np.random.default_rng(seed + 1), n_req, rate)
+    if verbose:
+        print(f"serve: {n_req} requests, {sum(r[0].ncell for r in reqs)} elements in all, arrivals over "
+              f"{arrivals[-1]:.1f} s", file=sys.stderr, flush=True)
     # untimed warm-up: the first launches of each kernel load its code object
     wrng = np.random.default_rng(1)
     for prec in ("f64", "f32"):
@@ -183,23 +226,30 @@ def run_serve(a):
         for mesh, base, cfg, ess, nat, p in reqs:
             cfg.set_scaled_rr(1e-10).set_pcg(1e-12, 50000).set_tangent_precision(prec)
             hs.append(Fem(describe(mesh, base, cfg, ess, nat), cfg, keep_history=False, initialize=False))
-        res = serve(hs, arrivals, hbm_budget_bytes=int(a.hbm_gb * (1 << 30)), max_active=a.max_active)
+        res = serve(hs, arrivals, hbm_budget_bytes=int(hbm_gb * (1 << 30)), max_active=max_active)
         ok = [r for r in res if r["status"] == 0]
         lat = np.array([r["t_done"] - r["t_arrival"] for r in ok])
         span = max(r["t_done"] for r in res) - min(arrivals)
-        runs[prec] = dict(res=res, u=[h.get_u()[0] if r["status"] == 0 else None for h, r in zip(hs, res)],
+        runs[prec] = dict(u=[h.get_u()[0] if r["status"] == 0 else None for h, r in zip(hs, res)],
                           solves_per_s=len(ok) / span, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
                           failed=len(res) - len(ok), newton=sum(r["n_iterations"] for r in ok),
                           pcg=sum(r["n_pcg_iterations"] for r in ok))
-        print(f"{prec}: {len(ok)}/{len(res)} solved in {span:.1f} s, p50 {runs[prec]['p50']:.2f} s, p99 {runs[prec]['p99']:.2f} s",
-              file=sys.stderr, flush=True)
-    du = [float(np.abs(u2 - u1).max() / max(np.abs(u1).max(), 1e-300))
-          for u1, u2 in zip(runs["f64"]["u"], runs["f32"]["u"]) if u1 is not None and u2 is not None]
-    plain = None
-    if a.plain:
-        # the plain FP64 GPU form: the same requests in arrival order, one at a time on one stream
-        # (set-up + Newton + single-request PCG each), no concurrency and no batching; latencies follow
-        # from FIFO service of the same arrival times
+        if verbose:
+            print(f"serve {prec}: {len(ok)}/{len(res)} solved in {span:.1f} s, p50 {runs[prec]['p50']:.2f} s, "
+                  f"p99 {runs[prec]['p99']:.2f} s", file=sys.stderr, flush=True)
+        del hs
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    du = [rel(u2, u1) for u1, u2 in zip(runs["f64"]["u"], runs["f32"]["u"]) if u1 is not None and u2 is not None]
+    keys = ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")
+    out = {"requests": n_req, "rate_per_s": rate, "seed": seed, "max_active": max_active,
+           "elements": [int(r[0].ncell) for r in reqs],
+           "workload": f"{n_req} strip-footing solves (Qua8 nx in [8,256], ny in [4,128], 5-40 increments), "
+                       f"Poisson arrivals at {rate}/s",
+           "served_fp64": {k: runs["f64"][k] for k in keys},
+           "served_fp32_tangent": {k: runs["f32"][k] for k in keys},
+           "u_rel_diff_fp32_tangent_vs_fp64": {"max": max(du) if du else None, "median": float(np.median(du)) if du else None}}
+    if plain:
         order = np.argsort(arrivals, kind="stable")
         t_free, lat, solve_s, up = 0.0, [], [], []
         for i in order:
@@ -210,33 +260,33 @@ def run_serve(a):
             h.solve()
             u = h.get_u()[0]
             del h
-            s = time.time() - t1
+            sec = time.time() - t1
             start = max(t_free, float(arrivals[i]))
-            t_free = start + s
+            t_free = start + sec
             lat.append(t_free - float(arrivals[i]))
-            solve_s.append(s)
-            u64 = runs["f64"]["u"][i]
-            if u64 is not None:
-                up.append(float(np.abs(u - u64).max() / max(np.abs(u).max(), 1e-300)))
-            print(f"plain {len(solve_s)}/{a.requests}: {mesh.ncell} cells, {s:.2f} s", file=sys.stderr, flush=True)
+            solve_s.append(sec)
+            if runs["f64"]["u"][i] is not None:
+                up.append(rel(u, runs["f64"]["u"][i]))
+            if verbose:
+                print(f"plain {len(solve_s)}/{n_req}: {mesh.ncell} cells, {sec:.2f} s", file=sys.stderr, flush=True)
         span = t_free - float(np.min(arrivals))
-        plain = {"solves_per_s": a.requests / span, "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                 "sum_solve_s": float(np.sum(solve_s)), "u_rel_diff_served_vs_alone_max": max(up) if up else None}
-    f64 = runs["f64"]
+        out["plain_fp64"] = {"solves_per_s": n_req / span, "p50": float(np.percentile(lat, 50)),
+                             "p99": float(np.percentile(lat, 99)), "sum_solve_s": float(np.sum(solve_s)),
+                             "u_rel_diff_vs_served_fp64_max": max(up) if up else None}
+        out["speedup_vs_plain_fp64"] = {"served_fp64": runs["f64"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"],
+                                        "served_fp32_tangent": runs["f32"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]}
+    return out
+
+
+def run_serve(a):
+    """`--workload serve`: the serving comparison as its own JSON line."""
+    sv = serve_comparison(a.requests, a.rate, a.seed, a.max_active, a.hbm_gb, plain=a.plain)
+    f64 = sv["served_fp64"]
     out = {"metric": "served solves/s", "value": f64["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": a.requests,
            "warmup": 0, "ms_per_step": 1e3 / f64["solves_per_s"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded parametric strip footings, Poisson arrivals)",
-           "config": {"workload": f"{a.requests} strip-footing solves (Qua8 nx in [8,256], ny in [4,128], 5-40 increments), "
-                                  f"Poisson arrivals at {a.rate}/s", "max_active": a.max_active, "hbm_budget_gb": a.hbm_gb,
-                      "elements": [int(r[0].ncell) for r in reqs]},
-           "latency_s": {"p50": f64["p50"], "p99": f64["p99"]},
-           "fp64": {k: f64[k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
-           "fp32_tangent": {k: runs["f32"][k] for k in ("solves_per_s", "p50", "p99", "failed", "newton", "pcg")},
-           "u_rel_diff_fp32_vs_fp64": {"max": max(du) if du else None, "median": float(np.median(du)) if du else None}}
-    if plain is not None:
-        out["plain_fp64_sequential"] = plain
-        out["speedup_vs_plain"] = {"served_fp64": f64["solves_per_s"] / plain["solves_per_s"],
-                                   "served_fp32_tangent": runs["f32"]["solves_per_s"] / plain["solves_per_s"]}
+           "config": {"workload": sv["workload"], "max_active": a.max_active, "hbm_budget_gb": a.hbm_gb},
+           "latency_s": {"p50": f64["p50"], "p99": f64["p99"]}, "served": sv}
     print(json.dumps(out), flush=True)
 
 
@@ -340,6 +390,9 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="hex20: leave out the served comparison")
+    ap.add_argument("--serve-requests", type=int, default=20, help="hex20: requests of the served comparison")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="hex20: Poisson rate of the served comparison (1/s)")
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision", "pcg"])
     ap.add_argument("--pcg-iters", type=int, default=1600, help="pcg: PCG iterations per timed solve")
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
@@ -365,9 +418,11 @@ def main():
         return run_precision(a)
     if a.workload == "pcg":
         return run_pcg(a)
-    if a.workload == "serve":
-        # more hardware queues than the default 4 so the concurrent requests' streams do not serialise
+    # more hardware queues than the default 4 so that the concurrent served requests' streams do not
+    # serialise (set before the first HIP call of the process)
+    if a.workload == "serve" or (a.workload == "hex20" and not a.no_serve):
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.max_active)))
+    if a.workload == "serve":
         return run_serve(a)
     from pmsim_amd.solver import Fem
     t0 = time.time()
@@ -386,9 +441,8 @@ def main():
     plastic = float(np.mean(st[:, f.ns + 2] == 0.0))
     nip = f.nip
     value = nip / (ms["pass_"] * 1e-3)
-    tangent_flops = FLOP_PER_IP * nip
-    achieved_tf = tangent_flops / (ms["tangent"] * 1e-3) / 1e12
     traffic, kname = measured_traffic(a.nx, a.ny, a.nz, a.precision, a.kernel)
+    ops = measured_ops(a.nx, a.ny, a.nz, a.precision, a.kernel)
     out = {
         "metric": "Hex20 IP-updates/s (K+R+return map)",
         "value": value,
@@ -406,11 +460,21 @@ def main():
                    "elements": int(mesh.ncell), "ips": int(nip), "dofs": int(f.ndof), "plastic_fraction": plastic,
                    "copy_calibration_bytes": int(2 * nip * (f.ns + 4) * 8)},
         "kernels_ms": {k: ms[k] for k in ("update", "residual", "tangent", "scatter", "total")},
-        "roofline": {"bound": "mfma", "kernel": kname or "k_tangent", "achieved": achieved_tf, "peak": PEAK_TFLOPS[a.precision],
-                     "unit": "TFLOP/s", "frac": achieved_tf / PEAK_TFLOPS[a.precision], "traffic": traffic,
-                     "flop_per_ip": FLOP_PER_IP, "executed_flop_per_ip": EXECUTED_FLOP_PER_IP,
-                     "executed_frac": EXECUTED_FLOP_PER_IP * nip / (ms["tangent"] * 1e-3) / 1e12 / PEAK_TFLOPS[a.precision]},
     }
+    # roofline of the dominant kernel (the tangent) against the unit it runs on, with the operations it executes
+    # as counted by the committed SQ counter summary of this workload; the dense B^T D B count is kept apart
+    dense_tf = FLOP_PER_IP * nip / (ms["tangent"] * 1e-3) / 1e12
+    if ops is not None:
+        unit_peak = PEAK_TFLOPS["f64"] if ops["bound"] == "valu-f64" else PEAK_TFLOPS[a.precision]
+        achieved = ops["ops_per_launch"] / (ms["tangent"] * 1e-3) / 1e12
+        out["roofline"] = {"bound": ops["bound"], "kernel": ops["kernel"], "achieved": achieved, "peak": unit_peak,
+                           "unit": "TFLOP/s", "frac": achieved / unit_peak, "traffic": traffic,
+                           "ops_per_launch": ops["ops_per_launch"], "ops_per_ip": ops["ops_per_launch"] / nip,
+                           "ops_counter": ops["counter"],
+                           "dense_count": {"flop_per_ip": FLOP_PER_IP, "tflops_equivalent": dense_tf}}
+    else:
+        out["roofline"] = {"bound": None, "kernel": kname or "k_tangent", "achieved": None, "peak": None, "unit": "TFLOP/s",
+                           "frac": None, "traffic": traffic, "dense_count": {"flop_per_ip": FLOP_PER_IP, "tflops_equivalent": dense_tf}}
     # the scatter is the HBM-bound half of the assembly: the stored element matrices in (upper node blocks,
     # 210 of the 400 per Hex20 element, 1,892 values with padding), the stored upper CSR blocks out (in the
     # tangent's precision), one map entry per element upper block and the map pointers
@@ -421,6 +485,10 @@ def main():
     scatter_bytes = ke_values * vbytes + nnzu * 9 * vbytes + nel * 210 * 4 + (nnzu + 1) * 4
     out["hbm"] = {"kernel": "k_scatter", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": scatter_bytes}
+    f.close()
+    del f
+    if not a.no_serve:
+        out["served"] = serve_comparison(a.serve_requests, a.serve_rate, a.seed, a.max_active, a.hbm_gb, plain=True)
     if not a.no_cpu:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb

@@ -75,6 +75,10 @@ typedef struct pm_stats {
 /* ---- errors ---------------------------------------------------------------------------- */
 const char* pm_last_error(const pm_fem* fem);      /* message of the last failure on `fem` */
 const char* pm_global_last_error(void);            /* message of the last pm_fem_create failure */
+/* Cell and integration point of the first IP that raised the last device-side material or Jacobian error
+ * (pmsim prints them as "cell_id = .., gauss_id = .." before returning the error, von_mises.rs:147-150,
+ * 213-216; this library prints the same line on stdout); -1 when the last error had none. */
+int pm_last_error_location(const pm_fem* fem, int64_t* cell, int32_t* gauss);
 void pm_default_config(pm_config* cfg);            /* fills pmsim's defaults (config.rs:184-231) */
 
 /* ---- problem definition (replaces gemlab Mesh + FemBase::new, src/fem/fem_base.rs:25-30) -- */
@@ -104,6 +108,13 @@ int pm_set_point_loads(pm_fem* fem, int64_t n, const int32_t* points, const int3
  * stations (nstation = 1: constant). */
 int pm_add_facet_qn(pm_fem* fem, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell,
                     const double* values, int32_t nstation);
+/* Distributed load of kind nbc on one boundary facet (BcDistributed::calc_f_ext, src/fem/bc_distributed.rs:156-236):
+ * nbc 0 = Qn (normal; the owner cell orients the outward normal; not on 3D edges), 1/2/3 = Qx/Qy/Qz (a traction
+ * in x/y/z per unit facet measure; the owner cell is not used and may be -1; allowed on 3D edges).  Errors use
+ * pmsim's texts: "Qn natural boundary condition is not available for 3D edge", and "cannot find the number of
+ * a (PointId, DOF) pair" for Qz in 2D. */
+int pm_add_facet_load(pm_fem* fem, int32_t nbc, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell,
+                      const double* values, int32_t nstation);
 
 /* Builds element batches, the node-block CSR pattern, the deterministic scatter maps and all
  * device buffers (replaces SolverImplicit::new / SolverData::new / LinearSystem::new,
@@ -146,6 +157,25 @@ int pm_sizes(const pm_fem* fem, int64_t* out /* [ndof, n_prescribed, nip, nsig, 
 int pm_set_ddu(pm_fem* fem, const double* ddu /* ndof */);
 int pm_get_u(const pm_fem* fem, double* u /* ndof */, double* lambda /* n_prescribed or NULL */);
 int pm_get_f_int(const pm_fem* fem, double* f /* ndof */);
+/* External forces at time t (SolverData::assemble_ff_int_and_ff_ext's F_ext half, src/fem/solver_data.rs:85-108:
+ * distributed loads bc_distributed.rs:156-236 and point loads bc_concentrated.rs:46-55 at the time station
+ * nearest t, prescribed equations skipped under RSM), assembled on the host and copied to the device. */
+int pm_assemble_f_ext(pm_fem* fem, double t);
+int pm_get_f_ext(const pm_fem* fem, double* f /* ndof */);
+/* R = F_int - loading_factor * F_ext on the device (SolverData::calculate_residuals_vector, solver_data.rs:111-120),
+ * from the last F_int and F_ext; the Lagrange-multiplier rows are not added (bc_prescribed.rs:99-108 does that). */
+int pm_calculate_residuals_vector(pm_fem* fem, double loading_factor);
+int pm_get_residual(const pm_fem* fem, double* r /* ndof */);
+/* y = K x over all DOFs with the last assembled tangent (the global matrix of LinearSystem, linear_system.rs:65-70,
+ * applied as an operator; FP32-held K is read in FP32 and accumulated in FP64). */
+int pm_apply_kk(pm_fem* fem, const double* x /* ndof */, double* y /* ndof */);
+/* Local tangent of one cell from the last pm_assemble_kk (ElementTrait::calc_jacobian, element_trait.rs:23;
+ * ElementSolid::calc_jacobian, element_solid.rs:168-185): (nnode*ndim)^2 values, row-major, local DOF order
+ * ndim*m + d (element_dofs.rs:123-131). */
+int pm_get_element_kk(const pm_fem* fem, int64_t cell, double* kk);
+/* Local internal force of one cell from the last update (ElementTrait::calc_f_int, element_solid.rs:124-138):
+ * nnode*ndim values in local DOF order. */
+int pm_get_element_f_int(const pm_fem* fem, int64_t cell, double* f);
 /* IP state in cell order: per IP [sigma (nsig Mandel), z, lambda, elastic(0/1), epbar]. */
 int pm_get_ip_state(const pm_fem* fem, double* v);
 int pm_set_ip_state(pm_fem* fem, const double* v);

@@ -42,6 +42,8 @@ def lib():
         L.orc_set_essential.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int]
         L.orc_set_point_loads.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int]
         L.orc_add_facet_qn.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int]
+        L.orc_add_facet_load.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int]
+        L.orc_f_ext.argtypes = [C.c_void_p, C.c_double, _dp]
         L.orc_setup.argtypes = [C.c_void_p]
         L.orc_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, C.c_int, _dp]
@@ -93,10 +95,12 @@ class Oracle:
         plt = np.ascontiguousarray(d["pl_table"], dtype=np.float64)
         if plt.shape[0]:
             L.orc_set_point_loads(self.h, plt.shape[0], d["pl_points"], d["pl_dofs"], plt.ravel(), plt.shape[1])
-        for (fk, pts, cell, v) in d["facets"]:
+        for facet in d["facets"]:
+            fk, pts, cell, v = facet[:4]
+            nbc = facet[4] if len(facet) > 4 else 0
             p = np.asarray(pts, dtype=np.int32)
             v = np.ascontiguousarray(v, dtype=np.float64)
-            L.orc_add_facet_qn(self.h, int(fk), len(p), p, int(cell), v, len(v))
+            L.orc_add_facet_load(self.h, int(nbc), int(fk), len(p), p, int(cell), v, len(v))
         sz = np.zeros(4, dtype=np.int32)
         self._check(L.orc_sizes(self.h, sz))
         self.ndof, self.neq_total, self.nip, self.ns = (int(v) for v in sz)
@@ -150,6 +154,17 @@ class Oracle:
     def f_int(self):
         f = np.zeros(self.ndof)
         lib().orc_assemble_f_int(self.h, f)
+        return f
+
+    def f_ext(self, t):
+        """F_ext at time t (distributed and point loads; prescribed equations skipped under RSM)."""
+        f = np.zeros(self.ndof)
+        self._check(lib().orc_f_ext(self.h, float(t), f))
+        return f
+
+    def element_f(self, cell, ne):
+        f = np.zeros(ne)
+        self._check(lib().orc_element_f(self.h, int(cell), f))
         return f
 
     def element_K(self, cell, ne):

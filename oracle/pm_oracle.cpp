@@ -388,7 +388,7 @@ const char* stiffness(const Param& p, const Ideal& id, int ns, const IPState& st
 // ---------------------------------------------------------------------------------------------
 // problem
 // ---------------------------------------------------------------------------------------------
-struct Facet { int kind; std::vector<int> pts; int cell; std::vector<double> values; };
+struct Facet { int kind; std::vector<int> pts; int cell; std::vector<double> values; int nbc = 0; };  // nbc 0 Qn, 1-3 Qx/Qy/Qz
 
 struct Problem {
   int ndim = 2, npoint = 0, ncell = 0;
@@ -642,7 +642,8 @@ const char* calc_K(const Problem& P, int e, double* K) {
   return nullptr;
 }
 
-// boundary facet Qn load: f = int N qn n dGamma (bc_distributed.rs:166-174)
+// boundary facet load (BcDistributed::calc_f_ext, bc_distributed.rs:156-236): Qn: f = int N qn n dGamma with
+// the outward unit normal n (bc_distributed.rs:182-187); Qx/Qy/Qz: f = int N q e_k dGamma (bc_distributed.rs:195-231)
 void add_facet_load(const Problem& P, const Facet& F, double* ff) {
   const double value = load_value(P, F.values.data(), (int)F.values.size(), P.t);
   const Basis& b = basis(F.kind);
@@ -651,20 +652,27 @@ void add_facet_load(const Problem& P, const Facet& F, double* ff) {
   if (b.family == 4) rule = gauss_legendre_1d(std::max(3, nn));
   else if (b.family == 0) rule = tri_rule(nn <= 3 ? 3 : 6);
   else rule = gauss_rule(F.kind, 9);
-  // cell centroid (outward orientation)
+  // cell centroid (outward orientation of Qn)
   double cc[3] = {0, 0, 0};
-  int cn = P.cptr[F.cell + 1] - P.cptr[F.cell];
-  for (int k = 0; k < cn; k++) for (int i = 0; i < nd; i++) cc[i] += P.X[(size_t)P.cpts[P.cptr[F.cell] + k] * nd + i] / cn;
+  if (F.cell >= 0) {
+    int cn = P.cptr[F.cell + 1] - P.cptr[F.cell];
+    for (int k = 0; k < cn; k++) for (int i = 0; i < nd; i++) cc[i] += P.X[(size_t)P.cpts[P.cptr[F.cell] + k] * nd + i] / cn;
+  }
   double fc[3] = {0, 0, 0};
   for (int k = 0; k < nn; k++) for (int i = 0; i < nd; i++) fc[i] += P.X[(size_t)F.pts[k] * nd + i] / nn;
+  const bool line = b.family == 4;
   for (auto& ip : rule) {
     double N[27], dN[27 * 3];
     eval_basis(b, ip.x, N, dN);
+    // area vector: a normal whose length is the measure of the facet per unit reference measure; for a
+    // line in 3D, its tangent (only its length is used)
     double n[3] = {0, 0, 0};
     if (nd == 2) {
       double dx = 0, dy = 0;
       for (int m = 0; m < nn; m++) { dx += P.X[(size_t)F.pts[m] * 2] * dN[m]; dy += P.X[(size_t)F.pts[m] * 2 + 1] * dN[m]; }
       n[0] = dy; n[1] = -dx;
+    } else if (line) {
+      for (int m = 0; m < nn; m++) for (int i = 0; i < 3; i++) n[i] += P.X[(size_t)F.pts[m] * 3 + i] * dN[m];
     } else {
       double a[3] = {0, 0, 0}, c[3] = {0, 0, 0};
       for (int m = 0; m < nn; m++) for (int i = 0; i < 3; i++) {
@@ -673,15 +681,22 @@ void add_facet_load(const Problem& P, const Facet& F, double* ff) {
       }
       n[0] = a[1] * c[2] - a[2] * c[1]; n[1] = a[2] * c[0] - a[0] * c[2]; n[2] = a[0] * c[1] - a[1] * c[0];
     }
-    double dot = 0;
-    for (int i = 0; i < nd; i++) dot += n[i] * (fc[i] - cc[i]);
-    double sgn = dot < 0 ? -1.0 : 1.0;
     double w = ip.w * (nd == 2 ? P.ideal.thickness : 1.0);
     if (nd == 2 && P.ideal.axisym) {
       double r = 0; for (int m = 0; m < nn; m++) r += N[m] * P.X[(size_t)F.pts[m] * 2];
       w *= r;
     }
-    for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) ff[(size_t)F.pts[m] * nd + i] += N[m] * value * sgn * n[i] * w;
+    double v[3] = {0, 0, 0};
+    if (F.nbc == 0) {
+      double dot = 0;
+      for (int i = 0; i < nd; i++) dot += n[i] * (fc[i] - cc[i]);
+      double sgn = dot < 0 ? -1.0 : 1.0;
+      for (int i = 0; i < nd; i++) v[i] = value * sgn * n[i];
+    } else {
+      const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      v[F.nbc - 1] = value * len;
+    }
+    for (int m = 0; m < nn; m++) for (int i = 0; i < nd; i++) ff[(size_t)F.pts[m] * nd + i] += N[m] * v[i] * w;
   }
 }
 
@@ -1109,6 +1124,28 @@ int orc_set_point_loads(void* h, int n, const int* pts, const int* dofs, const d
 int orc_add_facet_qn(void* h, int facet_kind, int npts, const int* pts, int cell, const double* values, int nstation) {
   auto* P = (Problem*)h;
   P->facets.push_back({facet_kind, std::vector<int>(pts, pts + npts), cell, std::vector<double>(values, values + nstation)});
+  return 0;
+}
+
+int orc_add_facet_load(void* h, int nbc, int facet_kind, int npts, const int* pts, int cell, const double* values, int nstation) {
+  auto* P = (Problem*)h;
+  Facet F{facet_kind, std::vector<int>(pts, pts + npts), cell, std::vector<double>(values, values + nstation)};
+  F.nbc = nbc;
+  P->facets.push_back(F);
+  return 0;
+}
+
+// F_ext at time t (solver_data.rs:85-108): distributed loads (ignored equations skipped), then point loads
+int orc_f_ext(void* h, double t, double* out) {
+  auto* P = (Problem*)h;
+  const double t_keep = P->t;
+  P->t = t;
+  std::vector<double> fx(P->ndof, 0.0);
+  for (auto& F : P->facets) add_facet_load(*P, F, fx.data());
+  for (int g = 0; g < P->ndof; g++) out[g] = P->ignore[g] ? 0.0 : fx[g];
+  for (size_t k = 0; k < P->pl_point.size(); k++)
+    out[P->ndim * P->pl_point[k] + P->pl_dof[k]] += load_value(*P, &P->pl_table[k * P->pl_nstation], P->pl_nstation, P->t);
+  P->t = t_keep;
   return 0;
 }
 

@@ -1,6 +1,7 @@
 // extern "C" entry points (include/pmsim_amd.h).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -18,6 +19,11 @@ static thread_local std::string g_last_error;
   do {                                                                         \
     if (!(f)) { g_last_error = "null handle"; return 1; }                      \
     if (!(f)->initialized) { (f)->err = "pm_initialize has not been called"; return 1; } \
+  } while (0)
+#define CHECK_INIT_CONST(f)                                                    \
+  do {                                                                         \
+    if (!(f)) { g_last_error = "null handle"; return 1; }                      \
+    if (!(f)->initialized) { const_cast<pm_fem*>(f)->err = "pm_initialize has not been called"; return 1; } \
   } while (0)
 
 extern "C" {
@@ -141,14 +147,28 @@ int pm_set_point_loads(pm_fem* f, int64_t n, const int32_t* points, const int32_
 
 int pm_add_facet_qn(pm_fem* f, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell, const double* values,
                     int32_t nstation) {
+  return pm_add_facet_load(f, 0, facet_kind, npts, points, cell, values, nstation);
+}
+
+int pm_add_facet_load(pm_fem* f, int32_t nbc, int32_t facet_kind, int32_t npts, const int32_t* points, int64_t cell,
+                      const double* values, int32_t nstation) {
   CHECK_HANDLE(f);
   if (!points || !values) { f->err = "null facet points or values"; return 1; }
   if (f->initialized) { f->err = "natural conditions must be set before pm_initialize"; return 1; }
-  if (cell < 0 || cell >= f->ncell) { f->err = "facet owner cell out of range"; return 1; }
+  if (nbc < 0 || nbc > 3) { f->err = "unknown distributed load (0 Qn, 1 Qx, 2 Qy, 3 Qz)"; return 1; }
   if (kind_nnode(facet_kind) != npts) { f->err = "wrong number of points for facet kind"; return 1; }
   if (nstation < 1) { f->err = "a load table needs at least one station"; return 1; }
-  f->facets.push_back(Facet{facet_kind, std::vector<int32_t>(points, points + npts), cell,
-                            std::vector<double>(values, values + nstation)});
+  const bool edge = kind_ndim(facet_kind) == 1;
+  if (f->ndim == 2 && !edge) { f->err = "a 2D mesh takes edge facets only"; return 1; }
+  if (nbc == 0 && f->ndim == 3 && edge) { f->err = "Qn natural boundary condition is not available for 3D edge"; return 1; }
+  if (nbc == 3 && f->ndim == 2) { f->err = "cannot find the number of a (PointId, DOF) pair"; return 1; }
+  if (nbc == 0 && (cell < 0 || cell >= f->ncell)) { f->err = "facet owner cell out of range"; return 1; }
+  if (cell >= f->ncell) { f->err = "facet owner cell out of range"; return 1; }
+  for (int32_t i = 0; i < npts; i++)
+    if (points[i] < 0 || points[i] >= f->npoint) { f->err = "cannot find the number of a (PointId, DOF) pair"; return 1; }
+  Facet F{facet_kind, std::vector<int32_t>(points, points + npts), cell < 0 ? -1 : cell, std::vector<double>(values, values + nstation)};
+  F.nbc = nbc;
+  f->facets.push_back(std::move(F));
   if (nstation > 1) f->loads_vary = true;
   return 0;
 }
@@ -297,6 +317,97 @@ int pm_get_f_int(const pm_fem* f, double* out) {
   if (!f->initialized) return 1;
   if (hipMemcpyAsync(out, f->d_fint, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
   return hipStreamSynchronize(f->stream) == hipSuccess ? 0 : 1;
+}
+
+int pm_assemble_f_ext(pm_fem* f, double t) {
+  CHECK_INIT(f);
+  const double k = (t - f->cfg.t_ini) / f->cfg.dt;
+  if (!std::isfinite(k)) { f->err = "Found NaN or Inf"; return 1; }
+  assemble_f_ext(f, (int64_t)std::llround(k));
+  PM_HIP(hipMemcpyAsync(f->d_fext, f->f_ext.data(), sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+int pm_get_f_ext(const pm_fem* f, double* out) {
+  CHECK_INIT_CONST(f);
+  if (!out) return 1;
+  if (hipMemcpyAsync(out, f->d_fext, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
+  return hipStreamSynchronize(f->stream) == hipSuccess ? 0 : 1;
+}
+
+int pm_calculate_residuals_vector(pm_fem* f, double loading_factor) {
+  CHECK_INIT(f);
+  dev_residual_plain(f, loading_factor);
+  PM_HIP(hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+int pm_get_residual(const pm_fem* f, double* out) {
+  CHECK_INIT_CONST(f);
+  if (!out) return 1;
+  if (hipMemcpyAsync(out, f->d_rr, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
+  return hipStreamSynchronize(f->stream) == hipSuccess ? 0 : 1;
+}
+
+int pm_apply_kk(pm_fem* f, const double* x, double* y) {
+  CHECK_INIT(f);
+  if (!x || !y) { f->err = "null argument"; return 1; }
+  double* dx = f->d_pcg;             // PCG work vectors: not live outside a linear solve
+  double* dy = f->d_pcg + f->ndof;
+  PM_HIP(hipMemcpyAsync(dx, x, sizeof(double) * f->ndof, hipMemcpyHostToDevice, f->stream));
+  dev_spmv(f, dx, dy, false);
+  PM_HIP(hipMemcpyAsync(y, dy, sizeof(double) * f->ndof, hipMemcpyDeviceToHost, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+int pm_get_element_kk(const pm_fem* f, int64_t cell, double* kk) {
+  CHECK_INIT_CONST(f);
+  if (!kk) return 1;
+  if (cell < 0 || cell >= f->ncell) { const_cast<pm_fem*>(f)->err = "cell out of range"; return 1; }
+  const Group& g = f->groups[f->cell_group[cell]];
+  const int64_t el = f->cell_local[cell];
+  const int nn = g.nnode, nd = g.ndim, ne = nn * nd;
+  std::vector<double> v(g.kew);
+  if (f->d_ke) {
+    if (hipMemcpyAsync(v.data(), f->d_ke + g.ke_base + el * g.kew, sizeof(double) * g.kew, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
+  } else {
+    std::vector<float> w(g.kew);
+    if (hipMemcpyAsync(w.data(), f->d_ke32 + g.ke_base + el * g.kew, sizeof(float) * g.kew, hipMemcpyDeviceToHost, f->stream) != hipSuccess) return 1;
+    if (hipStreamSynchronize(f->stream) != hipSuccess) return 1;
+    std::copy(w.begin(), w.end(), v.begin());
+  }
+  if (hipStreamSynchronize(f->stream) != hipSuccess) return 1;
+  int rk[32];
+  cell_ranks(f, cell, rk);
+  // stored: upper node blocks in rank order; block (a, b) of a lower rank pair is the transpose of (b, a)
+  for (int a = 0; a < nn; a++)
+    for (int b = 0; b < nn; b++) {
+      const bool up = rk[a] <= rk[b];
+      const double* blk = v.data() + (size_t)packed_block(nn, up ? rk[a] : rk[b], up ? rk[b] : rk[a]) * nd * nd;
+      for (int i = 0; i < nd; i++)
+        for (int j = 0; j < nd; j++) kk[(size_t)(a * nd + i) * ne + b * nd + j] = up ? blk[i * nd + j] : blk[j * nd + i];
+    }
+  return 0;
+}
+
+int pm_get_element_f_int(const pm_fem* f, int64_t cell, double* out) {
+  CHECK_INIT_CONST(f);
+  if (!out) return 1;
+  if (cell < 0 || cell >= f->ncell) { const_cast<pm_fem*>(f)->err = "cell out of range"; return 1; }
+  const Group& g = f->groups[f->cell_group[cell]];
+  const int64_t n = (int64_t)g.nnode * g.ndim;
+  if (hipMemcpyAsync(out, f->d_fe + g.fe_base + f->cell_local[cell] * n, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream) != hipSuccess)
+    return 1;
+  return hipStreamSynchronize(f->stream) == hipSuccess ? 0 : 1;
+}
+
+int pm_last_error_location(const pm_fem* f, int64_t* cell, int32_t* gauss) {
+  CHECK_HANDLE(f);
+  if (cell) *cell = f->err_cell;
+  if (gauss) *gauss = f->err_gauss;
+  return 0;
 }
 
 int pm_get_ip_state(const pm_fem* f, double* v) {

@@ -107,9 +107,9 @@ int device_alloc(pm_fem* f) {
   PM_HIP(hipMemsetAsync(f->d_u, 0, sizeof(double) * n, f->stream));
   PM_HIP(hipMemsetAsync(f->d_ddu, 0, sizeof(double) * n, f->stream));
   PM_HIP(hipMemsetAsync(f->d_lam, 0, sizeof(double) * npr, f->stream));
-  PM_HIP(hipMemsetAsync(f->d_err, 0, sizeof(int32_t) * 4, f->stream));
   PM_HIP(hipMemsetAsync(f->d_scal, 0, sizeof(double) * 32, f->stream));
   PM_HIP(hipStreamSynchronize(f->stream));
+  reset_device_error(f);
   for (auto& g : f->groups) { launch_init_state(f, g); launch_copy(f, g.d_backup, g.d_state, g.nel * g.nip * (ns + 4)); }
   PM_HIP(hipStreamSynchronize(f->stream));
   // host copies of the maps are no longer needed
@@ -132,10 +132,13 @@ int full_pattern_alloc(pm_fem* f) {
                   !dalloc(f, &f->d_f2u, f->nnzb) && !upload(f, f->d_f2u, f->h_full2up.data(), f->nnzb) &&
                   !(f->cfg.tangent_precision == 0 ? dalloc(f, &f->d_kfull, f->nnzb * b2) : dalloc(f, &f->d_kfull32, f->nnzb * b2));
   if (ok) return 0;
-  // a failed allocation leaves nothing behind, so a later solve tries again from scratch
+  // a failed allocation leaves nothing behind (and is not counted), so a later solve tries again from scratch
   (void)hipStreamSynchronize(f->stream);  // uploads into these buffers end before they are freed
-  for (void* q : {(void*)f->d_frp, (void*)f->d_fcol, (void*)f->d_f2u, (void*)f->d_kfull, (void*)f->d_kfull32})
-    if (q) (void)hipFree(q);
+  const int64_t vsz = f->cfg.tangent_precision == 0 ? (int64_t)sizeof(double) : (int64_t)sizeof(float);
+  const std::pair<void*, int64_t> made[] = {{f->d_frp, 4 * (f->npoint + 1)}, {f->d_fcol, 4 * f->nnzb}, {f->d_f2u, 4 * f->nnzb},
+                                            {f->d_kfull, vsz * f->nnzb * b2}, {f->d_kfull32, vsz * f->nnzb * b2}};
+  for (const auto& m : made)
+    if (m.first) { (void)hipFree(m.first); f->device_bytes -= std::max<int64_t>(m.second, (int64_t)sizeof(int32_t)); }
   f->d_frp = f->d_fcol = f->d_f2u = nullptr;
   f->d_kfull = nullptr;
   f->d_kfull32 = nullptr;

@@ -284,18 +284,23 @@ void assemble_f_ext(pm_fem* f, int64_t k) {
     else gauss_rule(QUA9, 9, R);
     // owner cell centroid and facet centroid (outward orientation)
     double cc[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
-    int64_t cn = f->cptr[F.cell + 1] - f->cptr[F.cell];
-    for (int64_t k = 0; k < cn; k++) for (int i = 0; i < nd; i++) cc[i] += f->X[(int64_t)f->cpts[f->cptr[F.cell] + k] * nd + i] / cn;
+    if (F.cell >= 0) {
+      int64_t cn = f->cptr[F.cell + 1] - f->cptr[F.cell];
+      for (int64_t k = 0; k < cn; k++) for (int i = 0; i < nd; i++) cc[i] += f->X[(int64_t)f->cpts[f->cptr[F.cell] + k] * nd + i] / cn;
+    }
     for (int k = 0; k < nn; k++) for (int i = 0; i < nd; i++) fc[i] += f->X[(int64_t)F.pts[k] * nd + i] / nn;
+    const int fd = kind_ndim(F.kind);
     for (int q = 0; q < R.n(); q++) {
       double N[20], dN[40];
       shape(F.kind, &R.x[3 * q], N, dN);
-      int fd = kind_ndim(F.kind);
+      // n: the facet's normal scaled by its measure (2D edge: (dy, -dx); 3D face: a x b); 3D edge: the tangent
       double n[3] = {0, 0, 0};
       if (nd == 2) {
         double dx = 0, dy = 0;
         for (int m = 0; m < nn; m++) { dx += f->X[(int64_t)F.pts[m] * 2] * dN[m]; dy += f->X[(int64_t)F.pts[m] * 2 + 1] * dN[m]; }
         n[0] = dy; n[1] = -dx;
+      } else if (fd == 1) {
+        for (int m = 0; m < nn; m++) for (int i = 0; i < 3; i++) n[i] += f->X[(int64_t)F.pts[m] * 3 + i] * dN[m];
       } else {
         double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
         for (int m = 0; m < nn; m++) for (int i = 0; i < 3; i++) {
@@ -304,19 +309,28 @@ void assemble_f_ext(pm_fem* f, int64_t k) {
         }
         n[0] = a[1] * b[2] - a[2] * b[1]; n[1] = a[2] * b[0] - a[0] * b[2]; n[2] = a[0] * b[1] - a[1] * b[0];
       }
-      double dot = 0;
-      for (int i = 0; i < nd; i++) dot += n[i] * (fc[i] - cc[i]);
-      double w = R.w[q] * (dot < 0 ? -1.0 : 1.0) * (nd == 2 ? f->thickness : 1.0);
+      double w = R.w[q] * (nd == 2 ? f->thickness : 1.0);
       if (nd == 2 && f->axisym) {
         double r = 0;
         for (int m = 0; m < nn; m++) r += N[m] * f->X[(int64_t)F.pts[m] * 2];
         w *= r;
       }
+      // Qn: value times the outward normal (bc_distributed.rs:182-187); Qx/Qy/Qz: value in one direction per
+      // unit facet measure |n| (bc_distributed.rs:195-231)
+      double v[3] = {0, 0, 0};
+      if (F.nbc == 0) {
+        double dot = 0;
+        for (int i = 0; i < nd; i++) dot += n[i] * (fc[i] - cc[i]);
+        const double sgn = dot < 0 ? -1.0 : 1.0;
+        for (int i = 0; i < nd; i++) v[i] = value * sgn * n[i];
+      } else {
+        v[F.nbc - 1] = value * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      }
       for (int m = 0; m < nn; m++)
         for (int i = 0; i < nd; i++) {
           const int64_t eq = (int64_t)F.pts[m] * nd + i;
           if (rsm && f->is_prescribed[eq]) continue;  // ignored equations (solver_data.rs:103)
-          f->f_ext[eq] += N[m] * value * n[i] * w;
+          f->f_ext[eq] += N[m] * v[i] * w;
         }
     }
   }

@@ -15,6 +15,17 @@ namespace pm {
 enum Kind { TRI3 = 0, TRI6, TRI10, TRI15, QUA4, QUA8, QUA9, TET4, TET10, HEX8, HEX20, NKIND };
 enum FacetKind { LIN2 = 100, LIN3, LIN4, LIN5 };
 
+// Device error buffer (4 int32 words): word 0 holds the OR of the error bits raised by the element kernels,
+// words 2-3 the smallest key (group << 40 | IP index within the group) of an IP that raised one, so the
+// first failing cell and integration point can be reported as pmsim prints them (von_mises.rs:147-150).
+enum DevErr { ERR_SIGD = 1, ERR_NORMS = 2, ERR_DETJ = 4, ERR_ASYM = 8 };
+constexpr uint64_t kErrKeyNone = ~0ull;
+constexpr int kErrKeyShift = 40;
+__device__ inline void raise_error(int32_t* err, int bit, uint64_t key) {
+  atomicOr(err, bit);
+  atomicMin(reinterpret_cast<unsigned long long*>(err + 2), (unsigned long long)key);
+}
+
 int kind_nnode(int kind);
 int kind_ndim(int kind);
 int kind_default_ngauss(int kind);
@@ -77,8 +88,9 @@ struct Group {
   int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
 
-// boundary facet with a normal distributed load: value at each time station (one entry: constant)
-struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; std::vector<double> values; };
+// boundary facet with a distributed load: value at each time station (one entry: constant);
+// nbc: 0 normal (Qn), 1/2/3 traction in x/y/z (Qx/Qy/Qz)
+struct Facet { int kind; std::vector<int32_t> pts; int64_t cell; std::vector<double> values; int nbc = 0; };
 
 struct Timers { double update = 0, residual = 0, tangent = 0, scatter = 0, linear = 0, total = 0; };
 
@@ -186,7 +198,9 @@ struct pm_fem {
   double* d_binv = nullptr;            // block-Jacobi inverses (npoint*bs*bs)
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
-  int32_t* d_err = nullptr;            // device error flags
+  int32_t* d_err = nullptr;            // device error flags and first failing IP (DevErr, raise_error)
+  int64_t err_cell = -1;               // cell and Gauss point of the last device-side error (-1: none)
+  int32_t err_gauss = -1;
   int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
   pm::PcgEngine* engine = nullptr;     // batched PCG of a serving run (set by pm_serve), else per handle
   hipGraphExec_t pass_graph[8] = {};   // captured element passes, by (update, residual, tangent)
@@ -235,6 +249,7 @@ void launch_init_state(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);
 void launch_copy(pm_fem* f, double* dst, const double* src, int64_t n);
 int check_device_error(pm_fem* f);
+void reset_device_error(pm_fem* f);
 size_t update_shared_bytes(const Group& g);
 size_t tangent_shared_bytes(const Group& g);
 // vector kernels (pm_kernels.hip)
@@ -242,6 +257,7 @@ double dev_max_abs(pm_fem* f, const double* x, int64_t n, const double* mask);
 double dev_max_rel(pm_fem* f, const double* x, const double* x0, int64_t n);
 double dev_dot(pm_fem* f, const double* a, const double* b, int64_t n);
 void dev_residual(pm_fem* f, double* g, double ell = 1.0);
+void dev_residual_plain(pm_fem* f, double ell);  // R = F_int - ell F_ext only
 void dev_rhs(pm_fem* f, const double* g, double* t, double* kt, double* b);
 void dev_complete_correction(pm_fem* f, const double* g, double* x, double* kx, double* y, const double* rhs = nullptr);
 void dev_arc_trial(pm_fem* f, int64_t n, bool extrapolate, double alpha, const double* u_old, const double* u_anc, double* u,

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -21,8 +22,6 @@ namespace {
 constexpr double SQ2 = 1.41421356237309504880;
 constexpr double ISQ2 = 0.70710678118654752440;
 constexpr int TPB = 256;
-
-enum { ERR_SIGD = 1, ERR_NORMS = 2, ERR_DETJ = 4 };
 
 struct Mat {
   int model;         // 0 linear elastic, 1 von Mises
@@ -69,7 +68,7 @@ __device__ __forceinline__ void jacobian(const double* __restrict__ Xe, const do
 // linear_elastic.rs:49-71).  Starts from the given state, writes the updated one.
 template <int NS>
 __device__ __forceinline__ void stress_update(const Mat& mp, double* sig, double& z, double& lam, double& elastic,
-                                              double& epbar, const double* de, int32_t* err) {
+                                              double& epbar, const double* de, int32_t* err, uint64_t key) {
   if (mp.plane_stress) {
     double c = mp.E / (1.0 - mp.nu * mp.nu);
     sig[0] += c * (de[0] + mp.nu * de[1]);
@@ -100,7 +99,7 @@ __device__ __forceinline__ void stress_update(const Mat& mp, double* sig, double
     sig[i] = (i < 3 ? sm : 0.0) + beta * dev;
     ss2 += (beta * dev) * (beta * dev);
   }
-  if (sqrt(1.5 * ss2) < 1e-10) *err = ERR_SIGD;
+  if (sqrt(1.5 * ss2) < 1e-10) raise_error(err, ERR_SIGD, key);
   z += mp.H * l;
   elastic = 0.0;
   lam = l;
@@ -125,7 +124,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
                                                 const double* __restrict__ X, const double* __restrict__ ddu,
                                                 const double* __restrict__ tab, const double* __restrict__ st_in,
                                                 double* __restrict__ st_out, double* __restrict__ fe, Mat mp, int axisym,
-                                                double thickness, int do_update, int32_t* err) {
+                                                double thickness, int do_update, int32_t* err, uint64_t ekey) {
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int SW = NS + ND * ND + 1;  // per IP: c*sigma, J^-1, radius
   extern __shared__ double sh[];
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
     const double* Ue = sU + el * NN * ND;
     double Ji[ND][ND], det;
     jacobian<ND, NN>(Xe, dN, Ji, det);
-    if (!(det > 0.0)) *err = ERR_DETJ;
+    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)(e * nip + q));
     double r = 0.0;
     if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * Xe[m * ND];
     // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
@@ -192,7 +191,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
     double z = st_in[NS * nipg + ip], lam = st_in[(NS + 1) * nipg + ip], el_ = st_in[(NS + 2) * nipg + ip];
     double ep = st_in[(NS + 3) * nipg + ip];
     if (do_update) {
-      stress_update<NS>(mp, sig, z, lam, el_, ep, de, err);
+      stress_update<NS>(mp, sig, z, lam, el_, ep, de, err, ekey + (uint64_t)ip);
 #pragma unroll
       for (int c = 0; c < NS; c++) st_out[c * nipg + ip] = sig[c];
       st_out[NS * nipg + ip] = z;
@@ -280,7 +279,8 @@ template <int ND, int NN, typename KT>
 __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, int ch, const int32_t* __restrict__ conn,
                                                  const uint8_t* __restrict__ rank, const double* __restrict__ X,
                                                  const double* __restrict__ tab, const double* __restrict__ st,
-                                                 KT* __restrict__ ke, Mat mp, int axisym, double thickness, int32_t* err) {
+                                                 KT* __restrict__ ke, Mat mp, int axisym, double thickness, int32_t* err,
+                                                 uint64_t ekey) {
   using TS = TanShape<ND, NN>;
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 #pragma unroll
       for (int j = 0; j < ND; j++) J[i][j] = Q[3 + i * ND + j];
     if constexpr (ND == 2) inv_jac2(J, Ji, det); else inv_jac3(J, Ji, det);
-    if (!(det > 0.0)) *err = ERR_DETJ;
+    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)(e * nip + q));
     double r = 0.0;
     if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * sX[(el * NN + m) * ND];
     double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
       double sd = sqrt(1.5 * ss);
       double sdtr = sd + lam * 3.0 * mp.G;
       double norm_s = sd * 0.81649658092772603273;  // sqrt(2/3)
-      if (norm_s < 1e-10) *err = ERR_NORMS;
+      if (norm_s < 1e-10) raise_error(err, ERR_NORMS, ekey + (uint64_t)ip);
       double d = 3.0 * mp.G + mp.H;
       alpha = 2.0 * mp.G * (1.0 - lam * 3.0 * mp.G / sdtr);
       gamma = 6.0 * mp.G * mp.G * (lam / sdtr - 1.0 / d) / (norm_s * norm_s);
@@ -529,10 +529,12 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
           const int ra = rk[a], rb = rk[b];
           const bool up = ra <= rb;
           KT* o = stage + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
+          // a diagonal node block takes its lower half from the upper one, so every element matrix (and K)
+          // is exactly symmetric (the local check of assembly.rs:78-90 holds by construction)
 #pragma unroll
           for (int i = 0; i < ND; i++)
 #pragma unroll
-            for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)K[u][v][i][j];
+            for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)(a == b && j < i ? K[u][v][j][i] : K[u][v][i][j]);
         }
       if (tile == 0)
         for (int i = EU; i < EW; i++) stage[i] = (KT)0;
@@ -651,6 +653,9 @@ static Mat mat_of(const pm_fem* f, const Group& g) {
   return m;
 }
 
+// error key base of a group's IPs (raise_error)
+static uint64_t group_key(const pm_fem* f, const Group& g) { return (uint64_t)(&g - f->groups.data()) << kErrKeyShift; }
+
 constexpr size_t SHM_CAP = 60 * 1024;  // dynamic LDS budget per workgroup (static arrays use the rest of 64 KB)
 
 static size_t shm_update(int nd, int nn, int nip, int epb) {
@@ -704,7 +709,8 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
   PM_DISPATCH_KIND(g.kind, ND, NN,
                    hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip, epb,
                                       g.d_conn, f->d_X, f->d_ddu, g.d_tab, do_update ? g.d_backup : g.d_state, g.d_state,
-                                      f->d_fe + g.fe_base, mp, f->axisym ? 1 : 0, f->thickness, do_update ? 1 : 0, f->d_err));
+                                      f->d_fe + g.fe_base, mp, f->axisym ? 1 : 0, f->thickness, do_update ? 1 : 0, f->d_err,
+                                      group_key(f, g)));
 }
 
 // tangent kernel of a group: the tuned choice, else the default (matrix cores for reduced precision)
@@ -732,12 +738,12 @@ void launch_tangent(pm_fem* f, Group& g) {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
                                         epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp,
-                                        f->axisym ? 1 : 0, f->thickness, f->d_err));
+                                        f->axisym ? 1 : 0, f->thickness, f->d_err, group_key(f, g)));
   } else {
     PM_DISPATCH_KIND(g.kind, ND, NN,
                      hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
                                         epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp,
-                                        f->axisym ? 1 : 0, f->thickness, f->d_err));
+                                        f->axisym ? 1 : 0, f->thickness, f->d_err, group_key(f, g)));
   }
 }
 
@@ -841,22 +847,40 @@ void dev_fill(pm_fem* f, double* x, double v, int64_t n) {
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, f->stream, x, v, n);
 }
 
+void reset_device_error(pm_fem* f) {
+  const int32_t init[4] = {0, 0, -1, -1};  // no bits; key kErrKeyNone
+  (void)hipMemcpyAsync(f->d_err, init, sizeof(init), hipMemcpyHostToDevice, f->stream);
+  (void)hipStreamSynchronize(f->stream);
+}
+
 int check_device_error(pm_fem* f) {
   if (f->launch_refused) {
     f->launch_refused = false;
     f->err = "internal error: an element-matrix range did not fit its buffer (kernel not launched)";
     return 1;
   }
-  int32_t e = 0;
-  PM_HIP(hipMemcpyAsync(&e, f->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
+  int32_t w[4] = {0, 0, 0, 0};
+  PM_HIP(hipMemcpyAsync(w, f->d_err, sizeof(w), hipMemcpyDeviceToHost, f->stream));
   PM_HIP(hipStreamSynchronize(f->stream));
+  const int32_t e = w[0];
   if (e == 0) return 0;
-  int32_t z = 0;
-  PM_HIP(hipMemcpyAsync(f->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice, f->stream));
-  PM_HIP(hipStreamSynchronize(f->stream));
+  uint64_t key = 0;
+  std::memcpy(&key, w + 2, sizeof(key));
+  reset_device_error(f);
+  // the first failing IP: its cell and Gauss point (pmsim prints them before returning the error)
+  f->err_cell = -1;
+  f->err_gauss = -1;
+  const uint64_t gi = key >> kErrKeyShift, ip = key & ((1ull << kErrKeyShift) - 1);
+  if (key != kErrKeyNone && gi < f->groups.size()) {
+    const Group& g = f->groups[gi];
+    const uint64_t el = ip / (uint64_t)g.nip;
+    if (el < (uint64_t)g.nel) { f->err_cell = g.cells[el]; f->err_gauss = (int32_t)(ip - el * g.nip); }
+  }
+  if (f->err_cell >= 0) std::printf("cell_id = %lld, gauss_id = %d\n", (long long)f->err_cell, f->err_gauss);
   if (e & ERR_DETJ) f->err = "negative or zero Jacobian determinant";
   else if (e & ERR_SIGD) f->err = "von Mises plastic update must not lead to zero sigma_d (sigma_d < 1e-10)";
-  else f->err = "von Mises stiffness cannot be computed with zero deviatoric norm (norm_s < 1e-10)";
+  else if (e & ERR_NORMS) f->err = "von Mises stiffness cannot be computed with zero deviatoric norm (norm_s < 1e-10)";
+  else f->err = "local matrix is not symmetric";
   return 1;
 }
 

@@ -574,7 +574,7 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
   }
   // K widened to the full pattern once per solve; the iterations stream rows of it (same values and
   // column order as the stored blocks give)
-  if (full_pattern_alloc(f)) return 1;
+  if (full_pattern_alloc(f)) return 3;  // out of device memory: the caller may fall back to the host solver
   {
     const unsigned gw = (unsigned)((f->nnzb + TPB - 1) / TPB);
 #define PM_WIDEN(BS) do { if (lowp) hipLaunchKernelGGL((k_widen<BS, float>), dim3(gw), dim3(TPB), 0, f->stream, f->nnzb, f->d_f2u, f->d_kval32, f->d_kfull32); \
@@ -664,6 +664,10 @@ void dev_residual(pm_fem* f, double* g, double ell) {
   if (f->nprescribed)
     hipLaunchKernelGGL(k_prescribed_rows, dim3(nblk(f->nprescribed)), dim3(TPB), 0, f->stream, f->nprescribed,
                        f->cfg.lagrange_mult_method, f->d_peq, f->d_lam, f->d_u, f->d_cval, f->d_rr, g);
+}
+
+void dev_residual_plain(pm_fem* f, double ell) {
+  hipLaunchKernelGGL(k_residual, dim3(nblk(f->ndof)), dim3(TPB), 0, f->stream, f->ndof, f->d_fint, f->d_fext, ell, f->d_rr);
 }
 
 // b = (rr - K t) * free with t = g on prescribed DOFs

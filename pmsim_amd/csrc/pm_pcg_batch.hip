@@ -41,6 +41,8 @@ struct PcgSys {
   double* rzs;            // rz slots [2]
   double* rr;
   int nq, nv;             // workgroups of the SpMV and of the vector kernels
+  int nrun;               // iterations of the current chunk this system takes (fewer than CHUNK only just
+                          // before its iteration limit, which the single-request PCG stops at exactly)
 };
 
 __device__ __forceinline__ double bsum(double v, double* sh) {
@@ -77,11 +79,12 @@ __device__ __forceinline__ void bj_apply(int64_t a, const double* __restrict__ b
 __device__ __forceinline__ void unpack(int32_t e, int& s, int& w) { s = e >> 20; w = e & ((1 << 20) - 1); }
 
 template <int BS, typename VT>
-__global__ __launch_bounds__(BTPB) void kb_q(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map) {
+__global__ __launch_bounds__(BTPB) void kb_q(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map, int k) {
   __shared__ double sh[BTPB];
   int si, w;
   unpack(map[blockIdx.x], si, w);
   const PcgSys& S = sys[si];
+  if (k >= S.nrun) return;  // uniform over the workgroup
   const int lpr = S.lpr;
   const int lane = (int)(threadIdx.x & (lpr - 1));
   const int64_t ngroups = (int64_t)S.nq * BTPB / lpr;
@@ -104,11 +107,13 @@ __global__ __launch_bounds__(BTPB) void kb_q(const PcgSys* __restrict__ sys, con
 }
 
 template <int BS>
-__global__ __launch_bounds__(BTPB) void kb_xrz(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map, int parity) {
+__global__ __launch_bounds__(BTPB) void kb_xrz(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map, int k) {
   __shared__ double sh[BTPB];
   int si, w;
   unpack(map[blockIdx.x], si, w);
   const PcgSys& S = sys[si];
+  if (k >= S.nrun) return;  // uniform over the workgroup
+  const int parity = k & 1;
   const double pq = bsum_partials(S.pa, S.nq, sh);
   const double alpha = pq != 0.0 ? S.rzs[parity] / pq : 0.0;
   double rz = 0.0, rr = 0.0;
@@ -131,11 +136,13 @@ __global__ __launch_bounds__(BTPB) void kb_xrz(const PcgSys* __restrict__ sys, c
 }
 
 template <int BS>
-__global__ __launch_bounds__(BTPB) void kb_p(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map, int parity) {
+__global__ __launch_bounds__(BTPB) void kb_p(const PcgSys* __restrict__ sys, const int32_t* __restrict__ map, int k) {
   __shared__ double sh[BTPB];
   int si, w;
   unpack(map[blockIdx.x], si, w);
   const PcgSys& S = sys[si];
+  if (k >= S.nrun) return;  // uniform over the workgroup
+  const int parity = k & 1;
   const double rzn = bsum_partials(S.pb, S.nv, sh);
   const double rzo = S.rzs[parity];
   const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
@@ -238,7 +245,7 @@ struct PcgEngine {
     Job job;
     job.f = f; job.b = d_b; job.x = d_x; job.bs = f->bs;
     job.lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
-    job.max_it = (f->cfg.pcg_max_iter + CHUNK - 1) / CHUNK * CHUNK;
+    job.max_it = std::max(0, f->cfg.pcg_max_iter);  // exactly the single-request limit
     PcgSys& S = job.desc;
     S.np = np; S.frp = f->d_frp; S.fcol = f->d_fcol;
     S.kfull = job.lowp ? (const void*)f->d_kfull32 : (const void*)f->d_kfull;
@@ -249,6 +256,7 @@ struct PcgEngine {
     S.nq = spmv_workgroups(np, S.lpr, BTPB);
     S.pa = f->d_red; S.pb = S.pa + S.nq; S.pc = S.pb + S.nv;
     S.rzs = f->d_scal + 16; S.rr = f->d_scal + 18;
+    S.nrun = CHUNK;
     if (hipStreamSynchronize(f->stream) != hipSuccess) { f->err = "stream synchronization failed"; return 1; }
     std::unique_lock<std::mutex> lk(m);
     pending.push_back(&job);
@@ -287,6 +295,7 @@ struct PcgEngine {
     if (bb == 0.0) return 0;
     if (!std::isfinite(bb)) return 1;
     j->tol2 = f->cfg.pcg_rtol * f->cfg.pcg_rtol * bb;
+    if (j->max_it == 0) return bb <= j->tol2 ? 0 : 2;  // no iteration allowed: the start residual decides
     return -1;
   }
 
@@ -314,6 +323,12 @@ struct PcgEngine {
       }
       if (any_done) cv_done.notify_all();
       if (active.empty()) continue;
+      // a system close to its iteration limit takes only the iterations left (the single-request PCG
+      // checks at every 16th iteration and at the limit itself)
+      for (size_t s = 0; s < active.size(); s++) {
+        const int left = std::min(CHUNK, active[s]->max_it - active[s]->it);
+        if (active[s]->desc.nrun != left) { active[s]->desc.nrun = left; dirty = true; }
+      }
       if (dirty) {
         h_sys.resize(active.size());
         for (auto& c : h_map) { c[0].clear(); c[1].clear(); }
@@ -342,16 +357,16 @@ struct PcgEngine {
           off += (int64_t)h_map[cls][1].size();
           const unsigned gq = (unsigned)h_map[cls][0].size(), gv = (unsigned)h_map[cls][1].size();
           if (!gq) continue;
-          if (cls == 0) hipLaunchKernelGGL((kb_q<2, double>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq);
-          if (cls == 1) hipLaunchKernelGGL((kb_q<2, float>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq);
-          if (cls == 2) hipLaunchKernelGGL((kb_q<3, double>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq);
-          if (cls == 3) hipLaunchKernelGGL((kb_q<3, float>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq);
+          if (cls == 0) hipLaunchKernelGGL((kb_q<2, double>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq, k);
+          if (cls == 1) hipLaunchKernelGGL((kb_q<2, float>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq, k);
+          if (cls == 2) hipLaunchKernelGGL((kb_q<3, double>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq, k);
+          if (cls == 3) hipLaunchKernelGGL((kb_q<3, float>), dim3(gq), dim3(BTPB), 0, stream, d_sys, mq, k);
           if (cls < 2) {
-            hipLaunchKernelGGL((kb_xrz<2>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k & 1);
-            hipLaunchKernelGGL((kb_p<2>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k & 1);
+            hipLaunchKernelGGL((kb_xrz<2>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k);
+            hipLaunchKernelGGL((kb_p<2>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k);
           } else {
-            hipLaunchKernelGGL((kb_xrz<3>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k & 1);
-            hipLaunchKernelGGL((kb_p<3>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k & 1);
+            hipLaunchKernelGGL((kb_xrz<3>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k);
+            hipLaunchKernelGGL((kb_p<3>), dim3(gv), dim3(BTPB), 0, stream, d_sys, mv, k);
           }
         }
       }
@@ -365,7 +380,7 @@ struct PcgEngine {
         std::lock_guard<std::mutex> g(m);
         for (int s = 0; s < ns; s++) {
           Job* j = active[s];
-          j->it += CHUNK;
+          j->it += j->desc.nrun;
           if (!ok) { err = "kernel execution"; finish(j, 3); }
           else if (!std::isfinite(h_rr[s])) finish(j, 1);
           else if (h_rr[s] <= j->tol2) finish(j, 0);

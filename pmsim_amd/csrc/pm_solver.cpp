@@ -111,7 +111,9 @@ int linear_solve_free(pm_fem* f, const double* d_b, double* d_x, int* iters) {
     int rc = pcg_solve(f, d_b, d_x, iters);
     if (rc == 0) return 0;
     if (rc == 1) return 1;  // NaN/Inf: no fallback
-    if (f->ndof > kHostDirectMaxDof) return 1;  // too large for the host envelope solver: report the PCG failure
+    // 2: PCG did not converge, 3: its full-pattern K could not be allocated; both fall back below the size
+    // the host envelope solver handles, else report the PCG failure
+    if (f->ndof > kHostDirectMaxDof) return 1;
     f->stats.n_host_fallbacks++;
   }
   std::vector<double> b(f->ndof), x;

@@ -26,7 +26,6 @@ namespace pm {
 namespace {
 
 constexpr double ISQ2 = 0.70710678118654752440;
-enum { ERR_SIGD = 1, ERR_NORMS = 2, ERR_DETJ = 4 };
 
 // material constants, with the products the D coefficients need formed on the host (they then stay in
 // scalar registers instead of occupying vector registers across the persistent loop)
@@ -131,7 +130,7 @@ template <int ND, int NN, int PREC>
 __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) void k_tangent_mfma(
     int64_t nel, int nip, int ipc, int64_t units_per_wg, const int32_t* __restrict__ conn, const uint8_t* __restrict__ rank,
     const double* __restrict__ X, const double* __restrict__ tab, const double* __restrict__ st,
-    typename MfmaT<PREC>::T* __restrict__ ke, MatM mp, int axisym, double thickness, int32_t* err) {
+    typename MfmaT<PREC>::T* __restrict__ ke, MatM mp, int axisym, double thickness, int32_t* err, uint64_t ekey) {
   using S = MShape<ND, NN>;
   using MT = MfmaT<PREC>;
   using T = typename MT::T;
@@ -261,7 +260,8 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         Ji[1][0] = c01 * id; Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id; Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
         Ji[2][0] = c02 * id; Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id; Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
       }
-      if (!(det > 0.0)) *err = ERR_DETJ;
+      const uint64_t ikey = ekey + (uint64_t)((u * EPW + el) * nip + q);
+      if (!(det > 0.0)) raise_error(err, ERR_DETJ, ikey);
       double r = 0.0;
       if (axisym) for (int m = 0; m < NN; m++) r += sN[q * NN + m] * sX[m * ND];
       const double coef = det * sw[q] * thickness * (axisym ? r : 1.0);
@@ -285,7 +285,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         const double sd = sqrt(1.5 * ss);
         const double sdtr = sd + lam * mp.G3;
         const double norm_s = sd * 0.81649658092772603273;  // sqrt(2/3)
-        if (norm_s < 1e-10) *err = ERR_NORMS;
+        if (norm_s < 1e-10) raise_error(err, ERR_NORMS, ikey);
         alpha = mp.G2 * (1.0 - lam * mp.G3 / sdtr);
         gamma = mp.G6G * (lam / sdtr - mp.inv_d) / (norm_s * norm_s);
         beta = mp.K - alpha / 3.0;
@@ -603,7 +603,8 @@ bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
   if constexpr (PREC == 0) ke = f->d_ke + g.ke_base;
   else ke = f->d_ke32 + g.ke_base;
   hipLaunchKernelGGL((k_tangent_mfma<ND, NN, PREC>), dim3((unsigned)grid), dim3(S::NTHR), shm, f->stream, g.nel, g.nip, ipc, upw,
-                     g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err);
+                     g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err,
+                     (uint64_t)(&g - f->groups.data()) << kErrKeyShift);
   return true;
 }
 

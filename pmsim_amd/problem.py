@@ -138,9 +138,9 @@ class Essential:
 
 
 class Natural:
-    """Concentrated point loads and normal distributed loads (Qn) on boundary facets
-    (src/base/natural.rs:40-140): each value is a constant or, with the ``_fn`` forms, ``f(t)``
-    evaluated at the step's time."""
+    """Concentrated point loads and distributed loads (Qn normal, Qx/Qy/Qz directional) on boundary
+    facets (src/base/natural.rs:40-140, src/fem/bc_distributed.rs:156-236): each value is a constant
+    or, with the ``_fn`` forms, ``f(t)`` evaluated at the step's time."""
 
     def __init__(self):
         self.at_points: List[Tuple[int, int, float, Optional[int]]] = []
@@ -337,9 +337,9 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
         pl_table[i, :] = vals  # a constant fills its row
     facets = []
     for (f, nbc, v, fi) in natural.on_facets:
-        if nbc != Nbc.Qn:
-            raise ValueError("only Qn distributed loads are supported on the solid path")
-        facets.append((f.kind, tuple(f.points), f.cell, natural.values(v, fi, times)))
+        if nbc not in (Nbc.Qn, Nbc.Qx, Nbc.Qy, Nbc.Qz):
+            raise ValueError("unknown distributed load")
+        facets.append((f.kind, tuple(f.points), f.cell, natural.values(v, fi, times), int(nbc)))
     return dict(
         ndim=mesh.ndim, coords=mesh.coords, kinds=mesh.kinds, attrs=mesh.attrs,
         cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts, params=params,

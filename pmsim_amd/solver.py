@@ -28,7 +28,9 @@ EXPORTED = [
     "pm_solve", "pm_get_stats", "pm_sizes", "pm_set_ddu", "pm_get_u", "pm_get_f_int", "pm_get_ip_state", "pm_set_ip_state",
     "pm_get_csr", "pm_history_count", "pm_get_history", "pm_time_element_pass", "pm_linear_solve",
     "pm_estimate_device_bytes", "pm_release_device", "pm_serve", "pm_gauss_rule", "pm_extrapolation_matrix",
-    "pm_shape_values", "pm_get_history_loading_factor",
+    "pm_shape_values", "pm_get_history_loading_factor", "pm_add_facet_load", "pm_assemble_f_ext", "pm_get_f_ext",
+    "pm_calculate_residuals_vector", "pm_get_residual", "pm_apply_kk", "pm_get_element_kk", "pm_get_element_f_int",
+    "pm_last_error_location",
 ]
 
 
@@ -87,6 +89,15 @@ def lib():
     L.pm_set_essential.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp, C.c_int32]
     L.pm_set_point_loads.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp, C.c_int32]
     L.pm_add_facet_qn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int64, _dp, C.c_int32]
+    L.pm_add_facet_load.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _ip, C.c_int64, _dp, C.c_int32]
+    L.pm_assemble_f_ext.argtypes = [C.c_void_p, C.c_double]
+    L.pm_get_f_ext.argtypes = [C.c_void_p, _dp]
+    L.pm_calculate_residuals_vector.argtypes = [C.c_void_p, C.c_double]
+    L.pm_get_residual.argtypes = [C.c_void_p, _dp]
+    L.pm_apply_kk.argtypes = [C.c_void_p, _dp, _dp]
+    L.pm_get_element_kk.argtypes = [C.c_void_p, C.c_int64, _dp]
+    L.pm_get_element_f_int.argtypes = [C.c_void_p, C.c_int64, _dp]
+    L.pm_last_error_location.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.pm_initialize.argtypes = [C.c_void_p, C.c_int32]
     for name in ["pm_initialize_internal_values", "pm_reset_algorithmic_variables", "pm_backup_secondary_values",
                  "pm_restore_secondary_values", "pm_update_secondary_values", "pm_assemble_f_int", "pm_assemble_kk",
@@ -182,10 +193,12 @@ class Fem:
         if plt.shape[0]:
             self._check(L.pm_set_point_loads(h, plt.shape[0], np.ascontiguousarray(d["pl_points"], dtype=np.int32),
                                              np.ascontiguousarray(d["pl_dofs"], dtype=np.int32), plt.ravel(), plt.shape[1]))
-        for (fk, pts, cell, v) in d["facets"]:
+        for facet in d["facets"]:
+            fk, pts, cell, v = facet[:4]
+            nbc = facet[4] if len(facet) > 4 else 0
             p = np.asarray(pts, dtype=np.int32)
             v = np.ascontiguousarray(v, dtype=np.float64)
-            self._check(L.pm_add_facet_qn(h, int(fk), len(p), p, int(cell), v, len(v)))
+            self._check(L.pm_add_facet_load(h, int(nbc), int(fk), len(p), p, int(cell), v, len(v)))
         self.device = device
         if initialize:
             self.initialize()
@@ -248,6 +261,50 @@ class Fem:
         f = np.zeros(self.ndof)
         self._check(lib().pm_get_f_int(self.h, f))
         return f
+
+    def assemble_f_ext(self, t):
+        """F_ext at time t (SolverData::assemble_ff_int_and_ff_ext, solver_data.rs:85-108)."""
+        self._check(lib().pm_assemble_f_ext(self.h, float(t)))
+
+    def get_f_ext(self):
+        f = np.zeros(self.ndof)
+        self._check(lib().pm_get_f_ext(self.h, f))
+        return f
+
+    def calculate_residuals_vector(self, loading_factor=1.0):
+        """R = F_int - loading_factor F_ext (solver_data.rs:111-120)."""
+        self._check(lib().pm_calculate_residuals_vector(self.h, float(loading_factor)))
+
+    def get_residual(self):
+        r = np.zeros(self.ndof)
+        self._check(lib().pm_get_residual(self.h, r))
+        return r
+
+    def apply_kk(self, x):
+        """y = K x with the last assembled tangent."""
+        y = np.zeros(self.ndof)
+        self._check(lib().pm_apply_kk(self.h, np.ascontiguousarray(x, dtype=np.float64), y))
+        return y
+
+    def element_kk(self, cell):
+        """Local tangent of a cell (ElementTrait::calc_jacobian), local DOF order ndim*m + d."""
+        n = int(self.d["cell_ptr"][cell + 1] - self.d["cell_ptr"][cell]) * self.ndim
+        k = np.zeros(n * n)
+        self._check(lib().pm_get_element_kk(self.h, int(cell), k))
+        return k.reshape(n, n)
+
+    def element_f_int(self, cell):
+        """Local internal force of a cell (ElementTrait::calc_f_int)."""
+        n = int(self.d["cell_ptr"][cell + 1] - self.d["cell_ptr"][cell]) * self.ndim
+        f = np.zeros(n)
+        self._check(lib().pm_get_element_f_int(self.h, int(cell), f))
+        return f
+
+    def last_error_location(self):
+        """(cell, gauss point) of the first IP that raised the last device-side error, or (-1, -1)."""
+        c, g = C.c_int64(-1), C.c_int32(-1)
+        lib().pm_last_error_location(self.h, C.byref(c), C.byref(g))
+        return c.value, g.value
 
     def get_ip_state(self):
         v = np.zeros(self.nip * (self.ns + 4))

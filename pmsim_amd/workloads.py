@@ -49,13 +49,20 @@ def strip_footing(nx=16, ny=8, ngauss=9, n_inc=20, young=1e4, poisson=0.3, z_ini
 def mixed_compression(n_elements=200_000, n_inc=8, uy=-0.02, lx=100.0, ly=100.0, seed=20261203, jitter=0.1,
                       young=1e4, poisson=0.25, z_ini=100.0):
     """Mixed 2D mesh over lx x ly (30% Qua8, 25% Qua4, 20% Tri6, 15% Qua9, 10% Tri15 by element count,
-    seeded, node jitter), plane strain von Mises E=1e4 nu=0.25 z0=100 H=0 with each kind's default
-    integration rule; bottom edges fixed, top edges uy ramping to `uy` in n_inc increments."""
+    seeded, node jitter), plane strain von Mises E=1e4 nu=0.25 z0=100 H=0; bottom edges fixed, top edges
+    uy ramping to `uy` in n_inc increments.  Each kind has its own attribute so that its integration rule
+    is given explicitly (pmsim's ngauss is per attribute), with point sets that pmsim's own data fix:
+    Qua8 and Qua9 3x3 and Qua4 2x2 Gauss-Legendre, Tri6 the 3-point interior rule of pmsim's Tri3 state
+    files, Tri15 the 12-point rule of Smith & Griffiths example 5.7 (tests/test_solid_smith_5d7_tri15_plane_strain.rs:67)."""
+    from .mesh import QUA4, QUA8, QUA9, TRI6, TRI15
     mesh = mixed_2d(n_elements, lx, ly, seed=seed, jitter=jitter)
+    attr_ng = {QUA8: (1, 9), QUA4: (2, 4), TRI6: (3, 3), QUA9: (4, 9), TRI15: (5, 12)}
+    mesh.attrs = np.asarray([attr_ng[int(k)][0] for k in mesh.kinds], dtype=np.int32)
     feat = Features(mesh)
     bottom = feat.search_edges(At.Y(0.0))
     top = feat.search_edges(At.Y(ly))
-    base = FemBase(mesh, [(1, Elem.Solid(ParamSolid(1.0, StressStrain.VonMises(young, poisson, 0.0, z_ini), None)))])
+    ss = StressStrain.VonMises(young, poisson, 0.0, z_ini)
+    base = FemBase(mesh, [(a, Elem.Solid(ParamSolid(1.0, ss, ng))) for a, ng in sorted(attr_ng.values())])
     ess = Essential()
     ess.edges(bottom, Dof.Ux, 0.0).edges(bottom, Dof.Uy, 0.0)
     ess.edges_fn(top, Dof.Uy, uy, lambda t: t / n_inc)

@@ -1,0 +1,63 @@
+"""Reduced-precision tangents (inexact Newton) on Hex20 against the FP64 oracle.
+
+The precision forms of the 96x96x48 Hex20 cube, at sizes the oracle solves in seconds: the same jittered
+unit cube and material (von Mises E=2e5, nu=0.3, z0=250, H=1e3, 27 IPs), bottom fixed and the top
+pressed down in 10 increments of uz = -0.001 (the loading of the 64x64x32 cube, uz -> -0.01 in 10
+increments), so most IPs end plastic.  The element tangents are formed on the matrix cores in FP32,
+or from BF16 operands with FP32 accumulation; the residual, the return map, the PCG vectors and the
+convergence tests stay FP64, so Newton converges to the FP64 fixed point.
+
+Measured on MI355X (largest relative difference to the oracle over all increments, Newton iterations
+against the oracle's 42 / 43):
+
+| mesh  | tangent | u       | sigma   | epbar   | Newton |
+|-------|---------|---------|---------|---------|--------|
+| 4x4x2 | FP32    | 3.0e-11 | 6.5e-11 | 3.8e-11 | 43     |
+| 4x4x2 | BF16    | 2.5e-9  | 2.8e-9  | 2.6e-9  | 70     |
+| 6x6x3 | FP32    | 8.3e-11 | 8.2e-11 | 8.8e-11 | 44     |
+| 6x6x3 | BF16    | 1.7e-9  | 6.7e-9  | 2.8e-9  | 79     |
+
+Both meet the 1e-8 relative displacement bar; the tests hold u, sigma and epbar to 1e-8, the plastic
+flags to equality, FP32 to at most two extra Newton iterations and BF16 to at most twice the oracle's.
+"""
+import numpy as np
+import pytest
+
+import cases
+from gpu_util import rel_err
+from oracle import oracle as orc
+from pmsim_amd import describe
+from pmsim_amd.solver import Fem
+
+pytestmark = pytest.mark.gpu
+BAR = 1e-8
+
+
+@pytest.mark.parametrize("n", [(4, 4, 2), (6, 6, 3)])
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_hex20_inexact_newton_vs_oracle(prec, n):
+    mesh, base, cfg, ess, nat = cases.hex20_cube(*n, n_inc=10, uz=-0.01)
+    cfg.set_n_max_iterations(60)
+    d = describe(mesh, base, cfg, ess, nat)
+    uo, ipo, its = orc.Oracle(d).solve()
+    cfg.set_tangent_precision(prec).set_lin_sol("pcg")
+    f = Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    f.solve()
+    hist = f.history()
+    ns = f.ns
+    assert len(hist) == len(uo) == 11
+    eu = es = ee = 0.0
+    for k, (t, u, lam, ip) in enumerate(hist):
+        eu = max(eu, rel_err(u, uo[k][: f.ndof]))
+        es = max(es, rel_err(ip[:, :ns], ipo[k][:, :ns]))
+        if k > 0:
+            ee = max(ee, rel_err(ip[:, ns + 3], ipo[k][:, ns + 3]))
+        assert np.array_equal(ip[:, ns + 2], ipo[k][:, ns + 2]), f"plastic flags differ at output {k}"
+    newton, newton_o = f.stats()["n_iterations"], int(np.abs(its).sum())
+    print(f"{prec} {n}: u {eu:.2e} sigma {es:.2e} epbar {ee:.2e} Newton {newton} (oracle {newton_o})", flush=True)
+    assert eu < BAR and es < BAR and ee < BAR
+    assert np.mean(hist[-1][3][:, ns + 2] == 0.0) > 0.9  # plastic almost everywhere at the end
+    if prec == "f32":
+        assert newton <= newton_o + 2
+    else:
+        assert newton_o < newton <= 2 * newton_o  # linear convergence of the rounded Jacobian costs iterations

@@ -149,3 +149,45 @@ def test_batched_pcg_matches_standalone_solve_hex20():
         assert f.stats()["n_pcg_iterations"] == r["n_pcg_iterations"]
         assert np.array_equal(f.get_u()[0], h.get_u()[0])
         assert np.array_equal(f.get_ip_state(), h.get_ip_state())
+
+
+def test_served_iteration_limit_matches_standalone():
+    """A PCG limit that is not a multiple of the engine's 16-iteration chunk (40, reached before convergence):
+    the batched engine stops each system at exactly 40 iterations as the single-request PCG does, both fall
+    back to the host direct solver, and served results equal standalone ones bit for bit."""
+    reqs = _requests(3, 23, nx=(40, 60), ny=(16, 24))
+    for d, cfg, p in reqs:
+        cfg.set_pcg(1e-12, 40).set_pcg_small_max(0)
+    hs = _handles(reqs)
+    res = serve(hs, np.zeros(len(hs)), max_active=3)
+    assert all(r["status"] == 0 for r in res), [r["error"] for r in res]
+    for (d, cfg, p), h, r in zip(reqs, hs, res):
+        f = Fem(d, cfg, keep_history=False)
+        f.solve()
+        st = f.stats()
+        assert st["n_host_fallbacks"] > 0
+        assert st["n_pcg_iterations"] == r["n_pcg_iterations"]
+        assert np.array_equal(f.get_u()[0], h.get_u()[0])
+
+
+def test_served_fp32_tangent_within_bar_of_fp64():
+    """The mixed-precision served form (FP32 element tangents, K held in FP32; residual, return map and PCG in
+    FP64) against the FP64 served form on the same requests, one of them at the top of the size range
+    (256 x 128 Qua8 = 32,768 elements, 198k DOFs): every final displacement within the 1e-8 relative bar."""
+    from pmsim_amd.workloads import strip_footing
+    reqs = _requests(4, 29)
+    mesh, base, cfg, ess, nat = strip_footing(256, 128, n_inc=5, young=2e4, poisson=0.3, z_ini=150.0, width=1.5)
+    cfg.set_scaled_rr(1e-10).set_pcg(1e-12, 50000)
+    reqs.append((describe(mesh, base, cfg, ess, nat), cfg, None))
+    u = {}
+    for prec in ("f64", "f32"):
+        for d, c, p in reqs:
+            c.set_tangent_precision(prec)
+        hs = _handles(reqs)
+        res = serve(hs, np.zeros(len(hs)), max_active=5)
+        assert all(r["status"] == 0 for r in res), [r["error"] for r in res]
+        u[prec] = [h.get_u()[0] for h in hs]
+    assert hs[-1].ndof == 2 * (2 * 256 + 1) * (2 * 128 + 1) - 2 * 256 * 128
+    errs = [rel_err(b, a) for a, b in zip(u["f64"], u["f32"])]
+    print("served FP32-tangent vs FP64 u:", " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) < 1e-8

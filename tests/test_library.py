@@ -107,7 +107,8 @@ def test_natural_load_tables():
     mesh, base, cfg, ess, nat = cases.spo_751_pres_cylin()
     d = describe(mesh, base, cfg, ess, nat)
     assert len(d["facets"]) == 3  # three Qua4 edges on the inner circle r = 100
-    for (_, pts, _, vals) in d["facets"]:
+    for (_, pts, _, vals, nbc) in d["facets"]:
+        assert nbc == 0  # Qn
         assert np.allclose(np.hypot(*mesh.coords[list(pts)].T), 100.0)
         assert np.array_equal(vals, -np.asarray(cases.P_751))
     nat.point(0, 0, 2.5).points_fn([1, 2], 1, lambda t: 10.0 * t)
