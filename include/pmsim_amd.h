@@ -47,7 +47,7 @@ typedef struct pm_config {
   int32_t pcg_max_iter;             /* default 20000 */
   int32_t keep_history;             /* store u and IP state of every output step on the host */
   double tol_rr_scaled;             /* > 0 adds ||R||inf <= tol_rr_scaled * ||F_int||inf as a convergence test */
-  int32_t tangent_kernel;           /* 0: default per element kind, 1: node-block vector kernel (FP64 arithmetic),
+  int32_t tangent_kernel;           /* 0: default (vector kernel for FP64 / FP32, matrix cores for BF16), 1: node-block vector kernel (FP64 arithmetic),
                                        2: matrix-core (MFMA) kernel in the tangent precision, 3: tune at pm_initialize
                                        (fastest measured; FP64 variants agree to rounding, not bitwise) */
   int32_t use_graphs;               /* 1 (default): element passes and 16-iteration PCG chunks replay captured HIP graphs */

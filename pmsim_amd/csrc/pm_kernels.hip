@@ -422,37 +422,44 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
         hA[u] = AX ? Pa[ND] : 0.0;
         if (AX) tIa[u][0] += hA[u];
       }
+      // the s(x)s term only where the IP is plastic (gamma = 0 at elastic IPs, so skipping it adds nothing):
+      // a branch that is uniform over the lanes of one element and IP
+      auto blocks = [&](auto plc) {
+        constexpr bool PL = decltype(plc)::value;
 #pragma unroll
-      for (int v = 0; v < 2; v++) {
-        const double* Pb = Pq + (2 * J + v) * PW;
-        double ub[ND], vb[ND], wb[ND];
-#pragma unroll
-        for (int i = 0; i < ND; i++) {
-          ub[i] = ha * Pb[i];
-          vb[i] = be * Pb[i];
-          wb[i] = ga * Pb[ND + 1 + i];
-        }
-        double hh = 0.0;
-        if (AX) { const double hb = Pb[ND]; vb[0] += be * hb; hh = 2.0 * ha * hb; }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int v = 0; v < 2; v++) {
+          const double* Pb = Pq + (2 * J + v) * PW;
+          double ub[ND], vb[ND], wb[ND];
 #pragma unroll
           for (int i = 0; i < ND; i++) {
-            Dg[u][v] += Ga[u][i] * ub[i];
-            // three chained accumulations: one FMA each into the accumulator (a parenthesised sum
-            // would cost a multiply, two FMAs and an add)
-#pragma unroll
-            for (int j = 0; j < ND; j++) {
-              double k = K[u][v][i][j];
-              k += ub[i] * Ga[u][j];
-              k += tIa[u][i] * vb[j];
-              k += ta[u][i] * wb[j];
-              K[u][v][i][j] = k;
-            }
+            ub[i] = ha * Pb[i];
+            vb[i] = be * Pb[i];
+            wb[i] = PL ? ga * Pb[ND + 1 + i] : 0.0;
           }
-          if (AX) K[u][v][0][0] += hA[u] * hh;
+          double hh = 0.0;
+          if (AX) { const double hb = Pb[ND]; vb[0] += be * hb; hh = 2.0 * ha * hb; }
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+#pragma unroll
+            for (int i = 0; i < ND; i++) {
+              Dg[u][v] += Ga[u][i] * ub[i];
+              // chained accumulations: one FMA each into the accumulator (a parenthesised sum would cost
+              // a multiply, two FMAs and an add)
+#pragma unroll
+              for (int j = 0; j < ND; j++) {
+                double k = K[u][v][i][j];
+                k += ub[i] * Ga[u][j];
+                k += tIa[u][i] * vb[j];
+                if (PL) k += ta[u][i] * wb[j];
+                K[u][v][i][j] = k;
+              }
+            }
+            if (AX) K[u][v][0][0] += hA[u] * hh;
+          }
         }
-      }
+      };
+      if (ga != 0.0) blocks(std::true_type{});
+      else blocks(std::false_type{});
     }
   };
   // the chunk loop is specialised as a whole (the branch on the uniform axisym flag sits outside it, so
@@ -463,9 +470,12 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
     __syncthreads();  // J^-1 and D ready (first chunk); the previous chunk's node vectors consumed
     // node vectors of every (element, IP of the chunk, node); the padding node of odd NN stays zero.
     // One item at a time: the tile accumulators are live here
+    const int per_el = nq * NNP;
 #pragma unroll 1
-    for (int w = tid; w < epb * nq * NNP; w += blockDim.x) {
-      const int e = w / (nq * NNP), rem = w - e * nq * NNP, ql = rem / NNP, m = rem - ql * NNP;
+    for (int w = tid; w < epb * per_el; w += blockDim.x) {
+      int e = 0;  // w / per_el without a division by a run-time value (epb is small)
+      while (w >= (e + 1) * per_el) e++;
+      const int rem = w - e * per_el, ql = rem / NNP, m = rem - ql * NNP;
       double* P = sP + e * ES + (ql * NNP + m) * PW;
       if (e0 + e >= nel || m >= NN) {
 #pragma unroll
@@ -713,12 +723,12 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
                                       group_key(f, g)));
 }
 
-// tangent kernel of a group: the tuned choice, else the default (matrix cores for reduced precision)
+// tangent kernel of a group: the tuned choice, else the default (matrix cores for BF16, the vector kernel otherwise)
 static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   const int prec = f->cfg.tangent_precision;
   if (prec == 2) return 2;  // BF16 runs on the matrix cores only
   if (g.tune_kernel) return g.tune_kernel;
-  return prec == 0 ? 1 : 2;
+  return 1;  // FP64 and FP32: the node-block vector kernel (measured faster than the FP32 matrix-core kernel)
 }
 
 void launch_tangent(pm_fem* f, Group& g) {
