@@ -195,6 +195,15 @@ int pm_time_element_pass(pm_fem* fem, int32_t reps, double* ms /* [update, resid
 /* Linear solve of the current K with a given right-hand side on the free DOFs (testing and
  * benchmarking the PCG): x = K_ff^-1 b_f, prescribed entries of x are zero. */
 int pm_linear_solve(pm_fem* fem, const double* b, double* x, int32_t* iterations);
+/* The same solve for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); it has no counterpart in pmsim.  b and x are device pointers of
+ * ndof doubles owned by the caller.  Allocates nothing: the PCG work vectors, partial sums and scalars live
+ * in one fixed 64 MiB device array of the library (taken per device at pm_initialize), so systems are
+ * limited to 2,000,000 DOFs (larger ones are refused with an error); every call on a device shares that
+ * array, so calls that overlap on one device race.  Block-Jacobi PCG over the stored upper-block K with
+ * eager launches (no graph, no full-pattern copy of K) and no host fallback: a solve that does not reach
+ * pcg_rtol within pcg_max_iter returns "PCG did not converge" with x as it stands. */
+int pm_linear_solve_noalloc(pm_fem* fem, const double* d_b, double* d_x, int32_t* iterations);
 
 /* ---- serving: many independent solves on one GPU ------------------------------------------------
  * pmsim solves one problem per SolverImplicit::new + solve (src/fem/solver_implicit.rs:27-178); the

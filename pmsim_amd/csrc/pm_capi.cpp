@@ -209,6 +209,7 @@ int pm_initialize(pm_fem* f, int32_t device) {
   if (build_groups(f) || build_patterns(f)) return 1;
   assemble_f_ext(f, 0);
   if (device_alloc(f)) { device_free(f); return 1; }
+  noalloc_setup(f->device);  // the no-allocation entry point's scratch address on this device
   f->initialized = true;
   return tune_tangent(f);
 }

@@ -270,6 +270,11 @@ int arc_alloc(pm_fem* f);
 int full_pattern_alloc(pm_fem* f);
 // linear solvers
 int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters);
+// PCG over the stored upper-block K in caller-given work memory (work 4 ndof, red 3 x 2048, scal 4); allocates nothing
+int pcg_solve_sym(pm_fem* f, const double* d_b, double* d_x, double* work, double* red, double* scal, int* iters);
+// takes the device address of the no-allocation entry point's scratch array (pm_noalloc.hip) on `device`
+// (current); called by pm_initialize
+void noalloc_setup(int device);
 int host_direct_solve(pm_fem* f, const std::vector<double>& b, std::vector<double>& x);
 void build_block_jacobi(pm_fem* f);
 void dev_spmv(pm_fem* f, const double* x, double* y, bool masked);

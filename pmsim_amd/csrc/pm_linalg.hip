@@ -402,6 +402,25 @@ __global__ void k_pcg_init_fin(int npart, const double* __restrict__ part_rz, co
   if (threadIdx.x == 0) { *rz_out = a; *bb_out = b; }
 }
 
+// K1 over the stored upper blocks (the lower ones read through their mirrors; no full-pattern copy):
+// q = (A p) * free, one thread per row component; partial p.q per workgroup
+template <int BS, typename VT>
+__global__ __launch_bounds__(PCG_TPB) void k_pcg_q_sym(int64_t np, KSym<VT> K, const double* __restrict__ free_,
+                                                       const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part) {
+  __shared__ double sh[PCG_TPB];
+  double acc = 0.0;
+  const int64_t n = np * BS;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = t / BS;
+    const int i = (int)(t - a * BS);
+    const double sv = ksym_row<BS, VT>(K, a, i, p) * free_[t];
+    q[t] = sv;
+    acc += p[t] * sv;
+  }
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // ---- single-workgroup persistent PCG for small systems: the whole loop runs in one launch ----------
 constexpr int SMALL_TPB = 1024;
 
@@ -646,6 +665,65 @@ int pcg_solve(pm_fem* f, const double* d_b, double* d_x, int* iters) {
       it++;
     }
     if (it % check_every == 0 || it == f->cfg.pcg_max_iter) {
+      PM_HIP(hipMemcpyAsync(&rr, rr_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
+      PM_HIP(hipStreamSynchronize(f->stream));
+      if (!std::isfinite(rr)) { f->err = "Found NaN or Inf in mdu"; *iters = it; return 1; }
+      if (rr <= tol2) break;
+    }
+  }
+  *iters = it;
+  f->stats.n_pcg_iterations += it;
+  if (rr > tol2) { f->err = "PCG did not converge"; return 2; }
+  return 0;
+}
+
+// PCG on the free DOFs over the stored (upper-block) K with caller-given work memory: r, z, p, q in work
+// (4 ndof), partials in red (3 x 2048), scalars in scal (4).  Fixed grids, eager launches (no graph), a
+// convergence check every 16 iterations and at the limit; no device memory is allocated.
+int pcg_solve_sym(pm_fem* f, const double* d_b, double* d_x, double* work, double* red, double* scal, int* iters) {
+  const int64_t np = f->npoint, n = np * f->bs;
+  double* r = work;
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  const bool lowp = f->cfg.tangent_precision != 0 && f->d_kval32;
+  const double rtol2 = f->cfg.pcg_rtol * f->cfg.pcg_rtol;
+  const int rb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (np + PCG_TPB - 1) / PCG_TPB));
+  const int rq = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (n + PCG_TPB - 1) / PCG_TPB));
+  double* pa = red;         // p.q partials (rq)
+  double* pb = red + 2048;  // r.z partials (rb)
+  double* pc = red + 4096;  // r.r partials (rb)
+  double* rzs = scal;       // rz slots [0], [1]; rr [2]; bb [3]
+  double* rr_s = scal + 2;
+  double* bb_s = scal + 3;
+  *iters = 0;
+  if (f->bs == 2) hipLaunchKernelGGL((k_pcg_init<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, pb, pc);
+  else hipLaunchKernelGGL((k_pcg_init<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, d_b, f->d_free, f->d_binv, d_x, r, z, p, pb, pc);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TPB), 0, f->stream, rb, pb, pc, rzs, bb_s);
+  double bb = 0.0;
+  PM_HIP(hipMemcpyAsync(&bb, bb_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  PM_HIP(hipStreamSynchronize(f->stream));
+  if (bb == 0.0) return 0;
+  if (!std::isfinite(bb)) { f->err = "Found NaN or Inf"; return 1; }
+  const double tol2 = rtol2 * bb;
+  int it = 0;
+  double rr = bb;
+  while (it < f->cfg.pcg_max_iter) {
+    const double* rz_old = rzs + (it & 1);
+    double* rz_new = rzs + ((it + 1) & 1);
+#define PM_QSYM(BS) do { if (lowp) hipLaunchKernelGGL((k_pcg_q_sym<BS, float>), dim3(rq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval32), f->d_free, p, q, pa); \
+                         else hipLaunchKernelGGL((k_pcg_q_sym<BS, double>), dim3(rq), dim3(PCG_TPB), 0, f->stream, np, ksym(f, f->d_kval), f->d_free, p, q, pa); } while (0)
+    if (f->bs == 2) {
+      PM_QSYM(2);
+      hipLaunchKernelGGL((k_pcg_xrz<2>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
+    } else {
+      PM_QSYM(3);
+      hipLaunchKernelGGL((k_pcg_xrz<3>), dim3(rb), dim3(PCG_TPB), 0, f->stream, np, rq, pa, rz_old, p, q, f->d_free, f->d_binv, d_x, r, z, pb, pc);
+    }
+#undef PM_QSYM
+    hipLaunchKernelGGL(k_pcg_p2, dim3(rb), dim3(PCG_TPB), 0, f->stream, n, rb, pb, pc, rz_old, rz_new, rr_s, z, p);
+    it++;
+    if (it % 16 == 0 || it == f->cfg.pcg_max_iter) {
       PM_HIP(hipMemcpyAsync(&rr, rr_s, sizeof(double), hipMemcpyDeviceToHost, f->stream));
       PM_HIP(hipStreamSynchronize(f->stream));
       if (!std::isfinite(rr)) { f->err = "Found NaN or Inf in mdu"; *iters = it; return 1; }

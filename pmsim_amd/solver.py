@@ -30,7 +30,7 @@ EXPORTED = [
     "pm_estimate_device_bytes", "pm_release_device", "pm_serve", "pm_gauss_rule", "pm_extrapolation_matrix",
     "pm_shape_values", "pm_get_history_loading_factor", "pm_add_facet_load", "pm_assemble_f_ext", "pm_get_f_ext",
     "pm_calculate_residuals_vector", "pm_get_residual", "pm_apply_kk", "pm_get_element_kk", "pm_get_element_f_int",
-    "pm_last_error_location",
+    "pm_last_error_location", "pm_linear_solve_noalloc",
 ]
 
 
