@@ -235,6 +235,26 @@ int pm_release_device(pm_fem* fem);
 int pm_serve(int32_t n, pm_fem* const* fems, const double* arrival_s, int32_t device, int64_t hbm_budget_bytes,
              int32_t max_active, pm_request_stats* out);
 
+/* The same serving loop with open submission: requests are submitted while others run and polled for.
+ * A request is a configured, not yet initialized handle; it is queued at pm_server_submit (t_arrival), admitted
+ * in submission order while fewer than max_active run and the footprint estimates of the running ones fit
+ * hbm_budget_bytes (a lone request always runs), then initialized, solved (pm_solve) and released on a worker
+ * thread with its own stream.  The element passes (update, residual gather, tangent, K gather) of all running
+ * requests are launched together, one launch per kernel class over the union of their element groups, and
+ * so are their multi-workgroup PCG iterations; each request's results are bitwise those of its own pm_solve.
+ * Times in the records are seconds since pm_server_create.  pm_server_destroy waits for every submitted
+ * request; the handles stay the caller's (results through pm_get_u / pm_get_ip_state / pm_get_stats). */
+typedef struct pm_server pm_server;
+int pm_server_create(pm_server** out, int32_t device, int64_t hbm_budget_bytes, int32_t max_active);
+int pm_server_submit(pm_server* server, pm_fem* fem, int64_t* ticket /* 0, 1, ... in submission order */);
+/* 0 and the record once the request is done (check out->status), 1 while it is queued or running, -1 for an
+ * unknown ticket. */
+int pm_server_poll(pm_server* server, int64_t ticket, pm_request_stats* out);
+int pm_server_wait(pm_server* server, int64_t ticket, pm_request_stats* out);
+/* Batching so far: out[0] = element-pass batches launched, out[1] = request passes they carried. */
+int pm_server_stats(pm_server* server, int64_t* out /* [2] */);
+int pm_server_destroy(pm_server* server);
+
 #ifdef __cplusplus
 }
 #endif

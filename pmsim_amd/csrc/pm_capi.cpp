@@ -11,6 +11,10 @@ using namespace pm;
 
 static thread_local std::string g_last_error;
 
+namespace pm {
+void set_global_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace pm
+
 #define CHECK_HANDLE(f)                                                        \
   do {                                                                         \
     if (!(f)) { g_last_error = "null handle"; return 1; }                      \

@@ -52,6 +52,7 @@ struct ParamSolid {
 
 // Per-kind element batch on the device.
 struct PcgEngine;
+struct PassEngine;
 
 // Element matrices are stored as their upper node blocks (a <= b in rank order: nodes sorted by global
 // id), row after row: block (a, b) starts at packed_block(nn, a, b) * nd * nd, each block row-major.
@@ -203,6 +204,7 @@ struct pm_fem {
   int32_t err_gauss = -1;
   int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
   pm::PcgEngine* engine = nullptr;     // batched PCG of a serving run (set by pm_serve), else per handle
+  pm::PassEngine* pass_engine = nullptr;  // batched element passes of a serving run (set by the server)
   hipGraphExec_t pass_graph[8] = {};   // captured element passes, by (update, residual, tangent)
   hipGraphExec_t pcg_graph = nullptr;  // captured chunk of PCG iterations
   const double* pcg_graph_x = nullptr; // solution vector the chunk was captured for
@@ -240,6 +242,17 @@ bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 int tune_tangent(pm_fem* f);
 // drops every captured graph (configuration, tuning or buffers changed)
 void release_graphs(pm_fem* f);
+// element passes of several handles in shared launches (serving): what each handle asks for
+struct PassJob { pm_fem* f; bool update = false, do_update = false, residual = false, tangent = false; };
+struct PassBatchBuf { void* d = nullptr; size_t cap = 0; std::vector<char> host; };  // staged descriptors and maps
+// runs the jobs' passes on `stream` (returns after the stream is idle); 0 on success
+int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& jobs);
+PassEngine* pass_engine_create(int device);
+void pass_engine_destroy(PassEngine* e);
+// hands the handle's pass to the engine and waits for it (the handle's stream must be idle)
+int pass_engine_run(PassEngine* e, const PassJob& job);
+// the handle's element-pass pieces, through its pass engine when it has one, else on its own stream
+int run_pass(pm_fem* f, bool update, bool do_update, bool residual, bool tangent);
 // batched PCG engine shared by the requests of one serving run
 PcgEngine* engine_create(int device);
 void engine_destroy(PcgEngine* e);
@@ -249,6 +262,7 @@ void launch_init_state(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);
 void launch_copy(pm_fem* f, double* dst, const double* src, int64_t n);
 int check_device_error(pm_fem* f);
+void set_global_error(const std::string& msg);  // pm_global_last_error's message (calls without a handle)
 void reset_device_error(pm_fem* f);
 size_t update_shared_bytes(const Group& g);
 size_t tangent_shared_bytes(const Group& g);

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -114,17 +116,43 @@ __device__ __forceinline__ int64_t xcd_block() {
   return x * per + (x < rem ? x : rem) + i;
 }
 
+// kernel arguments of one element group (or one handle, for the gathers)
+struct UpdArgs {
+  int64_t nel; int nip, epb;
+  const int32_t* conn; const double* X; const double* ddu; const double* tab; const double* st_in;
+  double* st_out; double* fe; Mat mp; int axisym; double thickness; int do_update; int32_t* err; uint64_t ekey;
+};
+template <typename KT>
+struct TanArgs {
+  int64_t nel; int nip, epb, ch;
+  const int32_t* conn; const uint8_t* rank; const double* X; const double* tab; const double* st; KT* ke;
+  Mat mp; int axisym; double thickness; int32_t* err; uint64_t ekey;
+};
+template <typename KT>
+struct ScatArgs { int64_t nnzb; const int32_t* korder; const int32_t* kptr; const int32_t* kmap; const KT* ke; double* kval; float* kval32; };
+struct RgArgs { int64_t np; const int32_t* fptr; const int32_t* fmap; const double* fe; double* fint; };
+
 // ------------------------------------------------------------------------------------------------
 // update kernel: gather (coords, ddu) -> strain increment -> stress update from the backup state ->
 // element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
 // Shape tables are read through the vector L1 (they are a few KB and shared by all workgroups).
 // ------------------------------------------------------------------------------------------------
 template <int ND, int NN>
-__global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, const int32_t* __restrict__ conn,
-                                                const double* __restrict__ X, const double* __restrict__ ddu,
-                                                const double* __restrict__ tab, const double* __restrict__ st_in,
-                                                double* __restrict__ st_out, double* __restrict__ fe, Mat mp, int axisym,
-                                                double thickness, int do_update, int32_t* err, uint64_t ekey) {
+__device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
+  const int64_t nel = A.nel;
+  const int nip = A.nip, epb = A.epb;
+  const int32_t* __restrict__ conn = A.conn;
+  const double* __restrict__ X = A.X;
+  const double* __restrict__ ddu = A.ddu;
+  const double* __restrict__ tab = A.tab;
+  const double* __restrict__ st_in = A.st_in;
+  double* __restrict__ st_out = A.st_out;
+  double* __restrict__ fe = A.fe;
+  const Mat mp = A.mp;
+  const int axisym = A.axisym, do_update = A.do_update;
+  const double thickness = A.thickness;
+  int32_t* err = A.err;
+  const uint64_t ekey = A.ekey;
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int SW = NS + ND * ND + 1;  // per IP: c*sigma, J^-1, radius
   extern __shared__ double sh[];
@@ -135,7 +163,7 @@ __global__ __launch_bounds__(TPB) void k_update(int64_t nel, int nip, int epb, c
   double* sU = sX + epb * NN * ND;
   double* sS = sU + epb * NN * ND;
   const int tid = threadIdx.x;
-  const int64_t e0 = xcd_block() * epb;
+  const int64_t e0 = blk * epb;
   for (int i = tid; i < epb * NN; i += blockDim.x) {
     int64_t e = e0 + i / NN;
     if (e < nel) {
@@ -276,11 +304,20 @@ struct TanShape {
 };
 
 template <int ND, int NN, typename KT>
-__global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, int ch, const int32_t* __restrict__ conn,
-                                                 const uint8_t* __restrict__ rank, const double* __restrict__ X,
-                                                 const double* __restrict__ tab, const double* __restrict__ st,
-                                                 KT* __restrict__ ke, Mat mp, int axisym, double thickness, int32_t* err,
-                                                 uint64_t ekey) {
+__device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) {
+  const int64_t nel = A.nel;
+  const int nip = A.nip, epb = A.epb, ch = A.ch;
+  const int32_t* __restrict__ conn = A.conn;
+  const uint8_t* __restrict__ rank = A.rank;
+  const double* __restrict__ X = A.X;
+  const double* __restrict__ tab = A.tab;
+  const double* __restrict__ st = A.st;
+  KT* __restrict__ ke = A.ke;
+  const Mat mp = A.mp;
+  const int axisym = A.axisym;
+  const double thickness = A.thickness;
+  int32_t* err = A.err;
+  const uint64_t ekey = A.ekey;
   using TS = TanShape<ND, NN>;
   constexpr int NS = ND == 2 ? 4 : 6;
   constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH;
@@ -300,7 +337,7 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
   double* sP = sQ + ((epb * nip * QW + (int)(sQ - sh)) % 2 == 0 ? epb * nip * QW : epb * nip * QW + 1);
   __shared__ uint8_t srank[2 * TPB];  // epb * NN <= TPB * NN / NT <= 4/3 TPB
   const int tid = threadIdx.x;
-  const int64_t e0 = xcd_block() * epb;
+  const int64_t e0 = blk * epb;
   for (int i = tid; i < nip; i += blockDim.x) sw[i] = tab[i];
   for (int i = tid; i < nip * NN * ND; i += blockDim.x) { const int q = i / (NN * ND); sdN[q * DS + (i - q * NN * ND)] = tab[nip + nip * NN + i]; }
   if (axisym) for (int i = tid; i < nip * NN; i += blockDim.x) sN[i] = tab[nip + i];
@@ -561,13 +598,18 @@ __global__ __launch_bounds__(TPB) void k_tangent(int64_t nel, int nip, int epb, 
 // entry is the value offset of a stored (upper, rank-order) element block; with the sign bit set the
 // block would be read transposed (the maps for the upper triangle never set it).
 template <int BS, typename KT>
-__global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __restrict__ korder, const int32_t* __restrict__ kptr,
-                                                 const int32_t* __restrict__ kmap, const KT* __restrict__ ke, double* __restrict__ kval,
-                                                 float* __restrict__ kval32) {
+__device__ __forceinline__ void scatter_body(int64_t blk_order, const ScatArgs<KT>& A) {
+  const int64_t nnzb = A.nnzb;
+  const int32_t* __restrict__ korder = A.korder;
+  const int32_t* __restrict__ kptr = A.kptr;
+  const int32_t* __restrict__ kmap = A.kmap;
+  const KT* __restrict__ ke = A.ke;
+  double* __restrict__ kval = A.kval;
+  float* __restrict__ kval32 = A.kval32;
   constexpr int B2 = BS * BS;
   constexpr int PF = 2;  // contributions whose map entries and blocks are in flight together
   __shared__ double stage[TPB * B2];
-  const int64_t b0 = (int64_t)korder[xcd_block()] * blockDim.x;  // chunks in their processing order
+  const int64_t b0 = (int64_t)korder[blk_order] * blockDim.x;  // chunks in their processing order
   const int64_t blk = b0 + threadIdx.x;
   double acc[B2];
 #pragma unroll
@@ -619,15 +661,56 @@ __global__ __launch_bounds__(TPB) void k_scatter(int64_t nnzb, const int32_t* __
 
 // deterministic gather of element forces into the global internal force vector (assembly.rs:38-46)
 template <int ND>
-__global__ __launch_bounds__(TPB) void k_rgather(int64_t np, const int32_t* __restrict__ fptr, const int32_t* __restrict__ fmap,
-                                                 const double* __restrict__ fe, double* __restrict__ fint) {
-  const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= np * ND) return;
+__device__ __forceinline__ void rgather_body(int64_t blk, const RgArgs& A) {
+  const int32_t* __restrict__ fptr = A.fptr;
+  const int32_t* __restrict__ fmap = A.fmap;
+  const double* __restrict__ fe = A.fe;
+  double* __restrict__ fint = A.fint;
+  const int64_t t = blk * blockDim.x + threadIdx.x;
+  if (t >= A.np * ND) return;
   const int64_t a = t / ND;
   const int d = (int)(t - a * ND);
   double s = 0.0;
   for (int32_t k = fptr[a]; k < fptr[a + 1]; k++) s += fe[(int64_t)fmap[k] * ND + d];
   fint[t] = s;
+}
+
+// one launch per kernel for a handle (workgroups remapped per XCD), and batched launches over the element
+// groups of several handles: map[blockIdx.x] = job << 20 | workgroup of that job (the per-element and per-row
+// arithmetic is the same either way, so are the bits)
+__device__ __forceinline__ void unpack_job(int32_t e, int& job, int& w) { job = e >> 20; w = e & ((1 << 20) - 1); }
+
+template <int ND, int NN>
+__global__ __launch_bounds__(TPB) void k_update(UpdArgs A) { update_body<ND, NN>(xcd_block(), A); }
+template <int ND, int NN>
+__global__ __launch_bounds__(TPB) void kb_update(const UpdArgs* __restrict__ jobs, const int32_t* __restrict__ map) {
+  int j, w;
+  unpack_job(map[blockIdx.x], j, w);
+  update_body<ND, NN>(w, jobs[j]);
+}
+template <int ND, int NN, typename KT>
+__global__ __launch_bounds__(TPB) void k_tangent(TanArgs<KT> A) { tangent_body<ND, NN, KT>(xcd_block(), A); }
+template <int ND, int NN, typename KT>
+__global__ __launch_bounds__(TPB) void kb_tangent(const TanArgs<KT>* __restrict__ jobs, const int32_t* __restrict__ map) {
+  int j, w;
+  unpack_job(map[blockIdx.x], j, w);
+  tangent_body<ND, NN, KT>(w, jobs[j]);
+}
+template <int BS, typename KT>
+__global__ __launch_bounds__(TPB) void k_scatter(ScatArgs<KT> A) { scatter_body<BS, KT>(xcd_block(), A); }
+template <int BS, typename KT>
+__global__ __launch_bounds__(TPB) void kb_scatter(const ScatArgs<KT>* __restrict__ jobs, const int32_t* __restrict__ map) {
+  int j, w;
+  unpack_job(map[blockIdx.x], j, w);
+  scatter_body<BS, KT>(w, jobs[j]);
+}
+template <int ND>
+__global__ __launch_bounds__(TPB) void k_rgather(RgArgs A) { rgather_body<ND>(xcd_block(), A); }
+template <int ND>
+__global__ __launch_bounds__(TPB) void kb_rgather(const RgArgs* __restrict__ jobs, const int32_t* __restrict__ map) {
+  int j, w;
+  unpack_job(map[blockIdx.x], j, w);
+  rgather_body<ND>(w, jobs[j]);
 }
 
 __global__ void k_init_state(int64_t nipg, int ns, double* st, double z0, int vm) {
@@ -710,17 +793,23 @@ static void split_tangent(int nd, int nn, int nip, bool axisym, int& epb, int& c
     case HEX20: { constexpr int ND = 3, NN = 20; __VA_ARGS__; } break; \
   }
 
+static UpdArgs upd_args(pm_fem* f, Group& g, bool do_update, int epb) {
+  UpdArgs A;
+  A.nel = g.nel; A.nip = g.nip; A.epb = epb;
+  A.conn = g.d_conn; A.X = f->d_X; A.ddu = f->d_ddu; A.tab = g.d_tab;
+  A.st_in = do_update ? g.d_backup : g.d_state; A.st_out = g.d_state; A.fe = f->d_fe + g.fe_base;
+  A.mp = mat_of(f, g); A.axisym = f->axisym ? 1 : 0; A.thickness = f->thickness; A.do_update = do_update ? 1 : 0;
+  A.err = f->d_err; A.ekey = group_key(f, g);
+  return A;
+}
+
 void launch_update(pm_fem* f, Group& g, bool do_update) {
   if (g.nel == 0) return;
   const int epb = epb_update(g.ndim, g.nnode, g.nip);
   const int64_t grid = (g.nel + epb - 1) / epb;
   const size_t shm = shm_update(g.ndim, g.nnode, g.nip, epb);
-  const Mat mp = mat_of(f, g);
-  PM_DISPATCH_KIND(g.kind, ND, NN,
-                   hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip, epb,
-                                      g.d_conn, f->d_X, f->d_ddu, g.d_tab, do_update ? g.d_backup : g.d_state, g.d_state,
-                                      f->d_fe + g.fe_base, mp, f->axisym ? 1 : 0, f->thickness, do_update ? 1 : 0, f->d_err,
-                                      group_key(f, g)));
+  const UpdArgs A = upd_args(f, g, do_update, epb);
+  PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
 }
 
 // tangent kernel of a group: the tuned choice, else the default (matrix cores for BF16, the vector kernel otherwise)
@@ -731,29 +820,38 @@ static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   return 1;  // FP64 and FP32: the node-block vector kernel (measured faster than the FP32 matrix-core kernel)
 }
 
+// work split of a group's vector tangent: the tuned one, else the heuristic
+static void tangent_split(const pm_fem* f, const Group& g, int& epb, int& ch) {
+  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, ch);
+  if (g.tune_epb > 0) { epb = g.tune_epb; ch = g.tune_ch; }
+}
+
+template <typename KT>
+static TanArgs<KT> tan_args(pm_fem* f, Group& g, int epb, int ch, KT* ke) {
+  TanArgs<KT> A;
+  A.nel = g.nel; A.nip = g.nip; A.epb = epb; A.ch = ch;
+  A.conn = g.d_conn; A.rank = g.d_rank; A.X = f->d_X; A.tab = g.d_tab; A.st = g.d_state; A.ke = ke + g.ke_base;
+  A.mp = mat_of(f, g); A.axisym = f->axisym ? 1 : 0; A.thickness = f->thickness; A.err = f->d_err; A.ekey = group_key(f, g);
+  return A;
+}
+
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
   if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
   int epb, ch;
-  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, ch);
-  if (g.tune_epb > 0) { epb = g.tune_epb; ch = g.tune_ch; }
+  tangent_split(f, g, epb, ch);
   const int64_t grid = (g.nel + epb - 1) / epb;
   const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym);
-  const Mat mp = mat_of(f, g);
   if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) {
     f->launch_refused = true;  // reported by check_device_error; never launched out of range
     return;
   }
   if (f->cfg.tangent_precision == 0) {
-    PM_DISPATCH_KIND(g.kind, ND, NN,
-                     hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp,
-                                        f->axisym ? 1 : 0, f->thickness, f->d_err, group_key(f, g)));
+    const TanArgs<double> A = tan_args(f, g, epb, ch, f->d_ke);
+    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
   } else {
-    PM_DISPATCH_KIND(g.kind, ND, NN,
-                     hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, g.nel, g.nip,
-                                        epb, ch, g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp,
-                                        f->axisym ? 1 : 0, f->thickness, f->d_err, group_key(f, g)));
+    const TanArgs<float> A = tan_args(f, g, epb, ch, f->d_ke32);
+    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
   }
 }
 
@@ -812,24 +910,174 @@ int tune_tangent(pm_fem* f) {
 
 static_assert(TPB == GATHER_CHUNK, "the gather's chunk order assumes one workgroup of TPB blocks per chunk");
 // the gather forms the stored (upper) node blocks of K only
+template <typename KT>
+static ScatArgs<KT> scat_args(const pm_fem* f, const KT* ke) {
+  ScatArgs<KT> A;
+  A.nnzb = f->nnzu; A.korder = f->d_korder; A.kptr = f->d_kmap_ptr; A.kmap = f->d_kmap; A.ke = ke;
+  A.kval = f->cfg.tangent_precision == 0 ? f->d_kval : nullptr;
+  A.kval32 = f->cfg.tangent_precision == 0 ? nullptr : f->d_kval32;
+  return A;
+}
+
+static RgArgs rg_args(const pm_fem* f) {
+  RgArgs A;
+  A.np = f->npoint; A.fptr = f->d_fmap_ptr; A.fmap = f->d_fmap; A.fe = f->d_fe; A.fint = f->d_fint;
+  return A;
+}
+
 void launch_scatter(pm_fem* f) {
   const int64_t n = f->nnzu;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
   const bool lowp = f->cfg.tangent_precision != 0;
-  if (f->bs == 2) {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+  if (!lowp) {
+    const ScatArgs<double> A = scat_args(f, (const double*)f->d_ke);
+    if (f->bs == 2) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    else hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
   } else {
-    if (!lowp) hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke, f->d_kval, (float*)nullptr);
-    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, n, f->d_korder, f->d_kmap_ptr, f->d_kmap, f->d_ke32, (double*)nullptr, f->d_kval32);
+    const ScatArgs<float> A = scat_args(f, (const float*)f->d_ke32);
+    if (f->bs == 2) hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
   }
 }
 
 void launch_residual_gather(pm_fem* f) {
   const int64_t n = f->npoint * f->ndim;
   const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
-  if (f->ndim == 2) hipLaunchKernelGGL((k_rgather<2>), dim3(grid), dim3(TPB), 0, f->stream, f->npoint, f->d_fmap_ptr, f->d_fmap, f->d_fe, f->d_fint);
-  else hipLaunchKernelGGL((k_rgather<3>), dim3(grid), dim3(TPB), 0, f->stream, f->npoint, f->d_fmap_ptr, f->d_fmap, f->d_fe, f->d_fint);
+  const RgArgs A = rg_args(f);
+  if (f->ndim == 2) hipLaunchKernelGGL((k_rgather<2>), dim3(grid), dim3(TPB), 0, f->stream, A);
+  else hipLaunchKernelGGL((k_rgather<3>), dim3(grid), dim3(TPB), 0, f->stream, A);
+}
+
+// ---- element passes of several handles in shared launches (serving) ---------------------------------
+// Every kernel class (kernel, element kind and rule, work split, precision) of the requested passes becomes
+// one launch over the union of the handles' groups; the jobs' descriptors and workgroup maps are staged in
+// one device buffer.  Order: all updates, all residual gathers, all tangents, all K gathers.
+namespace {
+
+template <typename A>
+struct Cls {
+  std::vector<A> args;
+  std::vector<int32_t> map;
+  size_t shm = 0;
+  size_t off_args = 0, off_map = 0;  // offsets in the staged buffer
+  bool add(const A& a, int64_t nblk) {  // false when the job or its workgroups do not fit a map entry
+    const int32_t j = (int32_t)args.size();
+    if (j >= (1 << 11) || nblk >= (1 << 20)) return false;
+    args.push_back(a);
+    for (int64_t w = 0; w < nblk; w++) map.push_back((j << 20) | (int32_t)w);
+    return true;
+  }
+};
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& jobs) {
+  using UK = std::tuple<int, int, int>;                  // kind, nip, epb
+  using TK = std::tuple<int, int, int, int, int>;        // kind, nip, epb, ch, axisym
+  std::map<UK, Cls<UpdArgs>> upd;
+  std::map<int, Cls<RgArgs>> rg;
+  std::map<TK, Cls<TanArgs<double>>> tan64;
+  std::map<TK, Cls<TanArgs<float>>> tan32;
+  std::map<int, Cls<ScatArgs<double>>> sc64;
+  std::map<int, Cls<ScatArgs<float>>> sc32;
+  std::vector<std::pair<pm_fem*, Group*>> mfma;           // groups whose tangent runs on the matrix cores
+  for (PassJob& J : jobs) {
+    pm_fem* f = J.f;
+    for (auto& g : f->groups) {
+      if (g.nel == 0) continue;
+      if (J.update) {
+        const int epb = epb_update(g.ndim, g.nnode, g.nip);
+        auto& c = upd[UK{g.kind, g.nip, epb}];
+        c.shm = shm_update(g.ndim, g.nnode, g.nip, epb);
+        if (!c.add(upd_args(f, g, J.do_update, epb), (g.nel + epb - 1) / epb)) return 1;
+      }
+      if (J.tangent) {
+        if (tangent_kernel_of(f, g) == 2) { mfma.push_back({f, &g}); continue; }
+        if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) { f->launch_refused = true; continue; }
+        int epb, ch;
+        tangent_split(f, g, epb, ch);
+        const TK key{g.kind, g.nip, epb, ch, f->axisym ? 1 : 0};
+        const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym);
+        const int64_t nblk = (g.nel + epb - 1) / epb;
+        if (f->cfg.tangent_precision == 0) { auto& c = tan64[key]; c.shm = shm; if (!c.add(tan_args(f, g, epb, ch, f->d_ke), nblk)) return 1; }
+        else { auto& c = tan32[key]; c.shm = shm; if (!c.add(tan_args(f, g, epb, ch, f->d_ke32), nblk)) return 1; }
+      }
+    }
+    if (J.residual && !rg[f->ndim].add(rg_args(f), (f->npoint * f->ndim + TPB - 1) / TPB)) return 1;
+    if (J.tangent) {
+      const int64_t nblk = (f->nnzu + TPB - 1) / TPB;
+      if (f->cfg.tangent_precision == 0 ? !sc64[f->bs].add(scat_args(f, (const double*)f->d_ke), nblk)
+                                        : !sc32[f->bs].add(scat_args(f, (const float*)f->d_ke32), nblk)) return 1;
+    }
+  }
+  // stage every class's descriptors and map in one upload
+  size_t total = 0;
+  auto place = [&](auto& classes) {
+    for (auto& kv : classes) {
+      auto& c = kv.second;
+      c.off_args = total; total += align256(c.args.size() * sizeof(c.args[0]));
+      c.off_map = total; total += align256(c.map.size() * sizeof(int32_t));
+    }
+  };
+  place(upd); place(rg); place(tan64); place(tan32); place(sc64); place(sc32);
+  if (total > buf->cap) {
+    if (buf->d) { (void)hipStreamSynchronize(stream); (void)hipFree(buf->d); buf->d = nullptr; buf->cap = 0; }
+    if (hipMalloc(&buf->d, total) != hipSuccess) { buf->d = nullptr; return 1; }
+    buf->cap = total;
+  }
+  buf->host.resize(total);
+  auto copy_in = [&](auto& classes) {
+    for (auto& kv : classes) {
+      auto& c = kv.second;
+      std::memcpy(buf->host.data() + c.off_args, c.args.data(), c.args.size() * sizeof(c.args[0]));
+      std::memcpy(buf->host.data() + c.off_map, c.map.data(), c.map.size() * sizeof(int32_t));
+    }
+  };
+  copy_in(upd); copy_in(rg); copy_in(tan64); copy_in(tan32); copy_in(sc64); copy_in(sc32);
+  char* d = static_cast<char*>(buf->d);
+  if (total && hipMemcpyAsync(d, buf->host.data(), total, hipMemcpyHostToDevice, stream) != hipSuccess) return 1;
+  auto dargs = [&](auto& c) { return reinterpret_cast<decltype(c.args.data())>(d + c.off_args); };
+  auto dmap = [&](auto& c) { return reinterpret_cast<const int32_t*>(d + c.off_map); };
+  for (auto& kv : upd) {
+    auto& c = kv.second;
+    PM_DISPATCH_KIND(std::get<0>(kv.first), ND, NN,
+                     hipLaunchKernelGGL((kb_update<ND, NN>), dim3((unsigned)c.map.size()), dim3(TPB), c.shm, stream, dargs(c), dmap(c)));
+  }
+  for (auto& kv : rg) {
+    auto& c = kv.second;
+    if (kv.first == 2) hipLaunchKernelGGL((kb_rgather<2>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+    else hipLaunchKernelGGL((kb_rgather<3>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+  }
+  for (auto& kv : tan64) {
+    auto& c = kv.second;
+    PM_DISPATCH_KIND(std::get<0>(kv.first), ND, NN,
+                     hipLaunchKernelGGL((kb_tangent<ND, NN, double>), dim3((unsigned)c.map.size()), dim3(TPB), c.shm, stream, dargs(c), dmap(c)));
+  }
+  for (auto& kv : tan32) {
+    auto& c = kv.second;
+    PM_DISPATCH_KIND(std::get<0>(kv.first), ND, NN,
+                     hipLaunchKernelGGL((kb_tangent<ND, NN, float>), dim3((unsigned)c.map.size()), dim3(TPB), c.shm, stream, dargs(c), dmap(c)));
+  }
+  for (auto& mg : mfma) {  // one launch per group, on the batch's stream (the handle's owner waits for the batch)
+    hipStream_t own = mg.first->stream;
+    mg.first->stream = stream;
+    (void)launch_tangent_mfma(mg.first, *mg.second, mg.first->cfg.tangent_precision);
+    mg.first->stream = own;
+  }
+  for (auto& kv : sc64) {
+    auto& c = kv.second;
+    if (kv.first == 2) hipLaunchKernelGGL((kb_scatter<2, double>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+    else hipLaunchKernelGGL((kb_scatter<3, double>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+  }
+  for (auto& kv : sc32) {
+    auto& c = kv.second;
+    if (kv.first == 2) hipLaunchKernelGGL((kb_scatter<2, float>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+    else hipLaunchKernelGGL((kb_scatter<3, float>), dim3((unsigned)c.map.size()), dim3(TPB), 0, stream, dargs(c), dmap(c));
+  }
+  if (hipGetLastError() != hipSuccess) return 1;
+  return hipStreamSynchronize(stream) == hipSuccess ? 0 : 1;  // the staged descriptors stay valid until here
 }
 
 void launch_init_state(pm_fem* f, Group& g) {

@@ -66,6 +66,27 @@ int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
   return 0;
 }
 
+// The pieces of an element pass the driver asks for: the update kernel (stress update from the backup when
+// do_update, else f_e of the current state), the residual gather, the tangent and its gather.  Served
+// requests hand them to the run's pass engine, which launches the passes of all running requests together;
+// otherwise they run on the handle's stream (the update combinations as captured graphs).
+int run_pass(pm_fem* f, bool update, bool do_update, bool residual, bool tangent) {
+  if (f->pass_engine) {
+    PM_HIP(hipStreamSynchronize(f->stream));  // the engine's stream reads what this stream wrote
+    PassJob job;
+    job.f = f; job.update = update; job.do_update = do_update; job.residual = residual; job.tangent = tangent;
+    if (pass_engine_run(f->pass_engine, job)) { f->err = "batched element pass failed"; return 1; }
+    return 0;
+  }
+  if (update) return element_pass(f, do_update, residual, tangent);
+  if (residual) launch_residual_gather(f);
+  if (tangent) {
+    for (auto& g : f->groups) launch_tangent(f, g);
+    launch_scatter(f);
+  }
+  return 0;
+}
+
 int copy_ip_state_to_host(const pm_fem* f, double* v) {
   const int nv = f->nsig + 4;
   for (const auto& g : f->groups) {
@@ -193,7 +214,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
   dev_fill(f, f->d_u, 0.0, n);
   dev_fill(f, f->d_ddu, 0.0, n);
   if (npr) dev_fill(f, f->d_lam, 0.0, npr);
-  if (element_pass(f, false, false, false)) return 1;  // f_e of the initial state
+  if (run_pass(f, true, false, false, false)) return 1;  // f_e of the initial state
   if (save_output(f)) return 1;
   double t_out = f->t + cfg.dt_out;
   // work vectors
@@ -248,7 +269,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
     converged = false;
     for (int it = 0; it < cfg.n_max_iterations; it++) {
       f->stats.n_iterations++;
-      launch_residual_gather(f);
+      if (run_pass(f, false, false, true, false)) return 1;
       dev_residual(f, g, f->ell);
       double norm_rr = dev_max_abs(f, f->d_rr, n, nullptr);
       if (lmm && npr) norm_rr = std::max(norm_rr, dev_max_abs(f, g, npr, nullptr));
@@ -273,8 +294,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
       }
       if (conv_rr) { converged = true; break; }
       if (it == 0 || !cfg.constant_tangent) {
-        for (auto& gr : f->groups) launch_tangent(f, gr);
-        launch_scatter(f);
+        if (run_pass(f, false, false, false, true)) return 1;
         if (check_device_error(f)) return 1;
         if (cfg.lin_sol == 0) build_block_jacobi(f);
       }
@@ -328,7 +348,7 @@ int newton_solve(pm_fem* f, int32_t* n_converged) {
       }
       // backup at the first iteration; later iterations update from the backup (= restore)
       if (it == 0 || cfg.linear_problem) for (auto& gr : f->groups) launch_copy(f, gr.d_backup, gr.d_state, gr.nel * gr.nip * (f->nsig + 4));
-      if (element_pass(f, true, false, false)) return 1;
+      if (run_pass(f, true, true, false, false)) return 1;
       if (check_device_error(f)) return 1;
       if (cfg.linear_problem) { converged = true; break; }
       if (!arc && it == cfg.n_max_iterations - 1) { f->err = "Newton-Raphson did not converge"; return 1; }

@@ -30,7 +30,8 @@ EXPORTED = [
     "pm_estimate_device_bytes", "pm_release_device", "pm_serve", "pm_gauss_rule", "pm_extrapolation_matrix",
     "pm_shape_values", "pm_get_history_loading_factor", "pm_add_facet_load", "pm_assemble_f_ext", "pm_get_f_ext",
     "pm_calculate_residuals_vector", "pm_get_residual", "pm_apply_kk", "pm_get_element_kk", "pm_get_element_f_int",
-    "pm_last_error_location", "pm_linear_solve_noalloc",
+    "pm_last_error_location", "pm_linear_solve_noalloc", "pm_server_create", "pm_server_submit", "pm_server_poll",
+    "pm_server_wait", "pm_server_destroy", "pm_server_stats",
 ]
 
 
@@ -123,6 +124,12 @@ def lib():
     L.pm_estimate_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.pm_release_device.argtypes = [C.c_void_p]
     L.pm_serve.argtypes = [C.c_int32, C.POINTER(C.c_void_p), _dp, C.c_int32, C.c_int64, C.c_int32, C.c_void_p]
+    L.pm_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int64, C.c_int32]
+    L.pm_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+    L.pm_server_poll.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    L.pm_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    L.pm_server_destroy.argtypes = [C.c_void_p]
+    L.pm_server_stats.argtypes = [C.c_void_p, _lp]
     _lib = L
     return L
 
@@ -447,3 +454,59 @@ def serve(fems, arrivals, device=0, hbm_budget_bytes=200 << 30, max_active=16):
             f.refresh_sizes()
         res.append(r)
     return res
+
+
+class Server:
+    """Open-submission serving loop (pm_server_*): submit configured, not yet initialized handles while others
+    run; poll or wait for each.  Element passes and PCG iterations of the running requests are batched."""
+
+    def __init__(self, device=0, hbm_budget_bytes=200 << 30, max_active=16):
+        self.h = C.c_void_p()
+        if lib().pm_server_create(C.byref(self.h), int(device), int(hbm_budget_bytes), int(max_active)) != 0:
+            raise PmError(lib().pm_global_last_error().decode())
+        self.fems = {}
+
+    def submit(self, fem):
+        t = C.c_int64(-1)
+        if lib().pm_server_submit(self.h, fem.h, C.byref(t)) != 0:
+            raise PmError(lib().pm_last_error(fem.h).decode())
+        self.fems[t.value] = fem
+        return t.value
+
+    def _record(self, ticket, o):
+        f = self.fems[ticket]
+        r = {k: getattr(o, k) for (k, _) in PmRequestStats._fields_}
+        r["error"] = lib().pm_last_error(f.h).decode() if o.status != 0 else ""
+        if o.status == 0:
+            f.refresh_sizes()
+        return r
+
+    def poll(self, ticket):
+        """The request's record once done, else None."""
+        o = PmRequestStats()
+        rc = lib().pm_server_poll(self.h, int(ticket), C.cast(C.pointer(o), C.c_void_p))
+        if rc < 0:
+            raise PmError("unknown ticket")
+        return self._record(ticket, o) if rc == 0 else None
+
+    def wait(self, ticket):
+        o = PmRequestStats()
+        if lib().pm_server_wait(self.h, int(ticket), C.cast(C.pointer(o), C.c_void_p)) != 0:
+            raise PmError("unknown ticket")
+        return self._record(ticket, o)
+
+    def stats(self):
+        v = np.zeros(2, dtype=np.int64)
+        lib().pm_server_stats(self.h, v)
+        return {"pass_batches": int(v[0]), "pass_jobs": int(v[1])}
+
+    def close(self):
+        if self.h:
+            lib().pm_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

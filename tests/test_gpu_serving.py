@@ -1,4 +1,4 @@
-"""Serving: many independent footing solves admitted concurrently on one GPU (pm_serve).
+"""Serving: many independent footing solves admitted concurrently on one GPU (pm_serve, pm_server_*).
 
 Each request's result must be bitwise the same whether it runs alone or in the mix (its kernels, their
 order and its stream are its own), admission must keep the sum of running footprints within the HBM
@@ -10,7 +10,7 @@ import pytest
 from gpu_util import rel_err
 from oracle import oracle as orc
 from pmsim_amd import describe
-from pmsim_amd.solver import Fem, serve
+from pmsim_amd.solver import Fem, Server, serve
 from pmsim_amd.workloads import footing_request
 
 pytestmark = pytest.mark.gpu
@@ -191,3 +191,38 @@ def test_served_fp32_tangent_within_bar_of_fp64():
     errs = [rel_err(b, a) for a, b in zip(u["f64"], u["f32"])]
     print("served FP32-tangent vs FP64 u:", " ".join(f"{e:.2e}" for e in errs), flush=True)
     assert max(errs) < 1e-8
+
+
+def test_open_submission_batches_passes_and_matches_standalone():
+    """pm_server_*: requests submitted while earlier ones run (the first is not done right after its submission),
+    the element passes of running requests share launches (fewer batches than passes), and every result is
+    bitwise the one of its own pm_solve (the shared launches do the same per-element arithmetic)."""
+    import time
+    reqs = _requests(8, 31, nx=(16, 60), ny=(8, 30))
+    hs = _handles(reqs)
+    with Server(max_active=8) as srv:
+        tickets = [srv.submit(h) for h in hs[:4]]
+        assert srv.poll(tickets[0]) is None
+        time.sleep(0.05)
+        tickets += [srv.submit(h) for h in hs[4:]]  # arrive while the first ones run
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert all(r["status"] == 0 for r in res), [r["error"] for r in res]
+    assert all(r["t_admit"] >= r["t_arrival"] and r["t_done"] > r["t_admit"] for r in res)
+    assert st["pass_jobs"] > st["pass_batches"] > 0, st
+    for (d, cfg, p), h, r in zip(reqs, hs, res):
+        f = Fem(d, cfg, keep_history=False)
+        f.solve()
+        assert f.stats()["n_iterations"] == r["n_iterations"]
+        assert np.array_equal(f.get_u()[0], h.get_u()[0])
+        assert np.array_equal(f.get_ip_state(), h.get_ip_state())
+
+
+def test_server_refuses_initialized_handles_and_unknown_tickets():
+    reqs = _requests(1, 37)
+    f = Fem(reqs[0][0], reqs[0][1], keep_history=False)  # initialized
+    with Server() as srv:
+        with pytest.raises(Exception, match="not yet initialized"):
+            srv.submit(f)
+        with pytest.raises(Exception, match="unknown ticket"):
+            srv.poll(5)
