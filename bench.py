@@ -391,8 +391,8 @@ def main():
     ap.add_argument("--kernel", default="default", choices=["default", "vector", "mfma", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="hex20: leave out the served comparison")
-    ap.add_argument("--serve-requests", type=int, default=20, help="hex20: requests of the served comparison")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="hex20: Poisson rate of the served comparison (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=24, help="hex20: requests of the served comparison")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="hex20: Poisson rate of the served comparison (1/s)")
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision", "pcg"])
     ap.add_argument("--pcg-iters", type=int, default=1600, help="pcg: PCG iterations per timed solve")
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")

@@ -164,20 +164,28 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   double* sS = sU + epb * NN * ND;
   const int tid = threadIdx.x;
   const int64_t e0 = blk * epb;
+  const int64_t nipg = nel * nip;
+  // one IP per thread (epb * nip <= blockDim); its state is requested before the gather so that the loads
+  // are in flight while the coordinates and displacements arrive and the strain is formed
+  const int el = tid / nip, q = tid - el * nip;
+  const int64_t e = e0 + el;
+  const bool mine = tid < epb * nip && e < nel;
+  const int64_t ip = mine ? e * nip + q : 0;
+  double sig[NS];
+#pragma unroll
+  for (int c = 0; c < NS; c++) sig[c] = mine ? st_in[c * nipg + ip] : 0.0;
+  double z = mine ? st_in[NS * nipg + ip] : 0.0, lam = mine ? st_in[(NS + 1) * nipg + ip] : 0.0;
+  double el_ = mine ? st_in[(NS + 2) * nipg + ip] : 0.0, ep = mine ? st_in[(NS + 3) * nipg + ip] : 0.0;
   for (int i = tid; i < epb * NN; i += blockDim.x) {
-    int64_t e = e0 + i / NN;
-    if (e < nel) {
+    int64_t e1 = e0 + i / NN;
+    if (e1 < nel) {
       int64_t p = conn[e0 * NN + i];
 #pragma unroll
       for (int d = 0; d < ND; d++) { sX[i * ND + d] = X[p * ND + d]; sU[i * ND + d] = ddu[p * ND + d]; }
     }
   }
   __syncthreads();
-  const int64_t nipg = nel * nip;
-  for (int w = tid; w < epb * nip; w += blockDim.x) {
-    const int el = w / nip, q = w - el * nip;
-    const int64_t e = e0 + el;
-    if (e >= nel) continue;
+  if (mine) {
     const double* dN = tdN + q * NN * ND;
     const double* Xe = sX + el * NN * ND;
     const double* Ue = sU + el * NN * ND;
@@ -212,12 +220,6 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
     double de[NS];
     de[0] = exx; de[1] = eyy; de[2] = ezz; de[3] = (exy * 0.5) * SQ2;
     if constexpr (NS == 6) { de[4] = (eyz * 0.5) * SQ2; de[5] = (exz * 0.5) * SQ2; }
-    const int64_t ip = e * nip + q;
-    double sig[NS];
-#pragma unroll
-    for (int c = 0; c < NS; c++) sig[c] = st_in[c * nipg + ip];
-    double z = st_in[NS * nipg + ip], lam = st_in[(NS + 1) * nipg + ip], el_ = st_in[(NS + 2) * nipg + ip];
-    double ep = st_in[(NS + 3) * nipg + ip];
     if (do_update) {
       stress_update<NS>(mp, sig, z, lam, el_, ep, de, err, ekey + (uint64_t)ip);
 #pragma unroll
@@ -229,8 +231,9 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
     }
     double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
     double* S = sS + (el * nip + q) * SW;
+    // c sigma, the Mandel shear components already divided by sqrt 2 (the tensor components)
 #pragma unroll
-    for (int c = 0; c < NS; c++) S[c] = coef * sig[c];
+    for (int c = 0; c < NS; c++) S[c] = c < 3 ? coef * sig[c] : (coef * sig[c]) * ISQ2;
 #pragma unroll
     for (int i = 0; i < ND; i++)
 #pragma unroll
@@ -257,14 +260,14 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
         G[j] = s;
       }
       if constexpr (ND == 2) {
-        double sxy = S[3] * ISQ2;
+        const double sxy = S[3];
         fm[0] += S[0] * G[0];
         fm[0] += sxy * G[1];
         if (axisym) fm[0] += S[2] * tN[q * NN + m] / S[NS + 4];
         fm[1] += sxy * G[0];
         fm[1] += S[1] * G[1];
       } else {
-        double sxy = S[3] * ISQ2, syz = S[4] * ISQ2, sxz = S[5] * ISQ2;
+        const double sxy = S[3], syz = S[4], sxz = S[5];
         // one FMA per term into the accumulators
         fm[0] += S[0] * G[0]; fm[0] += sxy * G[1]; fm[0] += sxz * G[2];
         fm[1] += sxy * G[0]; fm[1] += S[1] * G[1]; fm[1] += syz * G[2];
@@ -756,7 +759,7 @@ static size_t shm_update(int nd, int nn, int nip, int epb) {
   return sizeof(double) * (2 * (size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
 }
 static int epb_update(int nd, int nn, int nip) {
-  int epb = std::max(1, TPB / nip);
+  int epb = std::max(1, TPB / nip);  // the update kernel takes one IP per thread: epb * nip <= TPB (nip <= 27 here)
   while (epb > 1 && shm_update(nd, nn, nip, epb) > SHM_CAP) epb--;
   return epb;
 }
