@@ -66,6 +66,8 @@ int device_alloc(pm_fem* f) {
     if (dalloc(f, &g.d_rank, (int64_t)rank.size()) || upload(f, g.d_rank, rank.data(), (int64_t)rank.size())) return 1;
     if (dalloc(f, &g.d_tab, (int64_t)tab.size()) || upload(f, g.d_tab, tab.data(), (int64_t)tab.size())) return 1;
     if (dalloc(f, &g.d_state, nipg * (ns + 4)) || dalloc(f, &g.d_backup, nipg * (ns + 4))) return 1;
+    if (dalloc(f, &g.d_geo, nipg * (nd * nd + 2))) return 1;
+    launch_geometry(f, g);  // the mesh is fixed: J^-1 and det J of every IP are formed once
     PM_HIP(hipStreamSynchronize(f->stream));  // host vectors go out of scope
   }
   // patterns and maps
@@ -153,7 +155,7 @@ int arc_alloc(pm_fem* f) {
 void device_free(pm_fem* f) {
   release_graphs(f);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
-  for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = nullptr; }
+  for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); fr(g.d_geo); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = g.d_geo = nullptr; }
   fr(f->d_X); fr(f->d_urp); fr(f->d_ucol); fr(f->d_lrp); fr(f->d_lcol); fr(f->d_lidx); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);

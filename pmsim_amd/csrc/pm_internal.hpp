@@ -80,12 +80,13 @@ struct Group {
   int32_t* d_conn = nullptr;       // nel*nnode
   uint8_t* d_rank = nullptr;       // nel*nnode: rank of each local node among the cell's global ids
   double* d_tab = nullptr;         // [w (nip) | N (nip*nnode) | dN (nip*nnode*ndim)]
+  double* d_geo = nullptr;         // [nel*nip][ndim*ndim + 2]: J^-1, det J and radius (axisymmetric) of each IP
   double* d_state = nullptr;       // SoA: [nsig+4][nel*nip]  (sigma..., z, lambda, elastic, epbar)
   double* d_backup = nullptr;
   int64_t fe_base = 0;             // offset (in doubles) of this group's f_e in the element force buffer
   int64_t ke_base = 0;             // offset (in values) of this group's element matrices
   int kew = 0;                     // values per element matrix: packed upper node blocks, padded (packed_ke_values)
-  int tune_epb = 0, tune_ch = 0;   // tangent work split chosen by the tuner: elements per workgroup, IPs per chunk (0: heuristic)
+  int tune_ch = 0;                 // IPs per chunk of the vector tangent chosen by the tuner (0: heuristic)
   int tune_kernel = 0;             // tangent kernel chosen at initialize: 1 node-block vector, 2 matrix core
 };
 
@@ -259,13 +260,13 @@ void engine_destroy(PcgEngine* e);
 int engine_solve(PcgEngine* e, pm_fem* f, const double* d_b, double* d_x, int* iters);
 void launch_scatter(pm_fem* f);
 void launch_init_state(pm_fem* f, Group& g);
+void launch_geometry(pm_fem* f, Group& g);
 void launch_reset_lambda(pm_fem* f, Group& g);
 void launch_copy(pm_fem* f, double* dst, const double* src, int64_t n);
 int check_device_error(pm_fem* f);
 void set_global_error(const std::string& msg);  // pm_global_last_error's message (calls without a handle)
 void reset_device_error(pm_fem* f);
 size_t update_shared_bytes(const Group& g);
-size_t tangent_shared_bytes(const Group& g);
 // vector kernels (pm_kernels.hip)
 double dev_max_abs(pm_fem* f, const double* x, int64_t n, const double* mask);
 double dev_max_rel(pm_fem* f, const double* x, const double* x0, int64_t n);

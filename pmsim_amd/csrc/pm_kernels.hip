@@ -124,9 +124,10 @@ struct UpdArgs {
 };
 template <typename KT>
 struct TanArgs {
-  int64_t nel; int nip, epb, ch;
-  const int32_t* conn; const uint8_t* rank; const double* X; const double* tab; const double* st; KT* ke;
+  int64_t nel; int nip, ch;
+  const uint8_t* rank; const double* geo; const double* tab; const double* st; KT* ke;
   Mat mp; int axisym; double thickness; int32_t* err; uint64_t ekey;
+  int cap;  // output staging per pass, in KT values (a multiple of 4)
 };
 template <typename KT>
 struct ScatArgs { int64_t nnzb; const int32_t* korder; const int32_t* kptr; const int32_t* kmap; const KT* ke; double* kval; float* kval32; };
@@ -286,13 +287,17 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
 // with tI = B^T I, ts = B^T s, h = N/r (axisymmetric hoop term): three rank-1 terms and a diagonal per
 // node block (33 FMA per block and IP instead of the dense 6x6 triple product).
 //
-// Work split per workgroup: epb elements x NT upper 2x2 tiles of node blocks, one thread per tile.  The
-// J, J^-1 and D coefficients of every IP of the workgroup's elements are formed first; the IPs are
-// then taken in chunks of ch: the node vectors (G, h, ts) of a chunk are staged in LDS and every thread
-// adds the chunk's contributions to its tile in registers, so no reduction across threads is needed.
-// The element matrices are finally staged in LDS one element at a time, as their upper node blocks in
-// rank order (nodes sorted by global id; packed_block), and written contiguously.
+// Work split: one wavefront per workgroup (TW threads), EPW elements x NT upper 2x2 tiles of node blocks,
+// one thread per tile, so no phase waits on another wavefront.  The D coefficients of every IP of the
+// workgroup's elements are formed first from the state and the stored J^-1 and det J; the IPs are then
+// taken in chunks of ch: the node vectors (G, h, ts) of a chunk are staged in LDS and every thread adds
+// the chunk's contributions to its tile in registers, so no reduction across threads is needed.  The
+// element matrices, contiguous in memory, are finally staged in LDS (over the dead IP data, in at most a
+// few passes) as their upper node blocks in rank order (nodes sorted by global id; packed_block) and
+// written with 16-byte stores.
 // ------------------------------------------------------------------------------------------------
+constexpr int TW = 64;  // threads per tangent workgroup: one wavefront
+
 template <int ND>
 struct NodeVec { static constexpr int W = 2 * ND + 1; };  // G(ND), h, ts(ND): odd stride in dwords/2 -> no LDS bank conflicts
 
@@ -302,17 +307,21 @@ struct TanShape {
   static constexpr int NT = NH * (NH + 1) / 2;
   static constexpr int NNP = 2 * NH;
   static constexpr int PW = NodeVec<ND>::W;
-  static constexpr int QW = 3 + ND * ND + 6 + 1;  // alpha/2, beta, gamma (times c), J^-1 (J first), s, r
+  static constexpr int QW = 3 + ND * ND + 6 + 1;  // alpha/2, beta, gamma (times c), J^-1, s, r
   static constexpr int B2 = ND * ND;
+  static constexpr int EPW = TW / NT;             // elements per workgroup
+  static_assert(NT <= TW, "one element's tiles fit one wavefront");
 };
+
+// elements per tangent workgroup of a kind with nn nodes (TanShape::EPW)
+inline int tangent_epw(int nn) { const int nh = (nn + 1) / 2; return TW / (nh * (nh + 1) / 2); }
 
 template <int ND, int NN, typename KT>
 __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) {
   const int64_t nel = A.nel;
-  const int nip = A.nip, epb = A.epb, ch = A.ch;
-  const int32_t* __restrict__ conn = A.conn;
+  const int nip = A.nip, ch = A.ch, cap = A.cap;
   const uint8_t* __restrict__ rank = A.rank;
-  const double* __restrict__ X = A.X;
+  const double* __restrict__ geo = A.geo;
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ st = A.st;
   KT* __restrict__ ke = A.ke;
@@ -323,70 +332,35 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
   const uint64_t ekey = A.ekey;
   using TS = TanShape<ND, NN>;
   constexpr int NS = ND == 2 ? 4 : 6;
-  constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH;
-  constexpr int DS = NN * ND + 1;  // padded stride of one IP's dN rows in LDS
+  constexpr int NT = TS::NT, NNP = TS::NNP, PW = TS::PW, QW = TS::QW, B2 = TS::B2, NH = TS::NH, EPW = TS::EPW;
+  constexpr int GW = ND * ND + 2;               // per IP in geo: J^-1, det J, radius
   constexpr int EW = packed_ke_values(NN, ND);  // values per stored element matrix (padded)
   constexpr int EU = NN * (NN + 1) / 2 * B2;    // values in use
   extern __shared__ __attribute__((aligned(16))) double sh[];
-  double* sw = sh;                       // nip weights
-  double* sdN = sw + nip;                // nip * DS
-  double* sN = sdN + nip * DS;           // nip * NN (axisymmetric only)
-  double* sX = sN + (axisym ? nip * NN : 0);
-  double* sQ = sX + epb * NN * ND;
-  // node vectors of one IP chunk, ES doubles per element (odd, so that the reads of the two elements a
-  // wave straddles fall in disjoint LDS bank pairs); at the end the output staging (16-byte aligned
-  // for the copy out)
+  double* sQ = sh;                              // EPW * nip * QW: per-IP coefficients
+  // node vectors of one IP chunk, ES doubles per element (odd, so that the reads of neighbouring elements
+  // fall in disjoint LDS bank pairs)
   const int ES = (ch * NNP * PW) | 1;
-  double* sP = sQ + ((epb * nip * QW + (int)(sQ - sh)) % 2 == 0 ? epb * nip * QW : epb * nip * QW + 1);
-  __shared__ uint8_t srank[2 * TPB];  // epb * NN <= TPB * NN / NT <= 4/3 TPB
+  double* sP = sQ + EPW * nip * QW;
+  __shared__ uint8_t srank[EPW * NN];
   const int tid = threadIdx.x;
-  const int64_t e0 = blk * epb;
-  for (int i = tid; i < nip; i += blockDim.x) sw[i] = tab[i];
-  for (int i = tid; i < nip * NN * ND; i += blockDim.x) { const int q = i / (NN * ND); sdN[q * DS + (i - q * NN * ND)] = tab[nip + nip * NN + i]; }
-  if (axisym) for (int i = tid; i < nip * NN; i += blockDim.x) sN[i] = tab[nip + i];
-  const double* tw = sw;
-  const double* tN = sN;
-  for (int i = tid; i < epb * NN; i += blockDim.x) {
-    int64_t e = e0 + i / NN;
-    if (e < nel) {
-      int64_t p = conn[e0 * NN + i];
-      srank[i] = rank[e0 * NN + i];
-#pragma unroll
-      for (int d = 0; d < ND; d++) sX[i * ND + d] = X[p * ND + d];
-    }
-  }
-  __syncthreads();
-  // J entries in parallel: J[i][j] = sum_m X_m[i] dN_m[j]
-  for (int w = tid; w < epb * nip * B2; w += blockDim.x) {
-    const int el = w / (nip * B2), rem = w - el * nip * B2, q = rem / B2, ij = rem - q * B2;
-    if (e0 + el >= nel) continue;
-    const int i = ij / ND, j = ij - i * ND;
-    const double* Xe = sX + el * NN * ND;
-    const double* dN = sdN + q * DS;
-    double s = 0.0;
-#pragma unroll 4
-    for (int m = 0; m < NN; m++) s += Xe[m * ND + i] * dN[m * ND + j];
-    sQ[(el * nip + q) * QW + 3 + ij] = s;
-  }
-  __syncthreads();
+  const int64_t e0 = blk * EPW;
+  const double* __restrict__ tN = tab + nip;      // shape functions and derivatives, read through the L1
+  const double* __restrict__ tdN = tN + nip * NN;
+  for (int i = tid; i < EPW * NN; i += TW)
+    if (e0 + i / NN < nel) srank[i] = rank[e0 * NN + i];
   const int64_t nipg = nel * nip;
-  // per IP: J^-1 in place, D coefficients (von_mises.rs:124-167), deviator
-  for (int w = tid; w < epb * nip; w += blockDim.x) {
-    const int el = w / nip, q = w - el * nip;
+  // per IP: D coefficients (von_mises.rs:124-167), deviator, J^-1 (formed once for the fixed mesh)
+  for (int w = tid; w < EPW * nip; w += TW) {
+    const int el = EPW == 1 ? 0 : w / nip, q = w - el * nip;
     const int64_t e = e0 + el;
     if (e >= nel) continue;
-    double* Q = sQ + (el * nip + q) * QW;
-    double J[ND][ND], Ji[ND][ND], det;
-#pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) J[i][j] = Q[3 + i * ND + j];
-    if constexpr (ND == 2) inv_jac2(J, Ji, det); else inv_jac3(J, Ji, det);
-    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)(e * nip + q));
-    double r = 0.0;
-    if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * sX[(el * NN + m) * ND];
-    double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
+    double* Q = sQ + w * QW;
     const int64_t ip = e * nip + q;
+    const double* gq = geo + ip * GW;
+    const double det = gq[ND * ND], r = gq[ND * ND + 1];
+    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)ip);
+    double coef = det * tab[q] * thickness * (axisym ? r : 1.0);
     double alpha, beta, gamma = 0.0;
     double s[6] = {0, 0, 0, 0, 0, 0};
     if (mp.plane_stress) {
@@ -416,16 +390,14 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
     Q[1] = coef * beta;
     Q[2] = coef * gamma;
 #pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) Q[3 + i * ND + j] = Ji[i][j];
+    for (int ij = 0; ij < ND * ND; ij++) Q[3 + ij] = gq[ij];
 #pragma unroll
     for (int c = 0; c < 6; c++) Q[3 + ND * ND + c] = s[c];
     Q[3 + ND * ND + 6] = r;
   }
   // this thread: element el, tile (I <= J) -> blocks (2I+u, 2J+v)
   const int el = tid / NT, tile = tid - el * NT;
-  const bool active = el < epb && (e0 + el) < nel;
+  const bool active = el < EPW && (e0 + el) < nel;
   int I = 0, J = 0;
   {
     int rem = tile;
@@ -453,19 +425,25 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
       const double* Q = sQ + (el * nip + q0 + ql) * QW;
       const double ha = Q[0], be = Q[1], ga = Q[2];
       const double* Pq = sP + el * ES + ql * NNP * PW;
-      double Ga[2][ND], tIa[2][ND], ta[2][ND], hA[2];
+      double Ga[2][ND], tIa[2][ND], hA[2];
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         const double* Pa = Pq + (2 * I + u) * PW;
 #pragma unroll
-        for (int i = 0; i < ND; i++) { Ga[u][i] = Pa[i]; ta[u][i] = Pa[ND + 1 + i]; tIa[u][i] = Ga[u][i]; }
+        for (int i = 0; i < ND; i++) { Ga[u][i] = Pa[i]; tIa[u][i] = Ga[u][i]; }
         hA[u] = AX ? Pa[ND] : 0.0;
         if (AX) tIa[u][0] += hA[u];
       }
       // the s(x)s term only where the IP is plastic (gamma = 0 at elastic IPs, so skipping it adds nothing):
-      // a branch that is uniform over the lanes of one element and IP
+      // a branch that is uniform over the lanes of one element and IP; the ts vectors are read from LDS only
+      // in the plastic branch
       auto blocks = [&](auto plc) {
         constexpr bool PL = decltype(plc)::value;
+        double ta[2][ND];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+          for (int i = 0; i < ND; i++) ta[u][i] = PL ? Pq[(2 * I + u) * PW + ND + 1 + i] : 0.0;
 #pragma unroll
         for (int v = 0; v < 2; v++) {
           const double* Pb = Pq + (2 * J + v) * PW;
@@ -507,15 +485,12 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
   auto chunks = [&](auto axc) {
   for (int q0 = 0; q0 < nip; q0 += ch) {
     const int nq = min(ch, nip - q0);
-    __syncthreads();  // J^-1 and D ready (first chunk); the previous chunk's node vectors consumed
-    // node vectors of every (element, IP of the chunk, node); the padding node of odd NN stays zero.
+    __syncthreads();  // coefficients ready (first chunk); the previous chunk's node vectors consumed
+    // node vectors of every (IP of the chunk, element, node); the padding node of odd NN stays zero.
     // One item at a time: the tile accumulators are live here
-    const int per_el = nq * NNP;
 #pragma unroll 1
-    for (int w = tid; w < epb * per_el; w += blockDim.x) {
-      int e = 0;  // w / per_el without a division by a run-time value (epb is small)
-      while (w >= (e + 1) * per_el) e++;
-      const int rem = w - e * per_el, ql = rem / NNP, m = rem - ql * NNP;
+    for (int w = tid; w < nq * EPW * NNP; w += TW) {
+      const int m = w % NNP, r1 = w / NNP, e = r1 % EPW, ql = r1 / EPW;
       double* P = sP + e * ES + (ql * NNP + m) * PW;
       if (e0 + e >= nel || m >= NN) {
 #pragma unroll
@@ -524,7 +499,7 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
       }
       const int q = q0 + ql;
       const double* Q = sQ + (e * nip + q) * QW;
-      const double* dN = sdN + q * DS + m * ND;
+      const double* dN = tdN + (q * NN + m) * ND;
       double G[ND];
 #pragma unroll
       for (int j = 0; j < ND; j++) {
@@ -560,16 +535,20 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
     for (int v = 0; v < 2; v++)
 #pragma unroll
       for (int i = 0; i < ND; i++) K[u][v][i][i] += Dg[u][v];
-  // one element at a time: its upper node blocks in rank order (a block computed below the rank
-  // diagonal is stored transposed in its mirror position), then a contiguous 16-byte write
-  KT* stage = reinterpret_cast<KT*>(sP);
+  // the workgroup's element matrices are contiguous: staged in LDS over the IP data, cap values per pass,
+  // then copied out with 16-byte stores.  A block computed below the rank diagonal is stored transposed in
+  // its mirror position; a diagonal node block takes its lower half from the upper one, so every element
+  // matrix (and K) is exactly symmetric (the local check of assembly.rs:78-90 holds by construction)
+  KT* stage = reinterpret_cast<KT*>(sh);
   typedef KT V __attribute__((ext_vector_type(16 / sizeof(KT))));
-  constexpr int NV = EW * (int)sizeof(KT) / 16;
-  static_assert(EW * sizeof(KT) % 16 == 0, "element matrix is a whole number of 16-byte pieces");
-  for (int se = 0; se < epb && e0 + se < nel; se++) {
-    __syncthreads();  // node vectors / previous element's staging consumed
-    if (active && el == se) {
+  constexpr int VL = 16 / (int)sizeof(KT);
+  static_assert(EW % VL == 0, "element matrix is a whole number of 16-byte pieces");
+  const int nval = (int)min((int64_t)EPW, nel - e0) * EW;
+  for (int p0 = 0; p0 < nval; p0 += cap) {
+    __syncthreads();  // node vectors / the previous pass consumed
+    if (active) {
       const uint8_t* rk = srank + el * NN;
+      const int base = el * EW - p0;
 #pragma unroll
       for (int u = 0; u < 2; u++)
 #pragma unroll
@@ -578,21 +557,24 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
           if (a >= NN || b >= NN || a > b) continue;
           const int ra = rk[a], rb = rk[b];
           const bool up = ra <= rb;
-          KT* o = stage + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
-          // a diagonal node block takes its lower half from the upper one, so every element matrix (and K)
-          // is exactly symmetric (the local check of assembly.rs:78-90 holds by construction)
+          const int o = base + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
 #pragma unroll
           for (int i = 0; i < ND; i++)
 #pragma unroll
-            for (int j = 0; j < ND; j++) o[up ? i * ND + j : j * ND + i] = (KT)(a == b && j < i ? K[u][v][j][i] : K[u][v][i][j]);
+            for (int j = 0; j < ND; j++) {
+              const int x = o + (up ? i * ND + j : j * ND + i);
+              if ((unsigned)x < (unsigned)cap) stage[x] = (KT)(a == b && j < i ? K[u][v][j][i] : K[u][v][i][j]);
+            }
         }
       if (tile == 0)
-        for (int i = EU; i < EW; i++) stage[i] = (KT)0;
+        for (int i = EU; i < EW; i++)
+          if ((unsigned)(base + i) < (unsigned)cap) stage[base + i] = (KT)0;
     }
     __syncthreads();
-    V* dst = reinterpret_cast<V*>(ke + (e0 + se) * (int64_t)EW);
+    const int nv = min(cap, nval - p0) / VL;
+    V* dst = reinterpret_cast<V*>(ke + e0 * EW + p0);
     const V* src = reinterpret_cast<const V*>(stage);
-    for (int i = tid; i < NV; i += blockDim.x) dst[i] = src[i];
+    for (int i = tid; i < nv; i += TW) dst[i] = src[i];
   }
 }
 
@@ -692,9 +674,9 @@ __global__ __launch_bounds__(TPB) void kb_update(const UpdArgs* __restrict__ job
   update_body<ND, NN>(w, jobs[j]);
 }
 template <int ND, int NN, typename KT>
-__global__ __launch_bounds__(TPB) void k_tangent(TanArgs<KT> A) { tangent_body<ND, NN, KT>(xcd_block(), A); }
+__global__ __launch_bounds__(TW) void k_tangent(TanArgs<KT> A) { tangent_body<ND, NN, KT>(xcd_block(), A); }
 template <int ND, int NN, typename KT>
-__global__ __launch_bounds__(TPB) void kb_tangent(const TanArgs<KT>* __restrict__ jobs, const int32_t* __restrict__ map) {
+__global__ __launch_bounds__(TW) void kb_tangent(const TanArgs<KT>* __restrict__ jobs, const int32_t* __restrict__ map) {
   int j, w;
   unpack_job(map[blockIdx.x], j, w);
   tangent_body<ND, NN, KT>(w, jobs[j]);
@@ -724,6 +706,37 @@ __global__ void k_init_state(int64_t nipg, int ns, double* st, double z0, int vm
   st[(ns + 1) * nipg + i] = 0.0;
   st[(ns + 2) * nipg + i] = 1.0;
   st[(ns + 3) * nipg + i] = 0.0;
+}
+
+// J^-1, det J and the radius (axisymmetric) of every IP of a group, one IP per thread: the same arithmetic
+// as the update kernel's (jacobian), formed once since the mesh does not move (small strain)
+template <int ND, int NN>
+__global__ __launch_bounds__(TPB) void k_geometry(int64_t nel, int nip, const int32_t* __restrict__ conn,
+                                                  const double* __restrict__ X, const double* __restrict__ tab, int axisym,
+                                                  double* __restrict__ geo) {
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= nel * nip) return;
+  const int64_t e = ip / nip;
+  const int q = (int)(ip - e * nip);
+  double Xe[NN * ND];
+#pragma unroll
+  for (int m = 0; m < NN; m++) {
+    const int64_t p = conn[e * NN + m];
+#pragma unroll
+    for (int d = 0; d < ND; d++) Xe[m * ND + d] = X[p * ND + d];
+  }
+  const double* tN = tab + nip;
+  double Ji[ND][ND], det;
+  jacobian<ND, NN>(Xe, tN + nip * NN + q * NN * ND, Ji, det);
+  double r = 0.0;
+  if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * Xe[m * ND];
+  double* o = geo + ip * (ND * ND + 2);
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++) o[i * ND + j] = Ji[i][j];
+  o[ND * ND] = det;
+  o[ND * ND + 1] = r;
 }
 
 __global__ void k_fill(double* x, double v, int64_t n) {
@@ -763,22 +776,30 @@ static int epb_update(int nd, int nn, int nip) {
   while (epb > 1 && shm_update(nd, nn, nip, epb) > SHM_CAP) epb--;
   return epb;
 }
-static size_t shm_tangent(int nd, int nn, int nip, int epb, int ch, bool axisym) {
-  int nh = (nn + 1) / 2, pw = 2 * nd + 1;
-  size_t table = (size_t)nip * (1 + nn * nd + 1) + (axisym ? (size_t)nip * nn : 0);
-  size_t pblk = (size_t)epb * ((ch * 2 * nh * pw) | 1);           // node vectors of one IP chunk (odd stride)
-  size_t stg = (size_t)packed_ke_values(nn, nd);                  // output staging, one element (FP64 bound)
-  size_t uni = std::max(pblk, stg);
-  return sizeof(double) * (table + (size_t)epb * nn * nd + (size_t)epb * nip * (3 + nd * nd + 7) + 1 + uni);  // +1: alignment
+// dynamic LDS of a vector-tangent workgroup (bytes): per-IP coefficients and one chunk's node vectors, and
+// at least half of the workgroup's element matrices for the output staging (at most two passes)
+static size_t shm_tangent(int nd, int nn, int nip, int ch, int kt_bytes) {
+  const int epw = tangent_epw(nn), nh = (nn + 1) / 2, pw = 2 * nd + 1, qw = 3 + nd * nd + 7;
+  const size_t base = (size_t)epw * nip * qw + (size_t)epw * ((ch * 2 * nh * pw) | 1);
+  const size_t half = ((size_t)epw * packed_ke_values(nn, nd) / 2 + 3) / 4 * 4;  // values
+  const size_t dbl = std::max(base, (half * kt_bytes + 7) / 8);
+  return sizeof(double) * ((dbl + 1) / 2 * 2);
 }
-constexpr size_t TANGENT_LDS_TARGET = 52 * 1024;  // three workgroups per CU (160 KB LDS)
-// work split of the tangent kernel: elements per workgroup (one thread per tile) and IPs per chunk
-static void split_tangent(int nd, int nn, int nip, bool axisym, int& epb, int& ch) {
-  int nh = (nn + 1) / 2, nt = nh * (nh + 1) / 2;
-  epb = std::max(1, TPB / nt);
-  ch = std::min(nip, 8);
-  while (ch > 1 && shm_tangent(nd, nn, nip, epb, ch, axisym) > TANGENT_LDS_TARGET) ch--;
-  while (epb > 1 && shm_tangent(nd, nn, nip, epb, ch, axisym) > TANGENT_LDS_TARGET) epb--;
+// output staging per pass of a vector-tangent workgroup, in values (a multiple of 4)
+static int stage_cap(size_t shm, int kt_bytes) { return (int)(shm / kt_bytes) / 4 * 4; }
+constexpr size_t TANGENT_LDS_TARGET = 13 * 1024;  // twelve one-wavefront workgroups per CU (160 KB LDS)
+// IPs per chunk of the vector tangent: the fewest rounds of node-vector staging (one round: a pass of the
+// wavefront over its items; each chunk also costs a synchronisation) within the LDS target
+static int split_tangent(int nd, int nn, int nip, int kt_bytes) {
+  const int epw = tangent_epw(nn), nnp = 2 * ((nn + 1) / 2);
+  int best = 1, best_cost = 1 << 30;
+  for (int ch = 1; ch <= std::min(nip, 16); ch++) {
+    if (ch > 1 && shm_tangent(nd, nn, nip, ch, kt_bytes) > TANGENT_LDS_TARGET) break;
+    int cost = 0;
+    for (int q0 = 0; q0 < nip; q0 += ch) cost += (std::min(ch, nip - q0) * epw * nnp + TW - 1) / TW + 1;
+    if (cost <= best_cost) { best_cost = cost; best = ch; }
+  }
+  return best;
 }
 
 #define PM_DISPATCH_KIND(KIND, ND, NN, ...)          \
@@ -823,17 +844,17 @@ static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   return 1;  // FP64 and FP32: the node-block vector kernel (measured faster than the FP32 matrix-core kernel)
 }
 
-// work split of a group's vector tangent: the tuned one, else the heuristic
-static void tangent_split(const pm_fem* f, const Group& g, int& epb, int& ch) {
-  split_tangent(g.ndim, g.nnode, g.nip, f->axisym, epb, ch);
-  if (g.tune_epb > 0) { epb = g.tune_epb; ch = g.tune_ch; }
+// IPs per chunk of a group's vector tangent: the tuned value, else the heuristic
+static int tangent_split(const pm_fem* f, const Group& g) {
+  if (g.tune_ch > 0) return g.tune_ch;
+  return split_tangent(g.ndim, g.nnode, g.nip, f->cfg.tangent_precision == 0 ? 8 : 4);
 }
 
 template <typename KT>
-static TanArgs<KT> tan_args(pm_fem* f, Group& g, int epb, int ch, KT* ke) {
+static TanArgs<KT> tan_args(pm_fem* f, Group& g, int ch, KT* ke, size_t shm) {
   TanArgs<KT> A;
-  A.nel = g.nel; A.nip = g.nip; A.epb = epb; A.ch = ch;
-  A.conn = g.d_conn; A.rank = g.d_rank; A.X = f->d_X; A.tab = g.d_tab; A.st = g.d_state; A.ke = ke + g.ke_base;
+  A.nel = g.nel; A.nip = g.nip; A.ch = ch; A.cap = stage_cap(shm, (int)sizeof(KT));
+  A.rank = g.d_rank; A.geo = g.d_geo; A.tab = g.d_tab; A.st = g.d_state; A.ke = ke + g.ke_base;
   A.mp = mat_of(f, g); A.axisym = f->axisym ? 1 : 0; A.thickness = f->thickness; A.err = f->d_err; A.ekey = group_key(f, g);
   return A;
 }
@@ -841,20 +862,20 @@ static TanArgs<KT> tan_args(pm_fem* f, Group& g, int epb, int ch, KT* ke) {
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
   if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
-  int epb, ch;
-  tangent_split(f, g, epb, ch);
-  const int64_t grid = (g.nel + epb - 1) / epb;
-  const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym);
+  const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
+  const int64_t grid = (g.nel + epw - 1) / epw;
   if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) {
     f->launch_refused = true;  // reported by check_device_error; never launched out of range
     return;
   }
   if (f->cfg.tangent_precision == 0) {
-    const TanArgs<double> A = tan_args(f, g, epb, ch, f->d_ke);
-    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
+    const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, ch, 8);
+    const TanArgs<double> A = tan_args(f, g, ch, f->d_ke, shm);
+    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, double>), dim3((unsigned)grid), dim3(TW), shm, f->stream, A));
   } else {
-    const TanArgs<float> A = tan_args(f, g, epb, ch, f->d_ke32);
-    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
+    const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, ch, 4);
+    const TanArgs<float> A = tan_args(f, g, ch, f->d_ke32, shm);
+    PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_tangent<ND, NN, float>), dim3((unsigned)grid), dim3(TW), shm, f->stream, A));
   }
 }
 
@@ -864,9 +885,9 @@ void launch_tangent(pm_fem* f, Group& g) {
 int tune_tangent(pm_fem* f) {
   const int mode = f->cfg.tangent_kernel;
   release_graphs(f);
-  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_epb = g.tune_ch = 0; }
+  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_ch = 0; }
   if (mode != 3) return 0;
-  struct Cand { int kern, epb, ch; };
+  struct Cand { int kern, ch; };
   hipEvent_t ev0, ev1;
   PM_HIP(hipEventCreate(&ev0));
   PM_HIP(hipEventCreate(&ev1));
@@ -874,23 +895,17 @@ int tune_tangent(pm_fem* f) {
   for (auto& g : f->groups) {
     if (g.nel == 0) continue;
     std::vector<Cand> cands;
-    cands.push_back({2, 0, 0});
-    if (f->cfg.tangent_precision != 2) {
-      const int nh = (g.nnode + 1) / 2, nt = nh * (nh + 1) / 2;
-      const int emax = std::max(1, TPB / nt);
-      std::vector<int> epbs = {emax};
-      if ((emax + 1) / 2 < emax) epbs.push_back((emax + 1) / 2);
-      for (int epb : epbs)
-        for (int ch : {1, 2, 3, 4, 6, 9, 14}) {
-          if (ch > g.nip) continue;
-          if (shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym) > SHM_CAP) continue;
-          cands.push_back({1, epb, ch});
-        }
-    }
+    cands.push_back({2, 0});
+    if (f->cfg.tangent_precision != 2)
+      for (int ch : {1, 2, 3, 4, 6, 9, 14}) {
+        if (ch > g.nip) continue;
+        if (shm_tangent(g.ndim, g.nnode, g.nip, ch, f->cfg.tangent_precision == 0 ? 8 : 4) > SHM_CAP) continue;
+        cands.push_back({1, ch});
+      }
     float best = 1e30f;
     Cand pick = cands[0];
     for (const Cand& c : cands) {
-      g.tune_kernel = c.kern; g.tune_epb = c.epb; g.tune_ch = c.ch;
+      g.tune_kernel = c.kern; g.tune_ch = c.ch;
       launch_tangent(f, g);  // warm-up
       if (hipEventRecord(ev0, f->stream) != hipSuccess) { rc = 1; break; }
       for (int r = 0; r < 3; r++) launch_tangent(f, g);
@@ -898,11 +913,11 @@ int tune_tangent(pm_fem* f) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, ev0, ev1);
       if (f->cfg.verbose & 1)
-        std::fprintf(stderr, "tangent tuning: kind %d, %s kernel, %d elements per workgroup, %d IPs per chunk: %.3f ms\n", g.kind,
-                     c.kern == 2 ? "matrix-core" : "vector", c.epb, c.ch, ms / 3.0f);
+        std::fprintf(stderr, "tangent tuning: kind %d, %s kernel, %d IPs per chunk: %.3f ms\n", g.kind,
+                     c.kern == 2 ? "matrix-core" : "vector", c.ch, ms / 3.0f);
       if (ms < best) { best = ms; pick = c; }
     }
-    g.tune_kernel = pick.kern; g.tune_epb = pick.epb; g.tune_ch = pick.ch;
+    g.tune_kernel = pick.kern; g.tune_ch = pick.ch;
     if (rc) break;
   }
   (void)hipEventDestroy(ev0);
@@ -978,7 +993,7 @@ inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& jobs) {
   using UK = std::tuple<int, int, int>;                  // kind, nip, epb
-  using TK = std::tuple<int, int, int, int, int>;        // kind, nip, epb, ch, axisym
+  using TK = std::tuple<int, int, int, int>;             // kind, nip, ch, shm
   std::map<UK, Cls<UpdArgs>> upd;
   std::map<int, Cls<RgArgs>> rg;
   std::map<TK, Cls<TanArgs<double>>> tan64;
@@ -999,13 +1014,13 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
       if (J.tangent) {
         if (tangent_kernel_of(f, g) == 2) { mfma.push_back({f, &g}); continue; }
         if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) { f->launch_refused = true; continue; }
-        int epb, ch;
-        tangent_split(f, g, epb, ch);
-        const TK key{g.kind, g.nip, epb, ch, f->axisym ? 1 : 0};
-        const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, epb, ch, f->axisym);
-        const int64_t nblk = (g.nel + epb - 1) / epb;
-        if (f->cfg.tangent_precision == 0) { auto& c = tan64[key]; c.shm = shm; if (!c.add(tan_args(f, g, epb, ch, f->d_ke), nblk)) return 1; }
-        else { auto& c = tan32[key]; c.shm = shm; if (!c.add(tan_args(f, g, epb, ch, f->d_ke32), nblk)) return 1; }
+        const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
+        const int kb = f->cfg.tangent_precision == 0 ? 8 : 4;
+        const size_t shm = shm_tangent(g.ndim, g.nnode, g.nip, ch, kb);
+        const TK key{g.kind, g.nip, ch, (int)shm};
+        const int64_t nblk = (g.nel + epw - 1) / epw;
+        if (kb == 8) { auto& c = tan64[key]; c.shm = shm; if (!c.add(tan_args(f, g, ch, f->d_ke, shm), nblk)) return 1; }
+        else { auto& c = tan32[key]; c.shm = shm; if (!c.add(tan_args(f, g, ch, f->d_ke32, shm), nblk)) return 1; }
       }
     }
     if (J.residual && !rg[f->ndim].add(rg_args(f), (f->npoint * f->ndim + TPB - 1) / TPB)) return 1;
@@ -1056,12 +1071,12 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
   for (auto& kv : tan64) {
     auto& c = kv.second;
     PM_DISPATCH_KIND(std::get<0>(kv.first), ND, NN,
-                     hipLaunchKernelGGL((kb_tangent<ND, NN, double>), dim3((unsigned)c.map.size()), dim3(TPB), c.shm, stream, dargs(c), dmap(c)));
+                     hipLaunchKernelGGL((kb_tangent<ND, NN, double>), dim3((unsigned)c.map.size()), dim3(TW), c.shm, stream, dargs(c), dmap(c)));
   }
   for (auto& kv : tan32) {
     auto& c = kv.second;
     PM_DISPATCH_KIND(std::get<0>(kv.first), ND, NN,
-                     hipLaunchKernelGGL((kb_tangent<ND, NN, float>), dim3((unsigned)c.map.size()), dim3(TPB), c.shm, stream, dargs(c), dmap(c)));
+                     hipLaunchKernelGGL((kb_tangent<ND, NN, float>), dim3((unsigned)c.map.size()), dim3(TW), c.shm, stream, dargs(c), dmap(c)));
   }
   for (auto& mg : mfma) {  // one launch per group, on the batch's stream (the handle's owner waits for the batch)
     hipStream_t own = mg.first->stream;
@@ -1088,6 +1103,14 @@ void launch_init_state(pm_fem* f, Group& g) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_init_state, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, f->stream, n, f->nsig, g.d_state,
                      g.param.z_ini, g.param.model == 1 ? 1 : 0);
+}
+
+void launch_geometry(pm_fem* f, Group& g) {
+  if (g.nel == 0) return;
+  const int64_t n = g.nel * g.nip;
+  const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
+  PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_geometry<ND, NN>), dim3(grid), dim3(TPB), 0, f->stream, g.nel, g.nip,
+                                                      g.d_conn, f->d_X, g.d_tab, f->axisym ? 1 : 0, g.d_geo));
 }
 
 void launch_reset_lambda(pm_fem* f, Group& g) {
@@ -1146,6 +1169,5 @@ int check_device_error(pm_fem* f) {
 }
 
 size_t update_shared_bytes(const Group& g) { return shm_update(g.ndim, g.nnode, g.nip, epb_update(g.ndim, g.nnode, g.nip)); }
-size_t tangent_shared_bytes(const Group& g) { int e, q; split_tangent(g.ndim, g.nnode, g.nip, false, e, q); return shm_tangent(g.ndim, g.nnode, g.nip, e, q, false); }
 
 }  // namespace pm

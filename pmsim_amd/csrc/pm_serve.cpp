@@ -190,6 +190,7 @@ int pm_estimate_device_bytes(const pm_fem* f, int64_t* bytes) {
   b += f->npoint * nd * 8;                       // coordinates
   b += conn * 5;                                  // connectivity + ranks
   b += nip * (ns + 4) * 8 * 2;                    // IP state + backup
+  b += nip * (b2 + 2) * 8;                        // IP geometry (J^-1, det J, radius)
   b += kev * ts;                                  // element matrices (packed upper blocks)
   b += blocks * (b2 * ts + 16);                   // K (upper blocks) in the tangent's precision, upper and lower index
                                                   // lists and the gather map (bound: every count <= element blocks)
