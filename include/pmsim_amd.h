@@ -91,6 +91,13 @@ void pm_fem_destroy(pm_fem* fem);
  * ngauss <= 0 selects the kind's default rule. */
 int pm_set_param_solid(pm_fem* fem, int32_t attribute, int32_t model, double young, double poisson,
                        double hh, double z_ini, int32_t ngauss);
+/* Model settings of one attribute (Config::update_model_settings, src/base/config.rs:515-517; Settings,
+ * src/material/settings.rs:6-66), validated when the elements are allocated (pm_initialize) with pmsim's
+ * texts.  general_plasticity selects, for von Mises, the general Elastoplastic model
+ * (model_stress_strain.rs:65-71), whose stiffness pmsim does not provide (elastoplastic.rs:528-536): any
+ * tangent then fails with pmsim's "TODO"; the update without a tangent is refused as not available. */
+int pm_set_model_settings(pm_fem* fem, int32_t attribute, int32_t general_plasticity, int32_t gp_interp_nn_max,
+                          double nle_beta);
 /* Idealization (src/base/idealization.rs): 2D plane strain unless axisymmetric or plane stress. */
 int pm_set_idealization(pm_fem* fem, int32_t axisymmetric, int32_t plane_stress, double thickness);
 int pm_set_config(pm_fem* fem, const pm_config* cfg);

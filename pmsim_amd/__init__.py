@@ -5,7 +5,7 @@ Host-side mirror of pmsim's builder API; the compute path is the C-ABI library
 """
 from .mesh import At, Features, Mesh, any_x, points_of  # noqa: F401
 from .problem import (Config, Dof, Elem, Essential, FemBase, Natural, Nbc, ParamSolid, Pbc,  # noqa: F401
-                      StressStrain, describe)
+                      Settings, StressStrain, describe)
 
 __all__ = ["At", "Features", "Mesh", "any_x", "points_of", "Config", "Dof", "Elem", "Essential", "FemBase",
-           "Natural", "Nbc", "ParamSolid", "Pbc", "StressStrain", "describe"]
+           "Natural", "Nbc", "ParamSolid", "Pbc", "Settings", "StressStrain", "describe"]

@@ -100,6 +100,14 @@ int pm_set_param_solid(pm_fem* f, int32_t attribute, int32_t model, double young
   return 0;
 }
 
+int pm_set_model_settings(pm_fem* f, int32_t attribute, int32_t general_plasticity, int32_t gp_interp_nn_max, double nle_beta) {
+  CHECK_HANDLE(f);
+  if (f->initialized) { f->err = "model settings must be set before pm_initialize"; return 1; }
+  ModelSettings& m = f->model_settings[attribute];
+  m.general_plasticity = general_plasticity != 0; m.gp_interp_nn_max = gp_interp_nn_max; m.nle_beta = nle_beta;
+  return 0;
+}
+
 int pm_set_idealization(pm_fem* f, int32_t axisymmetric, int32_t plane_stress, double thickness) {
   CHECK_HANDLE(f);
   if (thickness <= 0.0) { f->err = "thickness must be > 0.0"; return 1; }
@@ -257,6 +265,7 @@ int pm_assemble_f_int(pm_fem* f) {
 
 int pm_assemble_kk(pm_fem* f) {
   CHECK_INIT(f);
+  if (gp_refuse(f, false, true)) return 1;
   for (auto& g : f->groups) launch_tangent(f, g);
   launch_scatter(f);
   return check_device_error(f);
@@ -372,6 +381,7 @@ int pm_get_element_kk(const pm_fem* f, int64_t cell, double* kk) {
   if (!kk) return 1;
   if (cell < 0 || cell >= f->ncell) { const_cast<pm_fem*>(f)->err = "cell out of range"; return 1; }
   const Group& g = f->groups[f->cell_group[cell]];
+  if (g.general_plasticity) { const_cast<pm_fem*>(f)->err = "TODO"; return 1; }  // no stiffness (elastoplastic.rs:528-536)
   const int64_t el = f->cell_local[cell];
   const int nn = g.nnode, nd = g.ndim, ne = nn * nd;
   std::vector<double> v(g.kew);
@@ -478,6 +488,7 @@ int pm_get_history_loading_factor(const pm_fem* f, int32_t index, double* ell) {
 int pm_time_element_pass(pm_fem* f, int32_t reps, double* ms) {
   CHECK_INIT(f);
   if (!ms || reps < 1) { f->err = "pm_time_element_pass needs reps >= 1 and an output array"; return 1; }
+  if (gp_refuse(f, true, true)) return 1;
   hipEvent_t ev[5];
   for (auto& e : ev) PM_HIP(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};

@@ -2,11 +2,34 @@
 // external forces, and the host direct solver used as fallback and judge.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
+#include <string>
 
 #include "pm_internal.hpp"
 
 namespace pm {
+
+// an f64 as Rust's {:?} prints it: the shortest form that reads back to the value, ".0" on integral values
+static std::string rust_debug_f64(double v) {
+  char b[48];
+  for (int p = 1; p <= 17; p++) {
+    std::snprintf(b, sizeof b, "%.*g", p, v);
+    if (std::strtod(b, nullptr) == v) break;
+  }
+  std::string s(b);
+  const size_t e = s.find('e');
+  if (e != std::string::npos) {  // exponent without '+' or leading zeros
+    std::string m = s.substr(0, e), x = s.substr(e + 1);
+    const bool neg = !x.empty() && x[0] == '-';
+    if (!x.empty() && (x[0] == '-' || x[0] == '+')) x = x.substr(1);
+    while (x.size() > 1 && x[0] == '0') x = x.substr(1);
+    return m + "e" + (neg ? "-" : "") + x;
+  }
+  if (s.find_first_of(".n") == std::string::npos) s += ".0";  // 'n': nan, inf
+  return s;
+}
 
 // Element batches: one per (kind, attribute); cells keep ascending global order.
 int build_groups(pm_fem* f) {
@@ -33,6 +56,20 @@ int build_groups(pm_fem* f) {
       g.nip = r.n();
       g.kew = packed_ke_values(g.nnode, g.ndim);
       const ParamSolid& p = g.param;
+      // ModelStressStrain::new (model_stress_strain.rs:52-72): the attribute's settings are validated first
+      // (settings.rs:55-66); von Mises under general plasticity is the Elastoplastic model
+      const auto sit = f->model_settings.find(g.attribute);
+      const ModelSettings ms = sit != f->model_settings.end() ? sit->second : ModelSettings();
+      std::string bad;
+      if (ms.nle_beta < 0.0) bad = "nle_beta = " + rust_debug_f64(ms.nle_beta) + " is incorrect; it must be ≥ 0.0";
+      else if (ms.gp_interp_nn_max < 1) bad = "gp_interp_nn_max = " + std::to_string(ms.gp_interp_nn_max) + " is incorrect; it must be ≥ 1";
+      if (!bad.empty()) {
+        std::printf("ERROR: %s\n", bad.c_str());  // settings.rs prints it before the allocation fails
+        std::fflush(stdout);
+        f->err = "cannot allocate ModelStressStrain because settings.validate() failed";
+        return 1;
+      }
+      g.general_plasticity = p.model == 1 && ms.general_plasticity;
       if (p.model == 1) {
         if (f->plane_stress) { f->err = "von Mises model does not work in plane-stress"; return 1; }
         if (p.z_ini <= 1e-6) { f->err = "von Mises initial size of the yield surface must > 1e-6"; return 1; }

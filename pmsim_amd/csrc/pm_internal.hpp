@@ -50,6 +50,13 @@ struct ParamSolid {
   bool set = false;
 };
 
+// model settings of one cell attribute (src/material/settings.rs:6-50; Config::update_model_settings)
+struct ModelSettings {
+  bool general_plasticity = false;  // von Mises through the general elastoplastic formulation (elastoplastic.rs)
+  int gp_interp_nn_max = 30;
+  double nle_beta = 0.0;
+};
+
 // Per-kind element batch on the device.
 struct PcgEngine;
 struct PassEngine;
@@ -69,6 +76,7 @@ inline bool ke_range_ok(int64_t ke_base, int64_t nel, int kew, int ew, int64_t n
 struct Group {
   int kind = 0, ndim = 2, nnode = 0, nip = 0, attribute = 0;
   ParamSolid param;
+  bool general_plasticity = false;  // von Mises under the general formulation: its stiffness is not available
   // material constants
   double K = 0, G = 0;
   // plane-stress elastic (LinearElastic only)
@@ -111,6 +119,7 @@ struct pm_fem {
   std::vector<int64_t> cptr;
   std::vector<int32_t> cpts;
   std::map<int, pm::ParamSolid> params;
+  std::map<int, pm::ModelSettings> model_settings;
   bool axisym = false, plane_stress = false;
   double thickness = 1.0;
   pm_config cfg;
@@ -175,6 +184,7 @@ struct pm_fem {
   float* d_ke32 = nullptr;             // FP32 element blocks
   int64_t n_ke_values = 0;
   bool launch_refused = false;         // a launch whose output range did not fit its buffer was skipped
+  const char* refuse_msg = nullptr;    // the refusal's message (null: an out-of-range element-matrix launch)
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
   int32_t* d_korder = nullptr;         // gather chunk processing order
   int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
@@ -237,6 +247,7 @@ void device_free(pm_fem* f);
 void launch_update(pm_fem* f, Group& g, bool do_update);
 void launch_residual_gather(pm_fem* f);
 void launch_tangent(pm_fem* f, Group& g);
+bool gp_refuse(pm_fem* f, bool do_update, bool tangent);
 // matrix-core tangent (prec: 0 FP64, 1 FP32, 2 BF16 with FP32 accumulation); false if the kind has none
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 // choose the tangent kernel (and work split) of every group, timing the candidates when asked to

@@ -827,8 +827,26 @@ static UpdArgs upd_args(pm_fem* f, Group& g, bool do_update, int epb) {
   return A;
 }
 
+// von Mises under general plasticity: pmsim's Elastoplastic model has no consistent stiffness
+// (elastoplastic.rs:528-536 returns "TODO"), so no tangent is formed; its ODE stress update is not built here
+constexpr const char* kGpStiffness = "TODO";
+constexpr const char* kGpUpdate = "the general-plasticity (Elastoplastic) stress update is not available on the device";
+static void refuse(pm_fem* f, const char* msg) { f->launch_refused = true; f->refuse_msg = msg; }
+
+// the refusal above, taken before a pass is launched or captured: true (with f->err) when the pass would
+// form a tangent, or update the stress, of a general-plasticity group
+bool gp_refuse(pm_fem* f, bool do_update, bool tangent) {
+  for (const auto& g : f->groups) {
+    if (!g.general_plasticity || g.nel == 0) continue;
+    if (tangent) { f->err = kGpStiffness; return true; }
+    if (do_update) { f->err = kGpUpdate; return true; }
+  }
+  return false;
+}
+
 void launch_update(pm_fem* f, Group& g, bool do_update) {
   if (g.nel == 0) return;
+  if (do_update && g.general_plasticity) { refuse(f, kGpUpdate); return; }
   const int epb = epb_update(g.ndim, g.nnode, g.nip);
   const int64_t grid = (g.nel + epb - 1) / epb;
   const size_t shm = shm_update(g.ndim, g.nnode, g.nip, epb);
@@ -861,6 +879,7 @@ static TanArgs<KT> tan_args(pm_fem* f, Group& g, int ch, KT* ke, size_t shm) {
 
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
+  if (g.general_plasticity) { refuse(f, kGpStiffness); return; }
   if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
   const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
   const int64_t grid = (g.nel + epw - 1) / epw;
@@ -893,7 +912,7 @@ int tune_tangent(pm_fem* f) {
   PM_HIP(hipEventCreate(&ev1));
   int rc = 0;
   for (auto& g : f->groups) {
-    if (g.nel == 0) continue;
+    if (g.nel == 0 || g.general_plasticity) continue;  // no tangent to time
     std::vector<Cand> cands;
     cands.push_back({2, 0});
     if (f->cfg.tangent_precision != 2)
@@ -1005,6 +1024,8 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
     pm_fem* f = J.f;
     for (auto& g : f->groups) {
       if (g.nel == 0) continue;
+      if (J.update && J.do_update && g.general_plasticity) { refuse(f, kGpUpdate); continue; }
+      if (J.tangent && g.general_plasticity) { refuse(f, kGpStiffness); continue; }
       if (J.update) {
         const int epb = epb_update(g.ndim, g.nnode, g.nip);
         auto& c = upd[UK{g.kind, g.nip, epb}];
@@ -1140,7 +1161,8 @@ void reset_device_error(pm_fem* f) {
 int check_device_error(pm_fem* f) {
   if (f->launch_refused) {
     f->launch_refused = false;
-    f->err = "internal error: an element-matrix range did not fit its buffer (kernel not launched)";
+    f->err = f->refuse_msg ? f->refuse_msg : "internal error: an element-matrix range did not fit its buffer (kernel not launched)";
+    f->refuse_msg = nullptr;
     return 1;
   }
   int32_t w[4] = {0, 0, 0, 0};

@@ -45,6 +45,7 @@ void release_graphs(pm_fem* f) {
 // instead of 2 + 2 x (element batches) kernel launches.  The captured kernels read only buffers whose
 // addresses are fixed for the handle's lifetime; tuning or a configuration change drops the graphs.
 int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
+  if (gp_refuse(f, update, tangent)) return 1;
   if (!f->cfg.use_graphs) {
     launch_pass(f, update, residual, tangent);
     if (hipGetLastError() != hipSuccess) { f->err = "kernel launch failed"; return 1; }
@@ -71,6 +72,7 @@ int element_pass(pm_fem* f, bool update, bool residual, bool tangent) {
 // requests hand them to the run's pass engine, which launches the passes of all running requests together;
 // otherwise they run on the handle's stream (the update combinations as captured graphs).
 int run_pass(pm_fem* f, bool update, bool do_update, bool residual, bool tangent) {
+  if (gp_refuse(f, update && do_update, tangent)) return 1;
   if (f->pass_engine) {
     PM_HIP(hipStreamSynchronize(f->stream));  // the engine's stream reads what this stream wrote
     PassJob job;

@@ -202,6 +202,42 @@ class Natural:
         return np.asarray([f(float(t)) for t in times], dtype=np.float64)
 
 
+class Settings:
+    """Model settings of one cell attribute (``src/material/settings.rs:6-121``), obtained from
+    ``Config.update_model_settings``.  ``general_plasticity`` selects, for von Mises, pmsim's general
+    ``Elastoplastic`` model (``model_stress_strain.rs:65-71``), whose consistent stiffness pmsim leaves
+    unimplemented (``elastoplastic.rs:528-536`` returns "TODO"): a solve then stops at its first tangent
+    with that error.  The other fields only matter to that model; they are validated as pmsim does
+    (``settings.rs:55-66``) when the elements are allocated."""
+
+    def __init__(self):
+        self.save_strain = False
+        self.nle_enabled = False
+        self.nle_beta = 0.0
+        self.nle_isotropic = False
+        self.general_plasticity = False
+        self.gp_ode_method = "DoPri8"
+        self.gp_interp_nn_max = 30
+        self.gp_allow_initial_drift = False
+        self.gp_save_history = False
+
+    def set_save_strain(self, flag: bool): self.save_strain = bool(flag); return self
+    def set_nle_enabled(self, flag: bool): self.nle_enabled = bool(flag); return self
+    def set_nle_beta(self, value: float): self.nle_beta = float(value); return self
+    def set_nle_isotropic(self, flag: bool): self.nle_isotropic = bool(flag); return self
+    def set_general_plasticity(self, flag: bool): self.general_plasticity = bool(flag); return self
+    def set_gp_ode_method(self, method: str): self.gp_ode_method = str(method); return self
+
+    def set_gp_interp_nn_max(self, nn_max: int):
+        if int(nn_max) < 0:
+            raise ValueError("gp_interp_nn_max is an unsigned integer")
+        self.gp_interp_nn_max = int(nn_max)
+        return self
+
+    def set_gp_allow_initial_drift(self, flag: bool): self.gp_allow_initial_drift = bool(flag); return self
+    def set_gp_save_history(self, flag: bool): self.gp_save_history = bool(flag); return self
+
+
 class Config:
     """Solver configuration (defaults of ``src/base/config.rs:184-231``)."""
 
@@ -233,6 +269,7 @@ class Config:
         self.tangent_precision = "f64"  # "f64", "f32" or "bf16" (inexact-Newton tangent)
         self.tangent_kernel = "default"  # "default", "vector", "mfma" (matrix cores) or "tune" (fastest measured)
         self.use_graphs = True  # replay captured HIP graphs for element passes and PCG chunks
+        self.model_settings: Dict[int, Settings] = {}
         self.pcg_small_max = -1  # DOF limit of the single-workgroup PCG (-1: library default)
         self.verbose_timesteps = False  # a line per converged step on stderr
         self.verbose_iterations = False  # a line per Newton iteration on stderr
@@ -285,6 +322,10 @@ class Config:
     # MI355X-specific knobs
     def set_lin_sol(self, kind: str): self.lin_sol = kind; return self
     def set_tangent_precision(self, p: str): self.tangent_precision = p; return self
+
+    def update_model_settings(self, cell_attribute: int) -> Settings:
+        """The attribute's model settings, created with the defaults on first use (config.rs:515-517)."""
+        return self.model_settings.setdefault(int(cell_attribute), Settings())
     def set_tangent_kernel(self, k: str): self.tangent_kernel = k; return self
     def set_use_graphs(self, v: bool): self.use_graphs = bool(v); return self
     def set_pcg_small_max(self, n: int): self.pcg_small_max = int(n); return self
@@ -343,6 +384,8 @@ def describe(mesh: Mesh, base: FemBase, config: Config, essential: Essential, na
     return dict(
         ndim=mesh.ndim, coords=mesh.coords, kinds=mesh.kinds, attrs=mesh.attrs,
         cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts, params=params,
+        model_settings={a: (int(m.general_plasticity), int(m.gp_interp_nn_max), float(m.nle_beta))
+                        for a, m in config.model_settings.items()},
         densities={a: float(p.density) for a, p in base.amap.items()},
         axisym=config.axisymmetric, plane_stress=config.plane_stress, thickness=config.thickness,
         lmm=config.lagrange_mult_method, linear_problem=config.linear_problem,

@@ -31,7 +31,7 @@ EXPORTED = [
     "pm_shape_values", "pm_get_history_loading_factor", "pm_add_facet_load", "pm_assemble_f_ext", "pm_get_f_ext",
     "pm_calculate_residuals_vector", "pm_get_residual", "pm_apply_kk", "pm_get_element_kk", "pm_get_element_f_int",
     "pm_last_error_location", "pm_linear_solve_noalloc", "pm_server_create", "pm_server_submit", "pm_server_poll",
-    "pm_server_wait", "pm_server_destroy", "pm_server_stats",
+    "pm_server_wait", "pm_server_destroy", "pm_server_stats", "pm_set_model_settings",
 ]
 
 
@@ -85,6 +85,7 @@ def lib():
     L.pm_fem_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int64, _dp, C.c_int64, _ip, _ip, _lp, _ip]
     L.pm_fem_destroy.argtypes = [C.c_void_p]
     L.pm_set_param_solid.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]
+    L.pm_set_model_settings.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double]
     L.pm_set_idealization.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double]
     L.pm_set_config.argtypes = [C.c_void_p, C.POINTER(PmConfig)]
     L.pm_set_essential.argtypes = [C.c_void_p, C.c_int64, _ip, _ip, _dp, C.c_int32]
@@ -188,6 +189,8 @@ class Fem:
         self.h = h
         for a, (model, E, nu, H, z0, ng) in d["params"].items():
             self._check(L.pm_set_param_solid(h, int(a), int(model), E, nu, H, z0, int(ng)))
+        for a, (gp, nn_max, beta) in d.get("model_settings", {}).items():
+            self._check(L.pm_set_model_settings(h, int(a), int(gp), int(nn_max), float(beta)))
         self._check(L.pm_set_idealization(h, int(d["axisym"]), int(d["plane_stress"]), float(d["thickness"])))
         self.cfg = pm_config if pm_config is not None else make_config(d, cfg, keep_history)
         self._check(L.pm_set_config(h, C.byref(self.cfg)))

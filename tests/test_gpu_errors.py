@@ -77,3 +77,40 @@ def test_pcg_failure_falls_back_to_host_direct():
     g.solve()
     assert g.stats()["n_host_fallbacks"] == 0
     assert rel_err(f.history()[-1][1], g.history()[-1][1]) < 1e-10
+
+
+def test_general_plasticity_has_no_stiffness():
+    """Settings::set_general_plasticity routes von Mises to pmsim's Elastoplastic model
+    (model_stress_strain.rs:65-71), whose stiffness returns "TODO" (elastoplastic.rs:528-536): the solve
+    stops at the first tangent with that text.  A linear-elastic attribute ignores the setting."""
+    mesh, base, cfg, ess, nat = cases.footing_c1(n_inc=2)
+    cfg.update_model_settings(1).set_general_plasticity(True)
+    assert _fem_error((mesh, base, cfg, ess, nat)) == "TODO"
+    f = Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    for call in (f.assemble_kk, f.element_pass):
+        with pytest.raises(PmError, match="^TODO$"):
+            call()
+    with pytest.raises(PmError, match="general-plasticity"):
+        f.update_secondary_values()
+    f.assemble_f_int()  # the internal forces of the current state need no stress update
+    mesh, base, cfg, ess, nat = _one_qua4(StressStrain.LinearElastic(1000.0, 0.25))
+    f0 = Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    f0.solve()
+    cfg.update_model_settings(1).set_general_plasticity(True).set_gp_interp_nn_max(20)
+    f1 = Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    f1.solve()
+    assert np.array_equal(f0.get_u()[0], f1.get_u()[0])
+
+
+@pytest.mark.parametrize("setting, printed", [
+    (lambda s: s.set_nle_beta(-1.0), "ERROR: nle_beta = -1.0 is incorrect; it must be ≥ 0.0"),
+    (lambda s: s.set_gp_interp_nn_max(0), "ERROR: gp_interp_nn_max = 0 is incorrect; it must be ≥ 1"),
+])
+def test_model_settings_are_validated(setting, printed, capfd):
+    """settings.rs:55-66 through ModelStressStrain::new (model_stress_strain.rs:53-58), for any model."""
+    mesh, base, cfg, ess, nat = _one_qua4(StressStrain.LinearElastic(1000.0, 0.25))
+    setting(cfg.update_model_settings(1))
+    with pytest.raises(PmError) as e:
+        Fem(describe(mesh, base, cfg, ess, nat), cfg)
+    assert str(e.value) == "cannot allocate ModelStressStrain because settings.validate() failed"
+    assert printed in capfd.readouterr().out

@@ -171,3 +171,18 @@ def test_mixed_2d_fractions():
     assert abs(m.ncell - 4000) < 200
     for k, want in [(M.QUA8, .30), (M.QUA4, .25), (M.TRI6, .20), (M.QUA9, .15), (M.TRI15, .10)]:
         assert abs(frac[k] - want) < 0.04
+
+
+def test_model_settings_in_the_description():
+    """Config::update_model_settings (config.rs:515-517): created with settings.rs's defaults on first use,
+    the same object afterwards; describe() hands each attribute's settings to the C ABI."""
+    from pmsim_amd import Settings
+    mesh, base, cfg, ess, nat = cases.footing_c1(n_inc=2)
+    d0 = describe(mesh, base, cfg, ess, nat)
+    assert d0["model_settings"] == {}
+    s = cfg.update_model_settings(1)
+    assert isinstance(s, Settings) and s.gp_interp_nn_max == 30 and s.gp_ode_method == "DoPri8" and not s.general_plasticity
+    assert cfg.update_model_settings(1).set_general_plasticity(True).set_gp_interp_nn_max(20) is s
+    assert describe(mesh, base, cfg, ess, nat)["model_settings"] == {1: (1, 20, 0.0)}
+    with pytest.raises(ValueError):
+        s.set_gp_interp_nn_max(-1)
