@@ -119,7 +119,7 @@ __device__ __forceinline__ int64_t xcd_block() {
 // kernel arguments of one element group (or one handle, for the gathers)
 struct UpdArgs {
   int64_t nel; int nip, epb;
-  const int32_t* conn; const double* X; const double* ddu; const double* tab; const double* st_in;
+  const int32_t* conn; const double* geo; const double* ddu; const double* tab; const double* st_in;
   double* st_out; double* fe; Mat mp; int axisym; double thickness; int do_update; int32_t* err; uint64_t ekey;
 };
 template <typename KT>
@@ -143,7 +143,7 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   const int64_t nel = A.nel;
   const int nip = A.nip, epb = A.epb;
   const int32_t* __restrict__ conn = A.conn;
-  const double* __restrict__ X = A.X;
+  const double* __restrict__ geo = A.geo;
   const double* __restrict__ ddu = A.ddu;
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ st_in = A.st_in;
@@ -160,8 +160,7 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   const double* __restrict__ tw = tab;
   const double* __restrict__ tN = tab + nip;
   const double* __restrict__ tdN = tN + nip * NN;
-  double* sX = sh;
-  double* sU = sX + epb * NN * ND;
+  double* sU = sh;
   double* sS = sU + epb * NN * ND;
   const int tid = threadIdx.x;
   const int64_t e0 = blk * epb;
@@ -177,24 +176,31 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   for (int c = 0; c < NS; c++) sig[c] = mine ? st_in[c * nipg + ip] : 0.0;
   double z = mine ? st_in[NS * nipg + ip] : 0.0, lam = mine ? st_in[(NS + 1) * nipg + ip] : 0.0;
   double el_ = mine ? st_in[(NS + 2) * nipg + ip] : 0.0, ep = mine ? st_in[(NS + 3) * nipg + ip] : 0.0;
+  // J^-1, det J and radius of the IP (formed once for the fixed mesh, k_geometry)
+  constexpr int GW = ND * ND + 2;
+  double Ji[ND][ND], det = 1.0, r = 0.0;
+  if (mine) {
+    const double* gq = geo + ip * GW;
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) Ji[i][j] = gq[i * ND + j];
+    det = gq[ND * ND];
+    r = gq[ND * ND + 1];
+  }
   for (int i = tid; i < epb * NN; i += blockDim.x) {
     int64_t e1 = e0 + i / NN;
     if (e1 < nel) {
       int64_t p = conn[e0 * NN + i];
 #pragma unroll
-      for (int d = 0; d < ND; d++) { sX[i * ND + d] = X[p * ND + d]; sU[i * ND + d] = ddu[p * ND + d]; }
+      for (int d = 0; d < ND; d++) sU[i * ND + d] = ddu[p * ND + d];
     }
   }
   __syncthreads();
   if (mine) {
     const double* dN = tdN + q * NN * ND;
-    const double* Xe = sX + el * NN * ND;
     const double* Ue = sU + el * NN * ND;
-    double Ji[ND][ND], det;
-    jacobian<ND, NN>(Xe, dN, Ji, det);
     if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)(e * nip + q));
-    double r = 0.0;
-    if (axisym) for (int m = 0; m < NN; m++) r += tN[q * NN + m] * Xe[m * ND];
     // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
     // shear sums are accumulated as 2 e_ij (one FMA per term) and halved once
     double exx = 0, eyy = 0, ezz = 0, exy = 0, eyz = 0, exz = 0;
@@ -769,7 +775,7 @@ constexpr size_t SHM_CAP = 60 * 1024;  // dynamic LDS budget per workgroup (stat
 
 static size_t shm_update(int nd, int nn, int nip, int epb) {
   int ns = nd == 2 ? 4 : 6;
-  return sizeof(double) * (2 * (size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
+  return sizeof(double) * ((size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
 }
 static int epb_update(int nd, int nn, int nip) {
   int epb = std::max(1, TPB / nip);  // the update kernel takes one IP per thread: epb * nip <= TPB (nip <= 27 here)
@@ -820,7 +826,7 @@ static int split_tangent(int nd, int nn, int nip, int kt_bytes) {
 static UpdArgs upd_args(pm_fem* f, Group& g, bool do_update, int epb) {
   UpdArgs A;
   A.nel = g.nel; A.nip = g.nip; A.epb = epb;
-  A.conn = g.d_conn; A.X = f->d_X; A.ddu = f->d_ddu; A.tab = g.d_tab;
+  A.conn = g.d_conn; A.geo = g.d_geo; A.ddu = f->d_ddu; A.tab = g.d_tab;
   A.st_in = do_update ? g.d_backup : g.d_state; A.st_out = g.d_state; A.fe = f->d_fe + g.fe_base;
   A.mp = mat_of(f, g); A.axisym = f->axisym ? 1 : 0; A.thickness = f->thickness; A.do_update = do_update ? 1 : 0;
   A.err = f->d_err; A.ekey = group_key(f, g);
