@@ -10,11 +10,11 @@
 // id), so the product comes out directly in the order the deterministic scatter reads.
 //
 // Organisation (persistent): each workgroup owns a contiguous range of units of EPW elements and keeps
-// the shape-function table in LDS.  While one unit is processed, the coordinates, state and ranks of
-// the next unit are already in flight in registers.  Per element: one lane per IP forms J, its inverse
-// and the D coefficients; all threads of the element stage the gradient rows; WPE waves share the
-// upper tiles (tile t on wave t % WPE); the epilogue parks M in LDS, forms K, stages the node blocks
-// in LDS and writes the element matrix contiguously.
+// the shape-function table in LDS.  While one unit is processed, the IP geometry (J^-1, det J, radius,
+// formed once for the fixed mesh), state and ranks of the next unit are already in flight in registers.
+// Per element: one lane per IP forms the D coefficients; all threads of the element stage the gradient
+// rows; WPE waves share the upper tiles (tile t on wave t % WPE); the epilogue parks M in LDS, forms K,
+// stages the node blocks in LDS and writes the element matrix contiguously.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,11 +81,13 @@ struct MShape {
   static constexpr int NTHR = TPE * EPW;
   static constexpr int QW = 3 + ND * ND + 6 + 1;  // c alpha, c beta, c gamma, J^-1, s, r
   static constexpr int SV = NS + 2;               // state values read per IP: sigma, lambda, elastic flag (z skipped)
+  static constexpr int GW = ND * ND + 2;          // geometry values per IP: J^-1, det J, radius
   static constexpr int MINW = ND == 3 ? 3 : 4;      // waves per SIMD the register budget is sized for
 };
 
 // LDS, in doubles: [table: w | dN (stride DSN per IP) | N (axisymmetric)] then EPW element slots of
-// [X | state | per-IP data | union(rows + weights of a chunk, M tiles, node-block staging)]
+// [state | per-IP data | union(IP geometry until the per-IP data is formed, rows + weights of a chunk,
+// M tiles, node-block staging)]
 struct MLayout {
   int rsk, nrow, dsn;            // row-slot stride per column (one row kind), row kinds, dN stride per IP
   size_t tdn, tn, tab;           // table offsets and size
@@ -112,7 +114,7 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   L.tn = L.tdn + (size_t)nip * L.dsn;
   L.tab = L.tn + (axisym ? (size_t)nip * NN : 0);
   L.tab = (L.tab + 1) & ~(size_t)1;  // element slots and their union start 16-byte aligned
-  L.s = (size_t)NN * ND;
+  L.s = 0;
   L.q = L.s + (size_t)nip * S::SV;
   L.uni = (L.q + (size_t)nip * S::QW + 1) & ~(size_t)1;
   const size_t rows = PREC == 2 ? (size_t)L.nrow * S::NCP * L.rsk * sizeof(short)
@@ -121,6 +123,7 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   const size_t kst = (size_t)packed_ke_values(NN, ND) * sizeof(T);
   size_t u = rows > mt ? rows : mt;
   u = u > kst ? u : kst;
+  u = u > (size_t)nip * S::GW * sizeof(double) ? u : (size_t)nip * S::GW * sizeof(double);
   L.per_el = (L.uni + (u + sizeof(double) - 1) / sizeof(double) + 1) & ~(size_t)1;
   L.total = L.tab + (size_t)S::EPW * L.per_el;
   return L;
@@ -128,15 +131,15 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
 
 template <int ND, int NN, int PREC>
 __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) void k_tangent_mfma(
-    int64_t nel, int nip, int ipc, int64_t units_per_wg, const int32_t* __restrict__ conn, const uint8_t* __restrict__ rank,
-    const double* __restrict__ X, const double* __restrict__ tab, const double* __restrict__ st,
+    int64_t nel, int nip, int ipc, int64_t units_per_wg, const uint8_t* __restrict__ rank,
+    const double* __restrict__ geo, const double* __restrict__ tab, const double* __restrict__ st,
     typename MfmaT<PREC>::T* __restrict__ ke, MatM mp, int axisym, double thickness, int32_t* err, uint64_t ekey) {
   using S = MShape<ND, NN>;
   using MT = MfmaT<PREC>;
   using T = typename MT::T;
   using Acc = typename MT::Acc;
   constexpr int NS = S::NS, NC = S::NC, NTI = S::NTI, NCP = S::NCP, NTILE = S::NTILE, WPE = S::WPE, EPW = S::EPW;
-  constexpr int TPW = S::TPW, TPE = S::TPE, NTHR = S::NTHR, QW = S::QW, SV = S::SV;
+  constexpr int TPW = S::TPW, TPE = S::TPE, NTHR = S::NTHR, QW = S::QW, SV = S::SV, GW = S::GW;
   constexpr int KS = MT::KS, B2 = ND * ND, EW = packed_ke_values(NN, ND), EU = NN * (NN + 1) / 2 * B2;
   extern __shared__ double shm[];
   __shared__ uint8_t srank[EPW * NN];
@@ -160,50 +163,47 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
   }
   if (axisym) for (int i = tid; i < nip * NN; i += NTHR) sN[i] = tab[nip + i];
 
-  // prefetch roles (fixed per thread): one coordinate (xe, xm, xd) and one state value (se, component, q)
-  const bool xrole = tid < EPW * NN * ND;
-  const int xe = tid / (NN * ND), xm = (tid - xe * NN * ND) / ND, xd = tid - xe * NN * ND - xm * ND;
-  const int xoff = xe * NN + xm;
+  // prefetch roles (fixed per thread): up to two geometry values (element ge0 / ge1, both units of the
+  // unit's contiguous geometry; EPW * nip * GW <= 2 NTHR, checked on the host), one rank (thread < EPW NN)
+  // and one state value (se, component, q)
+  const int gpe = nip * GW;
+  const int ge0 = tid / gpe, ge1 = (tid + NTHR) / gpe;
+  const bool grole0 = ge0 < EPW, grole1 = ge1 < EPW;
+  const int gl0 = ge0 * (int)L.per_el + (int)L.uni + (tid - ge0 * gpe);
+  const int gl1 = ge1 * (int)L.per_el + (int)L.uni + (tid + NTHR - ge1 * gpe);
+  const bool rrole = tid < EPW * NN;
+  const int re = tid / NN;
   const bool srole = tid < EPW * SV * nip;
   const int se = tid / (SV * nip), sc = (tid - se * SV * nip) / nip;
   const int64_t soff = (int64_t)(sc < NS ? sc : sc + 1) * nipg + (tid - se * SV * nip - sc * nip) + (int64_t)se * nip;  // skips z
   const int sl = se * (int)(L.per_el) + (int)L.s + (tid - se * SV * nip);  // LDS target of the state value
-  int rc = 0;
-  double rx = 0.0, rs = 0.0;
+  double rg0 = 0.0, rg1 = 0.0, rs = 0.0;
   uint8_t rr = 0;
   // The prefetch loads are issued unconditionally (surplus lanes re-read a valid entry) so the number of
   // memory operations after each is static and waiting for a prefetch never waits for earlier stores.
   // Offsets pass through an empty asm so that no 64-bit address is kept live, and spilled, across units.
-  auto load_conn = [&](int64_t u) {
+  auto load_unit = [&](int64_t u) {
     const int n = (int)min((int64_t)EPW, nel - u * EPW);
-    int xo = (xrole && xe < n) ? xoff : 0;
-    asm volatile("" : "+v"(xo));
-    rc = conn[u * (EPW * NN) + xo];
-  };
-  auto load_rest = [&](int64_t u) {
-    const int n = (int)min((int64_t)EPW, nel - u * EPW);
-    int xo = (xrole && xe < n) ? xoff : 0;
+    int go0 = (grole0 && ge0 < n) ? tid : 0, go1 = (grole1 && ge1 < n) ? tid + NTHR : 0;
+    int ro = (rrole && re < n) ? tid : 0;
     int64_t so = (srole && se < n) ? soff : 0;
-    asm volatile("" : "+v"(xo), "+v"(so));
-    rx = X[(int64_t)rc * ND + xd];
-    rr = rank[u * (EPW * NN) + xo];
+    asm volatile("" : "+v"(go0), "+v"(go1), "+v"(ro), "+v"(so));
+    const double* gu = geo + u * (EPW * (int64_t)gpe);
+    rg0 = gu[go0];
+    rg1 = gu[go1];
+    rr = rank[u * (EPW * NN) + ro];
     rs = st[so + u * (EPW * (int64_t)nip)];
   };
-  load_conn(u0);
-  load_rest(u0);
-  load_conn(min(u0 + 1, u1 - 1));
+  load_unit(u0);
 
   for (int64_t u = u0; u < u1; u++) {
     const int nloc = (int)min((int64_t)EPW, nel - u * EPW);
     __syncthreads();  // previous unit finished with LDS (and the table is in place)
-    if (xrole && xe < nloc) {
-      double* dX = shm + L.tab + xe * L.per_el;
-      dX[xm * ND + xd] = rx;
-      if (xd == 0) srank[xe * NN + xm] = rr;
-    }
+    if (grole0 && ge0 < nloc) shm[L.tab + gl0] = rg0;
+    if (grole1 && ge1 < nloc) shm[L.tab + gl1] = rg1;
+    if (rrole && re < nloc) srank[tid] = rr;
     if (srole && se < nloc) shm[L.tab + sl] = rs;
-    load_rest(min(u + 1, u1 - 1));  // past the range: a harmless repeat
-    load_conn(min(u + 2, u1 - 1));
+    load_unit(min(u + 1, u1 - 1));  // past the range: a harmless repeat
     __syncthreads();
     // Per-thread indices are re-derived each unit from an opaque copy of the thread id, so the compiler
     // does not hoist every loop-invariant LDS address out of the unit loop (that would exhaust registers).
@@ -212,7 +212,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
     const int l = tq & 63, el = (tq >> 6) / WPE, wv = (tq >> 6) - el * WPE, te = tq - el * TPE;
     const bool active = EPW == 1 || el < nloc;  // a unit of one element is always full
     double* slot = shm + L.tab + el * L.per_el;  // this thread's element slot
-    double* sX = slot;
+    const double* sG = slot + L.uni;  // IP geometry: [nip][GW], until the per-IP data is formed
     double* sS = slot + L.s;  // state: [SV][nip]
     double* sQ = slot + L.q;
     T* sL = reinterpret_cast<T*>(slot + L.uni);  // row kind k: sL[(k * NCP + col) * RSK + slot]; weights after
@@ -225,45 +225,13 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
       tIv[s] = I; tJv[s] = I + t;
     }
 
-    // J entries over all threads of the element (one per IP and entry, the sum over the nodes in
-    // ascending order), parked where J^-1 goes
-    if (active)
-      for (int w = te; w < nip * ND * ND; w += TPE) {
-        const int q = w / (ND * ND), ij = w - q * ND * ND, i = ij / ND, j = ij - i * ND;
-        const double* dN = sdN + q * DSN;
-        double sum = 0.0;
-#pragma unroll 4
-        for (int m = 0; m < NN; m++) sum += sX[m * ND + i] * dN[m * ND + j];
-        sQ[q * QW + 3 + ij] = sum;
-      }
-    __syncthreads();
-    // one lane per IP: J^-1, c times the D coefficients (von_mises.rs:124-167), deviator, radius
+    // one lane per IP: c times the D coefficients (von_mises.rs:124-167), deviator; J^-1 and radius copied
     if (active && wv == 0 && l < nip) {
       const int q = l;
-      double J[ND][ND];
-#pragma unroll
-      for (int i = 0; i < ND; i++)
-#pragma unroll
-        for (int j = 0; j < ND; j++) J[i][j] = sQ[q * QW + 3 + i * ND + j];
-      double Ji[ND][ND], det;
-      if constexpr (ND == 2) {
-        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        const double id = 1.0 / det;
-        Ji[0][0] = J[1][1] * id; Ji[0][1] = -J[0][1] * id; Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
-      } else {
-        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-        const double id = 1.0 / det;
-        Ji[0][0] = c00 * id; Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id; Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-        Ji[1][0] = c01 * id; Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id; Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-        Ji[2][0] = c02 * id; Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id; Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-      }
+      const double* gq = sG + q * GW;
+      const double det = gq[ND * ND], r = gq[ND * ND + 1];
       const uint64_t ikey = ekey + (uint64_t)((u * EPW + el) * nip + q);
       if (!(det > 0.0)) raise_error(err, ERR_DETJ, ikey);
-      double r = 0.0;
-      if (axisym) for (int m = 0; m < NN; m++) r += sN[q * NN + m] * sX[m * ND];
       const double coef = det * sw[q] * thickness * (axisym ? r : 1.0);
       double alpha, beta, gamma = 0.0;
       double s[6] = {0, 0, 0, 0, 0, 0};
@@ -295,9 +263,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
       Q[1] = coef * beta;
       Q[2] = coef * gamma;
 #pragma unroll
-      for (int i = 0; i < ND; i++)
-#pragma unroll
-        for (int j = 0; j < ND; j++) Q[3 + i * ND + j] = Ji[i][j];
+      for (int ij = 0; ij < ND * ND; ij++) Q[3 + ij] = gq[ij];
 #pragma unroll
       for (int c = 0; c < 6; c++) Q[3 + ND * ND + c] = s[c];
       Q[3 + ND * ND + 6] = r;
@@ -595,6 +561,7 @@ bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
   const int64_t nwg = std::min<int64_t>(nunits, per_cu * f->n_cu);
   const int64_t upw = (nunits + nwg - 1) / nwg;
   const int64_t grid = (nunits + upw - 1) / upw;
+  if ((int64_t)S::EPW * g.nip * S::GW > 2 * (int64_t)S::NTHR) return false;  // the geometry prefetch roles
   if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(NN, ND), f->n_ke_values)) {
     f->launch_refused = true;  // reported by check_device_error; never launched out of range
     return true;
@@ -603,7 +570,7 @@ bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
   if constexpr (PREC == 0) ke = f->d_ke + g.ke_base;
   else ke = f->d_ke32 + g.ke_base;
   hipLaunchKernelGGL((k_tangent_mfma<ND, NN, PREC>), dim3((unsigned)grid), dim3(S::NTHR), shm, f->stream, g.nel, g.nip, ipc, upw,
-                     g.d_conn, g.d_rank, f->d_X, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err,
+                     g.d_rank, g.d_geo, g.d_tab, g.d_state, ke, mp, ax, f->thickness, f->d_err,
                      (uint64_t)(&g - f->groups.data()) << kErrKeyShift);
   return true;
 }
