@@ -82,10 +82,12 @@ struct MShape {
   static constexpr int QW = 3 + ND * ND + 6 + 1;  // c alpha, c beta, c gamma, J^-1, s, r
   static constexpr int SV = NS + 2;               // state values read per IP: sigma, lambda, elastic flag (z skipped)
   static constexpr int GW = ND * ND + 2;          // geometry values per IP: J^-1, det J, radius
-  static constexpr int MINW = ND == 3 ? 3 : 4;      // waves per SIMD the register budget is sized for
 };
 
-// LDS, in doubles: [table: w | dN (stride DSN per IP) | N (axisymmetric)] then EPW element slots of
+// waves per SIMD the register budget is sized for: four (128 VGPRs) except for the 3D FP64 kernels
+template <int ND, int PREC> constexpr int mfma_minw() { return ND == 3 && PREC == 0 ? 3 : 4; }
+
+// LDS, in doubles: [table: w | N (axisymmetric)] then EPW element slots of
 // [state | per-IP data | union(IP geometry until the per-IP data is formed, rows + weights of a chunk,
 // M tiles, node-block staging)]
 struct MLayout {
@@ -111,7 +113,7 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
   }
   L.dsn = NN * ND + 1;
   L.tdn = (size_t)nip;
-  L.tn = L.tdn + (size_t)nip * L.dsn;
+  L.tn = L.tdn;  // dN is read through the L1 (it is shared by every workgroup)
   L.tab = L.tn + (axisym ? (size_t)nip * NN : 0);
   L.tab = (L.tab + 1) & ~(size_t)1;  // element slots and their union start 16-byte aligned
   L.s = 0;
@@ -130,7 +132,7 @@ __host__ __device__ inline MLayout mfma_layout(int nip, int ipc, int axisym) {
 }
 
 template <int ND, int NN, int PREC>
-__global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) void k_tangent_mfma(
+__global__ __launch_bounds__((MShape<ND, NN>::NTHR), (mfma_minw<ND, PREC>())) void k_tangent_mfma(
     int64_t nel, int nip, int ipc, int64_t units_per_wg, const uint8_t* __restrict__ rank,
     const double* __restrict__ geo, const double* __restrict__ tab, const double* __restrict__ st,
     typename MfmaT<PREC>::T* __restrict__ ke, MatM mp, int axisym, double thickness, int32_t* err, uint64_t ekey) {
@@ -144,7 +146,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
   extern __shared__ double shm[];
   __shared__ uint8_t srank[EPW * NN];
   const MLayout L = mfma_layout<ND, NN, PREC>(nip, ipc, axisym);
-  const int RSK = L.rsk, NR = L.nrow, DSN = L.dsn;
+  const int RSK = L.rsk, NR = L.nrow;
   const int tid = threadIdx.x, wave = tid >> 6;
   (void)wave;
   const int64_t nunits = (nel + EPW - 1) / EPW;
@@ -154,13 +156,9 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
   const int64_t nipg = nel * nip;
 
   double* sw = shm;
-  double* sdN = shm + L.tdn;
+  const double* __restrict__ tdN = tab + nip + nip * NN;
   double* sN = shm + L.tn;
   for (int i = tid; i < nip; i += NTHR) sw[i] = tab[i];
-  for (int i = tid; i < nip * NN * ND; i += NTHR) {
-    const int q = i / (NN * ND);
-    sdN[q * DSN + (i - q * NN * ND)] = tab[nip + nip * NN + i];
-  }
   if (axisym) for (int i = tid; i < nip * NN; i += NTHR) sN[i] = tab[nip + i];
 
   // prefetch roles (fixed per thread): up to two geometry values (element ge0 / ge1, both units of the
@@ -287,7 +285,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         for (int w = te; w < nv * NN; w += TPE) {
           const int ql = w / NN, m = w - ql * NN, q = ch * ipc + ql;
           const double* Q = sQ + q * QW;
-          const double* dN = sdN + q * DSN + m * ND;
+          const double* dN = tdN + (q * NN + m) * ND;
           double G[ND];
 #pragma unroll
           for (int j = 0; j < ND; j++) {
@@ -347,7 +345,7 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (MShape<ND, NN>::MINW)) voi
         for (int w = te; w < nv * NN; w += TPE) {
           const int ql = w / NN, m = w - ql * NN, q = ch * ipc + ql;
           const double* Q = sQ + q * QW;
-          const double* dN = sdN + q * DSN + m * ND;
+          const double* dN = tdN + (q * NN + m) * ND;
           double G[ND];
 #pragma unroll
           for (int j = 0; j < ND; j++) {
@@ -557,7 +555,11 @@ bool launch_mfma_kind(pm_fem* f, Group& g, const MatM& mp) {
   const size_t shm = mfma_layout<ND, NN, PREC>(g.nip, ipc, ax).total * sizeof(double);
   if (shm > MFMA_LDS_MAX) return false;
   const int64_t nunits = (g.nel + S::EPW - 1) / S::EPW;
-  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(f->lds_per_cu / (int64_t)(shm + 1024), 32 / (S::WPE * S::EPW)));
+  // resident workgroups per CU (registers and LDS), so that the persistent grid is one wave of workgroups
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&k_tangent_mfma<ND, NN, PREC>), S::NTHR, shm) != hipSuccess || occ < 1)
+    occ = (int)std::max<int64_t>(1, f->lds_per_cu / (int64_t)(shm + 1024));
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(occ, 32 / (S::WPE * S::EPW)));
   const int64_t nwg = std::min<int64_t>(nunits, per_cu * f->n_cu);
   const int64_t upw = (nunits + nwg - 1) / nwg;
   const int64_t grid = (nunits + upw - 1) / upw;
