@@ -408,8 +408,8 @@ def test_hex20_full_size_consistent_tangent():
 
 def test_hex20_96_fp32_tangent_full_size_properties():
     """The 96x96x48 Hex20 cube of the precision study (442,368 elements, 11.9 M IPs, 5.48 M DOFs) after the same
-    partly plastic increment, FP32 element tangents (matrix cores, K held in FP32) against FP64 ones: K32 v
-    within 1e-5 of K64 v (relative, in the max norm; FP32 products and accumulation), f_int bit for bit the FP64
+    partly plastic increment, FP32 element tangents (the FP32 default, K held in FP32) against FP64 ones: K32 v
+    within 1e-5 of K64 v (relative, in the max norm; FP32 storage), f_int bit for bit the FP64
     handle's (the stress update and B^T sigma stay FP64 whatever the tangent precision), and two FP32 passes
     bitwise identical."""
     mesh = G.hex20_block(96, 96, 48, seed=20261203, jitter=0.05)

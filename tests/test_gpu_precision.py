@@ -3,9 +3,10 @@
 The precision forms of the 96x96x48 Hex20 cube, at sizes the oracle solves in seconds: the same jittered
 unit cube and material (von Mises E=2e5, nu=0.3, z0=250, H=1e3, 27 IPs), bottom fixed and the top
 pressed down in 10 increments of uz = -0.001 (the loading of the 64x64x32 cube, uz -> -0.01 in 10
-increments), so most IPs end plastic.  The element tangents are formed on the matrix cores in FP32,
-or from BF16 operands with FP32 accumulation; the residual, the return map, the PCG vectors and the
-convergence tests stay FP64, so Newton converges to the FP64 fixed point.
+increments), so most IPs end plastic.  The element tangents are held in FP32 (the FP32 default: the
+vector kernel's products rounded to FP32 when stored, as is K) or formed on the matrix cores from BF16
+operands with FP32 accumulation; the residual, the return map, the PCG vectors and the convergence tests
+stay FP64, so Newton converges to the FP64 fixed point.
 
 Measured on MI355X (largest relative difference to the oracle over all increments, Newton iterations
 against the oracle's 42 / 43):
