@@ -122,7 +122,7 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   if (c->dt <= 0.0) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
-  if (c->tangent_kernel < 0 || c->tangent_kernel > 3) { f->err = "unknown tangent kernel"; return 1; }
+  if (c->tangent_kernel < 0 || c->tangent_kernel > 4) { f->err = "unknown tangent kernel"; return 1; }
   const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   if (f->initialized) release_graphs(f);  // captured launches depend on the configuration
   f->cfg = *c;
@@ -400,7 +400,7 @@ int pm_get_element_kk(const pm_fem* f, int64_t cell, double* kk) {
   for (int a = 0; a < nn; a++)
     for (int b = 0; b < nn; b++) {
       const bool up = rk[a] <= rk[b];
-      const double* blk = v.data() + (size_t)packed_block(nn, up ? rk[a] : rk[b], up ? rk[b] : rk[a]) * nd * nd;
+      const double* blk = v.data() + (size_t)ke_block_offset(nn, nd, up ? rk[a] : rk[b], up ? rk[b] : rk[a]);
       for (int i = 0; i < nd; i++)
         for (int j = 0; j < nd; j++) kk[(size_t)(a * nd + i) * ne + b * nd + j] = up ? blk[i * nd + j] : blk[j * nd + i];
     }

@@ -34,6 +34,7 @@ int device_alloc(pm_fem* f) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, f->device) == hipSuccess && v > 0) f->n_cu = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, f->device) == hipSuccess && v > 0) f->lds_per_cu = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, f->device) == hipSuccess && v > 0) f->lds_per_block = v;
   }
   const int nd = f->ndim, ns = f->nsig;
   const int64_t b2 = (int64_t)nd * nd;
@@ -100,7 +101,8 @@ int device_alloc(pm_fem* f) {
       dalloc(f, &f->d_pcg, 8 * n + 2 * npr) || dalloc(f, &f->d_binv, f->npoint * b2))
     return 1;
   const int64_t gn = (f->npoint + 255) / 256;
-  if (dalloc(f, &f->d_red, std::max<int64_t>(4096 + 2 * std::min<int64_t>(gn, 2048), 1024) + 64) || dalloc(f, &f->d_scal, 32) || dalloc(f, &f->d_err, 4)) return 1;
+  if (dalloc(f, &f->d_red, std::max<int64_t>(4096 + 2 * std::min<int64_t>(gn, 2048), 1024) + 64) || dalloc(f, &f->d_scal, 32) || dalloc(f, &f->d_err, 4) ||
+      dalloc(f, &f->d_work, 64)) return 1;
   std::vector<double> freev(n, 1.0);
   for (auto eq : f->prescribed_eq) freev[eq] = 0.0;
   if (upload(f, f->d_free, freev.data(), n)) return 1;
@@ -159,7 +161,7 @@ void device_free(pm_fem* f) {
   fr(f->d_X); fr(f->d_urp); fr(f->d_ucol); fr(f->d_lrp); fr(f->d_lcol); fr(f->d_lidx); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
   fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
-  fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err);
+  fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err); fr(f->d_work);
   fr(f->d_arc); f->d_arc = nullptr;
   fr(f->d_frp); fr(f->d_fcol); fr(f->d_f2u); fr(f->d_kfull); fr(f->d_kfull32);
   f->d_frp = f->d_fcol = f->d_f2u = nullptr; f->d_kfull = nullptr; f->d_kfull32 = nullptr;
@@ -168,7 +170,7 @@ void device_free(pm_fem* f) {
   f->d_X = nullptr; f->d_urp = f->d_ucol = f->d_lrp = f->d_lcol = f->d_lidx = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;
   f->d_kmap_ptr = f->d_kmap = nullptr; f->d_korder = nullptr; f->d_fe = nullptr; f->d_fmap_ptr = f->d_fmap = nullptr;
   f->d_u = f->d_ddu = f->d_fint = f->d_fext = f->d_rr = f->d_mdu = f->d_mdu0 = f->d_lam = nullptr;
-  f->d_y = f->d_cval = nullptr; f->d_peq = nullptr; f->d_free = f->d_pcg = f->d_binv = f->d_red = f->d_scal = nullptr; f->d_err = nullptr;
+  f->d_y = f->d_cval = nullptr; f->d_peq = nullptr; f->d_free = f->d_pcg = f->d_binv = f->d_red = f->d_scal = nullptr; f->d_err = nullptr; f->d_work = nullptr;
 }
 
 }  // namespace pm

@@ -256,7 +256,7 @@ int build_patterns(pm_fem* f) {
           if (pts[la] > pts[lb]) continue;
           const int ra = rk[la], rb = rk[lb];
           if (ra > rb) { f->err = "internal error: element ranks out of order"; return 1; }
-          const int64_t off = g.ke_base + el * g.kew + (int64_t)packed_block((int)n, ra, rb) * g.ndim * g.ndim;
+          const int64_t off = g.ke_base + el * g.kew + (int64_t)ke_block_offset((int)n, g.ndim, ra, rb);
           kmap[fill[pairblk[w++]]++] = (int32_t)off;
         }
     }

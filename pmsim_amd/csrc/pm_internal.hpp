@@ -61,13 +61,22 @@ struct ModelSettings {
 struct PcgEngine;
 struct PassEngine;
 
-// Element matrices are stored as their upper node blocks (a <= b in rank order: nodes sorted by global
-// id), row after row: block (a, b) starts at packed_block(nn, a, b) * nd * nd, each block row-major.
-// The lower blocks are the transposes.  Each element's storage is padded to a multiple of 4 values
-// so that every element starts on a 16-byte boundary in FP32 and FP64.
+// Element matrices are stored as their upper node blocks in rank order (a <= b, the nodes of a cell sorted
+// by global id), row after row, each block row-major; the lower blocks are the transposes.  Rank order makes
+// the stored blocks exactly the upper CSR blocks they feed, and the blocks that consecutive CSR blocks of a
+// row read adjacent in memory.  Block (a, b) starts at ke_block_offset(nn, nd, a, b).  Hex20 matrices carry
+// one padding value after rank row 5, so that rows 0-5 and rows 6-19 (105 blocks each) are two halves of
+// 946 values, each a whole number of 16-byte pieces in FP64.  Each element's storage is padded to a multiple
+// of 4 values so that every element starts on a 16-byte boundary in FP32 and FP64.
 constexpr int GATHER_CHUNK = 256;  // CSR blocks per gather workgroup (its block size)
 constexpr int packed_block(int nn, int a, int b) { return a * (2 * nn - a + 1) / 2 + (b - a); }
 constexpr int packed_ke_values(int nn, int nd) { return ((nn * (nn + 1) / 2 * nd * nd + 3) / 4) * 4; }
+constexpr int ke_split_row(int nn, int nd) { return nn == 20 && nd == 3 ? 6 : nn; }
+constexpr int ke_block_offset(int nn, int nd, int a, int b) {
+  return packed_block(nn, a, b) * nd * nd + (a >= ke_split_row(nn, nd) ? 1 : 0);
+}
+static_assert(ke_block_offset(20, 3, 6, 6) == 946 && ke_block_offset(20, 3, 19, 19) + 9 == 1891 &&
+              packed_ke_values(20, 3) == 1892, "Hex20 element-matrix halves");
 // true when a group's element matrices, ew values each, lie inside the element-matrix buffer
 inline bool ke_range_ok(int64_t ke_base, int64_t nel, int kew, int ew, int64_t n_ke_values) {
   return kew == ew && ke_base >= 0 && ke_base + nel * (int64_t)ew <= n_ke_values;
@@ -163,6 +172,7 @@ struct pm_fem {
   int device = 0;
   int n_cu = 256;                      // compute units of the device
   int64_t lds_per_cu = 160 * 1024;     // LDS bytes per compute unit
+  int64_t lds_per_block = 64 * 1024;   // LDS bytes one workgroup may allocate
   hipStream_t stream = nullptr;
   bool initialized = false;
   bool released = false;               // device buffers freed after a solve; results kept below
@@ -180,7 +190,7 @@ struct pm_fem {
   float* d_kfull32 = nullptr;          // nnzb*bs*bs (FP32 / BF16 tangents)
   double* d_kval = nullptr;            // nnzu*bs*bs (FP64), FP64 tangents only
   float* d_kval32 = nullptr;           // nnzu*bs*bs (FP32), reduced-precision tangents only
-  double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order, packed (n_ke_values)
+  double* d_ke = nullptr;              // element matrices FP64: upper node blocks in rank order (ke_block_offset)
   float* d_ke32 = nullptr;             // FP32 element blocks
   int64_t n_ke_values = 0;
   bool launch_refused = false;         // a launch whose output range did not fit its buffer was skipped
@@ -211,6 +221,7 @@ struct pm_fem {
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags and first failing IP (DevErr, raise_error)
+  int32_t* d_work = nullptr;           // work counter of the persistent Hex20 tangent (reset before each launch)
   int64_t err_cell = -1;               // cell and Gauss point of the last device-side error (-1: none)
   int32_t err_gauss = -1;
   int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
@@ -238,7 +249,6 @@ namespace pm {
 // host setup (pm_host.cpp)
 int build_groups(pm_fem* f);
 int build_patterns(pm_fem* f);
-void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 void assemble_f_ext(pm_fem* f, int64_t station);
 int copy_kval_to_host(const pm_fem* f, std::vector<double>& kv);  // K block values in FP64 (widened from FP32)
 // device (pm_kernels.hip)
@@ -249,7 +259,10 @@ void launch_residual_gather(pm_fem* f);
 void launch_tangent(pm_fem* f, Group& g);
 bool gp_refuse(pm_fem* f, bool do_update, bool tangent);
 // matrix-core tangent (prec: 0 FP64, 1 FP32, 2 BF16 with FP32 accumulation); false if the kind has none
+void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
+bool launch_tangent_hex20(pm_fem* f, Group& g);
+bool tangent_hex20_fits(const pm_fem* f, const Group& g);
 // choose the tangent kernel (and work split) of every group, timing the candidates when asked to
 int tune_tangent(pm_fem* f);
 // drops every captured graph (configuration, tuning or buffers changed)

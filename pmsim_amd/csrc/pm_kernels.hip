@@ -134,9 +134,13 @@ struct ScatArgs { int64_t nnzb; const int32_t* korder; const int32_t* kptr; cons
 struct RgArgs { int64_t np; const int32_t* fptr; const int32_t* fmap; const double* fe; double* fint; };
 
 // ------------------------------------------------------------------------------------------------
-// update kernel: gather (coords, ddu) -> strain increment -> stress update from the backup state ->
-// element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
-// Shape tables are read through the vector L1 (they are a few KB and shared by all workgroups).
+// update kernel: gather of the displacement increments -> strain increment -> stress update from the
+// backup state -> element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
+// One IP per thread, epb elements per workgroup.  The workgroup's IP geometry (contiguous in HBM) and the
+// shape-function derivatives are copied into LDS with coalesced loads (the derivatives with an odd stride
+// per IP, so that the lanes of one element, which read different IPs, hit different banks).  For the force
+// loop each IP leaves T = c sigma J^-T (and the hoop term c sigma_zz / r, axisymmetric) in place of its
+// geometry, so f_e[m] = sum_q T_q dN_mq: one thread per element node, ND*ND FMA per IP.
 // ------------------------------------------------------------------------------------------------
 template <int ND, int NN>
 __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
@@ -155,52 +159,54 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   int32_t* err = A.err;
   const uint64_t ekey = A.ekey;
   constexpr int NS = ND == 2 ? 4 : 6;
-  constexpr int SW = NS + ND * ND + 1;  // per IP: c*sigma, J^-1, radius
+  constexpr int GW = ND * ND + 2;      // per IP in geo: J^-1, det J, radius
+  constexpr int DS = NN * ND + 1;      // LDS stride of the derivatives of one IP (odd)
+  static_assert(ND * ND + 1 <= GW, "T and the hoop term fit the IP's geometry slot");
   extern __shared__ double sh[];
   const double* __restrict__ tw = tab;
   const double* __restrict__ tN = tab + nip;
   const double* __restrict__ tdN = tN + nip * NN;
-  double* sU = sh;
-  double* sS = sU + epb * NN * ND;
+  double* sdN = sh;                    // [nip][DS]
+  double* sU = sdN + nip * DS;         // [epb][NN][ND] displacement increments of the block's nodes
+  double* sG = sU + epb * NN * ND;     // [epb * nip][GW] geometry, then T and the hoop term
   const int tid = threadIdx.x;
   const int64_t e0 = blk * epb;
+  const int nloc = (int)min((int64_t)epb, nel - e0);
   const int64_t nipg = nel * nip;
-  // one IP per thread (epb * nip <= blockDim); its state is requested before the gather so that the loads
-  // are in flight while the coordinates and displacements arrive and the strain is formed
+  // one IP per thread (epb * nip <= blockDim); its state is requested first, so that the loads are in flight
+  // while the workgroup's tables arrive
   const int el = tid / nip, q = tid - el * nip;
-  const int64_t e = e0 + el;
-  const bool mine = tid < epb * nip && e < nel;
-  const int64_t ip = mine ? e * nip + q : 0;
+  const bool mine = tid < nloc * nip;
+  const int64_t ip = mine ? (e0 + el) * nip + q : 0;
   double sig[NS];
 #pragma unroll
   for (int c = 0; c < NS; c++) sig[c] = mine ? st_in[c * nipg + ip] : 0.0;
   double z = mine ? st_in[NS * nipg + ip] : 0.0, lam = mine ? st_in[(NS + 1) * nipg + ip] : 0.0;
   double el_ = mine ? st_in[(NS + 2) * nipg + ip] : 0.0, ep = mine ? st_in[(NS + 3) * nipg + ip] : 0.0;
-  // J^-1, det J and radius of the IP (formed once for the fixed mesh, k_geometry)
-  constexpr int GW = ND * ND + 2;
-  double Ji[ND][ND], det = 1.0, r = 0.0;
-  if (mine) {
-    const double* gq = geo + ip * GW;
-#pragma unroll
-    for (int i = 0; i < ND; i++)
-#pragma unroll
-      for (int j = 0; j < ND; j++) Ji[i][j] = gq[i * ND + j];
-    det = gq[ND * ND];
-    r = gq[ND * ND + 1];
-  }
-  for (int i = tid; i < epb * NN; i += blockDim.x) {
-    int64_t e1 = e0 + i / NN;
-    if (e1 < nel) {
-      int64_t p = conn[e0 * NN + i];
+  {
+    const double* __restrict__ gsrc = geo + e0 * nip * GW;
+    const int ng = nloc * nip * GW;
+    for (int i = tid; i < ng; i += blockDim.x) sG[i] = gsrc[i];
+    for (int i = tid; i < nip * NN * ND; i += blockDim.x) sdN[i + i / (NN * ND)] = tdN[i];
+    for (int i = tid; i < nloc * NN; i += blockDim.x) {
+      const int64_t p = conn[e0 * NN + i];
 #pragma unroll
       for (int d = 0; d < ND; d++) sU[i * ND + d] = ddu[p * ND + d];
     }
   }
   __syncthreads();
   if (mine) {
-    const double* dN = tdN + q * NN * ND;
+    // J^-1, det J and radius of the IP (formed once for the fixed mesh, k_geometry)
+    double* gq = sG + tid * GW;
+    double Ji[ND][ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) Ji[i][j] = gq[i * ND + j];
+    const double det = gq[ND * ND], r = gq[ND * ND + 1];
+    const double* dN = sdN + q * DS;
     const double* Ue = sU + el * NN * ND;
-    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)(e * nip + q));
+    if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)ip);
     // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
     // shear sums are accumulated as 2 e_ij (one FMA per term) and halved once
     double exx = 0, eyy = 0, ezz = 0, exy = 0, eyz = 0, exz = 0;
@@ -236,50 +242,42 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
       st_out[(NS + 2) * nipg + ip] = el_;
       st_out[(NS + 3) * nipg + ip] = ep;
     }
-    double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
-    double* S = sS + (el * nip + q) * SW;
-    // c sigma, the Mandel shear components already divided by sqrt 2 (the tensor components)
-#pragma unroll
-    for (int c = 0; c < NS; c++) S[c] = c < 3 ? coef * sig[c] : (coef * sig[c]) * ISQ2;
+    const double coef = det * tw[q] * thickness * (axisym ? r : 1.0);
+    // c sigma as a tensor (the Mandel shear components divided by sqrt 2), then T = c sigma J^-T
+    double S[ND][ND];
+    S[0][0] = coef * sig[0]; S[1][1] = coef * sig[1];
+    S[0][1] = S[1][0] = (coef * sig[3]) * ISQ2;
+    if constexpr (ND == 3) {
+      S[2][2] = coef * sig[2];
+      S[1][2] = S[2][1] = (coef * sig[4]) * ISQ2;
+      S[0][2] = S[2][0] = (coef * sig[5]) * ISQ2;
+    }
 #pragma unroll
     for (int i = 0; i < ND; i++)
 #pragma unroll
-      for (int j = 0; j < ND; j++) S[NS + i * ND + j] = Ji[i][j];
-    S[NS + ND * ND] = r;
+      for (int k = 0; k < ND; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < ND; j++) t += S[i][j] * Ji[k][j];
+        gq[i * ND + k] = t;
+      }
+    gq[ND * ND] = axisym ? (coef * sig[2]) / r : 0.0;
   }
   __syncthreads();
-  for (int w = tid; w < epb * NN; w += blockDim.x) {
+  for (int w = tid; w < nloc * NN; w += blockDim.x) {
     const int el = w / NN, m = w - el * NN;
     const int64_t e = e0 + el;
-    if (e >= nel) continue;
     double fm[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) fm[d] = 0.0;
-    for (int q = 0; q < nip; q++) {
-      const double* S = sS + (el * nip + q) * SW;
-      const double* dN = tdN + (q * NN + m) * ND;
-      double G[ND];
+    const double* T = sG + el * nip * GW;
+    for (int q = 0; q < nip; q++, T += GW) {
+      const double* dN = sdN + q * DS + m * ND;
 #pragma unroll
-      for (int j = 0; j < ND; j++) {
-        double s = 0.0;
+      for (int i = 0; i < ND; i++)
 #pragma unroll
-        for (int k = 0; k < ND; k++) s += dN[k] * S[NS + k * ND + j];
-        G[j] = s;
-      }
-      if constexpr (ND == 2) {
-        const double sxy = S[3];
-        fm[0] += S[0] * G[0];
-        fm[0] += sxy * G[1];
-        if (axisym) fm[0] += S[2] * tN[q * NN + m] / S[NS + 4];
-        fm[1] += sxy * G[0];
-        fm[1] += S[1] * G[1];
-      } else {
-        const double sxy = S[3], syz = S[4], sxz = S[5];
-        // one FMA per term into the accumulators
-        fm[0] += S[0] * G[0]; fm[0] += sxy * G[1]; fm[0] += sxz * G[2];
-        fm[1] += sxy * G[0]; fm[1] += S[1] * G[1]; fm[1] += syz * G[2];
-        fm[2] += sxz * G[0]; fm[2] += syz * G[1]; fm[2] += S[2] * G[2];
-      }
+        for (int k = 0; k < ND; k++) fm[i] += T[i * ND + k] * dN[k];
+      if (axisym) fm[0] += T[ND * ND] * tN[q * NN + m];
     }
 #pragma unroll
     for (int d = 0; d < ND; d++) fe[(e * NN + m) * ND + d] = fm[d];
@@ -296,11 +294,11 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
 // Work split: one wavefront per workgroup (TW threads), EPW elements x NT upper 2x2 tiles of node blocks,
 // one thread per tile, so no phase waits on another wavefront.  The D coefficients of every IP of the
 // workgroup's elements are formed first from the state and the stored J^-1 and det J; the IPs are then
-// taken in chunks of ch: the node vectors (G, h, ts) of a chunk are staged in LDS and every thread adds
-// the chunk's contributions to its tile in registers, so no reduction across threads is needed.  The
-// element matrices, contiguous in memory, are finally staged in LDS (over the dead IP data, in at most a
-// few passes) as their upper node blocks in rank order (nodes sorted by global id; packed_block) and
-// written with 16-byte stores.
+// taken in chunks of ch: the node vectors (G, h, ts) of a chunk are staged in LDS, node m in the slot of
+// its rank (nodes sorted by global id), and every thread adds the chunk's contributions to its tile in
+// registers, so no reduction across threads is needed.  The element matrices, contiguous in memory, are
+// finally staged in LDS (over the dead IP data, in at most a few passes) as their upper node blocks in rank
+// order (ke_block_offset) and written with 16-byte stores.
 // ------------------------------------------------------------------------------------------------
 constexpr int TW = 64;  // threads per tangent workgroup: one wavefront
 
@@ -342,6 +340,8 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
   constexpr int GW = ND * ND + 2;               // per IP in geo: J^-1, det J, radius
   constexpr int EW = packed_ke_values(NN, ND);  // values per stored element matrix (padded)
   constexpr int EU = NN * (NN + 1) / 2 * B2;    // values in use
+  constexpr int SR = ke_split_row(NN, ND);
+  constexpr int PADS = SR < NN ? ke_block_offset(NN, ND, SR, SR) - 1 : -1;  // padding value between the halves
   extern __shared__ __attribute__((aligned(16))) double sh[];
   double* sQ = sh;                              // EPW * nip * QW: per-IP coefficients
   // node vectors of one IP chunk, ES doubles per element (odd, so that the reads of neighbouring elements
@@ -440,50 +440,58 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
         hA[u] = AX ? Pa[ND] : 0.0;
         if (AX) tIa[u][0] += hA[u];
       }
-      // the s(x)s term only where the IP is plastic (gamma = 0 at elastic IPs, so skipping it adds nothing):
-      // a branch that is uniform over the lanes of one element and IP; the ts vectors are read from LDS only
-      // in the plastic branch
-      auto blocks = [&](auto plc) {
-        constexpr bool PL = decltype(plc)::value;
+      // alpha and beta terms (and the axisymmetric hoop term): chained accumulations, one FMA each into the
+      // accumulator (a parenthesised sum would cost a multiply, two FMAs and an add)
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        const double* Pb = Pq + (2 * J + v) * PW;
+        double ub[ND], vb[ND];
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+          ub[i] = ha * Pb[i];
+          vb[i] = be * Pb[i];
+        }
+        double hh = 0.0;
+        if (AX) { const double hb = Pb[ND]; vb[0] = fma(be, hb, vb[0]); hh = 2.0 * ha * hb; }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+#pragma unroll
+          for (int i = 0; i < ND; i++) {
+            Dg[u][v] = fma(Ga[u][i], ub[i], Dg[u][v]);
+#pragma unroll
+            for (int j = 0; j < ND; j++) {
+              double k = K[u][v][i][j];
+              k = fma(ub[i], Ga[u][j], k);
+              k = fma(tIa[u][i], vb[j], k);
+              K[u][v][i][j] = k;
+            }
+          }
+          if (AX) K[u][v][0][0] = fma(hA[u], hh, K[u][v][0][0]);
+        }
+      }
+      // the s(x)s term only where the IP is plastic (gamma = 0 at elastic IPs, so skipping it adds nothing): a
+      // branch that only adds to the accumulators (uniform over the lanes of one element and IP); the ts
+      // vectors are read from LDS only here
+      if (ga != 0.0) {
         double ta[2][ND];
 #pragma unroll
         for (int u = 0; u < 2; u++)
 #pragma unroll
-          for (int i = 0; i < ND; i++) ta[u][i] = PL ? Pq[(2 * I + u) * PW + ND + 1 + i] : 0.0;
+          for (int i = 0; i < ND; i++) ta[u][i] = Pq[(2 * I + u) * PW + ND + 1 + i];
 #pragma unroll
         for (int v = 0; v < 2; v++) {
           const double* Pb = Pq + (2 * J + v) * PW;
-          double ub[ND], vb[ND], wb[ND];
+          double wb[ND];
 #pragma unroll
-          for (int i = 0; i < ND; i++) {
-            ub[i] = ha * Pb[i];
-            vb[i] = be * Pb[i];
-            wb[i] = PL ? ga * Pb[ND + 1 + i] : 0.0;
-          }
-          double hh = 0.0;
-          if (AX) { const double hb = Pb[ND]; vb[0] += be * hb; hh = 2.0 * ha * hb; }
+          for (int i = 0; i < ND; i++) wb[i] = ga * Pb[ND + 1 + i];
 #pragma unroll
-          for (int u = 0; u < 2; u++) {
+          for (int u = 0; u < 2; u++)
 #pragma unroll
-            for (int i = 0; i < ND; i++) {
-              Dg[u][v] += Ga[u][i] * ub[i];
-              // chained accumulations: one FMA each into the accumulator (a parenthesised sum would cost
-              // a multiply, two FMAs and an add)
+            for (int i = 0; i < ND; i++)
 #pragma unroll
-              for (int j = 0; j < ND; j++) {
-                double k = K[u][v][i][j];
-                k += ub[i] * Ga[u][j];
-                k += tIa[u][i] * vb[j];
-                if (PL) k += ta[u][i] * wb[j];
-                K[u][v][i][j] = k;
-              }
-            }
-            if (AX) K[u][v][0][0] += hA[u] * hh;
-          }
+              for (int j = 0; j < ND; j++) K[u][v][i][j] = fma(ta[u][i], wb[j], K[u][v][i][j]);
         }
-      };
-      if (ga != 0.0) blocks(std::true_type{});
-      else blocks(std::false_type{});
+      }
     }
   };
   // the chunk loop is specialised as a whole (the branch on the uniform axisym flag sits outside it, so
@@ -492,13 +500,15 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
   for (int q0 = 0; q0 < nip; q0 += ch) {
     const int nq = min(ch, nip - q0);
     __syncthreads();  // coefficients ready (first chunk); the previous chunk's node vectors consumed
-    // node vectors of every (IP of the chunk, element, node); the padding node of odd NN stays zero.
-    // One item at a time: the tile accumulators are live here
+    // node vectors of every (IP of the chunk, element, node), node m in the slot of its rank (the tiles
+    // and the stored blocks are in rank order); the padding slot of odd NN stays zero.  One item at a time:
+    // the tile accumulators are live here
 #pragma unroll 1
     for (int w = tid; w < nq * EPW * NNP; w += TW) {
       const int m = w % NNP, r1 = w / NNP, e = r1 % EPW, ql = r1 / EPW;
-      double* P = sP + e * ES + (ql * NNP + m) * PW;
-      if (e0 + e >= nel || m >= NN) {
+      const bool pad = e0 + e >= nel || m >= NN;
+      double* P = sP + e * ES + (ql * NNP + (pad ? m : srank[e * NN + m])) * PW;
+      if (pad) {
 #pragma unroll
         for (int k = 0; k < PW; k++) P[k] = 0.0;
         continue;
@@ -542,8 +552,8 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
 #pragma unroll
       for (int i = 0; i < ND; i++) K[u][v][i][i] += Dg[u][v];
   // the workgroup's element matrices are contiguous: staged in LDS over the IP data, cap values per pass,
-  // then copied out with 16-byte stores.  A block computed below the rank diagonal is stored transposed in
-  // its mirror position; a diagonal node block takes its lower half from the upper one, so every element
+  // then copied out with 16-byte stores.  The tiles are in rank order, so a tile's upper blocks are stored
+  // blocks as they are.  A diagonal node block takes its lower half from the upper one, so every element
   // matrix (and K) is exactly symmetric (the local check of assembly.rs:78-90 holds by construction)
   KT* stage = reinterpret_cast<KT*>(sh);
   typedef KT V __attribute__((ext_vector_type(16 / sizeof(KT))));
@@ -553,7 +563,6 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
   for (int p0 = 0; p0 < nval; p0 += cap) {
     __syncthreads();  // node vectors / the previous pass consumed
     if (active) {
-      const uint8_t* rk = srank + el * NN;
       const int base = el * EW - p0;
 #pragma unroll
       for (int u = 0; u < 2; u++)
@@ -561,20 +570,20 @@ __device__ __forceinline__ void tangent_body(int64_t blk, const TanArgs<KT>& A) 
         for (int v = 0; v < 2; v++) {
           const int a = 2 * I + u, b = 2 * J + v;
           if (a >= NN || b >= NN || a > b) continue;
-          const int ra = rk[a], rb = rk[b];
-          const bool up = ra <= rb;
-          const int o = base + packed_block(NN, up ? ra : rb, up ? rb : ra) * B2;
+          const int o = base + ke_block_offset(NN, ND, a, b);
 #pragma unroll
           for (int i = 0; i < ND; i++)
 #pragma unroll
             for (int j = 0; j < ND; j++) {
-              const int x = o + (up ? i * ND + j : j * ND + i);
+              const int x = o + i * ND + j;
               if ((unsigned)x < (unsigned)cap) stage[x] = (KT)(a == b && j < i ? K[u][v][j][i] : K[u][v][i][j]);
             }
         }
-      if (tile == 0)
-        for (int i = EU; i < EW; i++)
+      if (tile == 0) {  // padding values: the one between the halves (Hex20) and the tail
+        if (PADS >= 0 && (unsigned)(base + PADS) < (unsigned)cap) stage[base + PADS] = (KT)0;
+        for (int i = EU + (PADS >= 0 ? 1 : 0); i < EW; i++)
           if ((unsigned)(base + i) < (unsigned)cap) stage[base + i] = (KT)0;
+      }
     }
     __syncthreads();
     const int nv = min(cap, nval - p0) / VL;
@@ -774,8 +783,7 @@ static uint64_t group_key(const pm_fem* f, const Group& g) { return (uint64_t)(&
 constexpr size_t SHM_CAP = 60 * 1024;  // dynamic LDS budget per workgroup (static arrays use the rest of 64 KB)
 
 static size_t shm_update(int nd, int nn, int nip, int epb) {
-  int ns = nd == 2 ? 4 : 6;
-  return sizeof(double) * ((size_t)epb * nn * nd + (size_t)epb * nip * (ns + nd * nd + 1));
+  return sizeof(double) * ((size_t)nip * (nn * nd + 1) + (size_t)epb * nn * nd + (size_t)epb * nip * (nd * nd + 2));
 }
 static int epb_update(int nd, int nn, int nip) {
   int epb = std::max(1, TPB / nip);  // the update kernel takes one IP per thread: epb * nip <= TPB (nip <= 27 here)
@@ -860,12 +868,18 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
   PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
 }
 
-// tangent kernel of a group: the tuned choice, else the default (matrix cores for BF16, the vector kernel otherwise)
+// tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16, the vector
+// kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
+// cores, 4: vector kernel with one workgroup per element group (never the persistent form)
 static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   const int prec = f->cfg.tangent_precision;
   if (prec == 2) return 2;  // BF16 runs on the matrix cores only
   if (g.tune_kernel) return g.tune_kernel;
-  return 1;  // FP64 and FP32: the node-block vector kernel (measured faster than the FP32 matrix-core kernel)
+  return 1;  // FP64 and FP32: the vector kernel (measured faster than the FP32 matrix-core kernel)
+}
+// true when a group's tangent runs as the persistent Hex20 kernel
+static bool runs_hex20(const pm_fem* f, const Group& g) {
+  return tangent_kernel_of(f, g) == 1 && g.tune_ch == 0 && tangent_hex20_fits(f, g);
 }
 
 // IPs per chunk of a group's vector tangent: the tuned value, else the heuristic
@@ -887,6 +901,7 @@ void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
   if (g.general_plasticity) { refuse(f, kGpStiffness); return; }
   if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
+  if (runs_hex20(f, g) && launch_tangent_hex20(f, g)) return;
   const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
   const int64_t grid = (g.nel + epw - 1) / epw;
   if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) {
@@ -904,13 +919,13 @@ void launch_tangent(pm_fem* f, Group& g) {
   }
 }
 
-// Tangent kernel choice per group.  Modes 1 and 2 force a kernel; mode 3 times every candidate (the
-// matrix-core kernel and the vector kernel under each feasible work split) on the current state and keeps
-// the fastest; mode 0 keeps the default.
+// Tangent kernel choice per group.  Modes 1, 2 and 4 force a kernel; mode 3 times every candidate (the
+// matrix-core kernel, the persistent Hex20 vector kernel where it fits and the vector kernel under each
+// feasible work split) on the current state and keeps the fastest; mode 0 keeps the default.
 int tune_tangent(pm_fem* f) {
   const int mode = f->cfg.tangent_kernel;
   release_graphs(f);
-  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2) ? mode : 0; g.tune_ch = 0; }
+  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2 || mode == 4) ? mode : 0; g.tune_ch = 0; }
   if (mode != 3) return 0;
   struct Cand { int kern, ch; };
   hipEvent_t ev0, ev1;
@@ -921,11 +936,12 @@ int tune_tangent(pm_fem* f) {
     if (g.nel == 0 || g.general_plasticity) continue;  // no tangent to time
     std::vector<Cand> cands;
     cands.push_back({2, 0});
+    if (f->cfg.tangent_precision != 2 && tangent_hex20_fits(f, g)) cands.push_back({1, 0});
     if (f->cfg.tangent_precision != 2)
       for (int ch : {1, 2, 3, 4, 6, 9, 14}) {
         if (ch > g.nip) continue;
         if (shm_tangent(g.ndim, g.nnode, g.nip, ch, f->cfg.tangent_precision == 0 ? 8 : 4) > SHM_CAP) continue;
-        cands.push_back({1, ch});
+        cands.push_back({4, ch});
       }
     float best = 1e30f;
     Cand pick = cands[0];
@@ -939,7 +955,7 @@ int tune_tangent(pm_fem* f) {
       (void)hipEventElapsedTime(&ms, ev0, ev1);
       if (f->cfg.verbose & 1)
         std::fprintf(stderr, "tangent tuning: kind %d, %s kernel, %d IPs per chunk: %.3f ms\n", g.kind,
-                     c.kern == 2 ? "matrix-core" : "vector", c.ch, ms / 3.0f);
+                     c.kern == 2 ? "matrix-core" : (c.kern == 1 ? "persistent vector" : "vector"), c.ch, ms / 3.0f);
       if (ms < best) { best = ms; pick = c; }
     }
     g.tune_kernel = pick.kern; g.tune_ch = pick.ch;
@@ -1026,6 +1042,7 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
   std::map<int, Cls<ScatArgs<double>>> sc64;
   std::map<int, Cls<ScatArgs<float>>> sc32;
   std::vector<std::pair<pm_fem*, Group*>> mfma;           // groups whose tangent runs on the matrix cores
+  std::vector<std::pair<pm_fem*, Group*>> h20;            // groups whose tangent runs as the persistent Hex20 kernel
   for (PassJob& J : jobs) {
     pm_fem* f = J.f;
     for (auto& g : f->groups) {
@@ -1040,6 +1057,7 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
       }
       if (J.tangent) {
         if (tangent_kernel_of(f, g) == 2) { mfma.push_back({f, &g}); continue; }
+        if (runs_hex20(f, g)) { h20.push_back({f, &g}); continue; }
         if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) { f->launch_refused = true; continue; }
         const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
         const int kb = f->cfg.tangent_precision == 0 ? 8 : 4;
@@ -1110,6 +1128,12 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
     mg.first->stream = stream;
     (void)launch_tangent_mfma(mg.first, *mg.second, mg.first->cfg.tangent_precision);
     mg.first->stream = own;
+  }
+  for (auto& hg : h20) {  // likewise
+    hipStream_t own = hg.first->stream;
+    hg.first->stream = stream;
+    (void)launch_tangent_hex20(hg.first, *hg.second);
+    hg.first->stream = own;
   }
   for (auto& kv : sc64) {
     auto& c = kv.second;

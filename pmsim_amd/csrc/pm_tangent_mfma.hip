@@ -495,8 +495,8 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (mfma_minw<ND, PREC>())) vo
       }
     }
     __syncthreads();  // M no longer needed: its space takes the node blocks
-    // epilogue 3: the upper node blocks (a <= b; rows and columns are already in rank order), packed
-    // (packed_block); the lower half of a diagonal block comes from its mirror entry
+    // epilogue 3: the upper node blocks (a <= b; rows and columns are already in rank order) at their
+    // stored offsets (ke_block_offset); the lower half of a diagonal block comes from its mirror entry
     T* sK = sL;
     if (active) {
 #pragma unroll
@@ -508,12 +508,16 @@ __global__ __launch_bounds__((MShape<ND, NN>::NTHR), (mfma_minw<ND, PREC>())) vo
           if (R >= NC || Cc >= NC || R > Cc) continue;
           const int i = R % ND, j = Cc % ND, a = R / ND, b = Cc / ND;
           const T v = accC[s][r];
-          T* blk = sK + packed_block(NN, a, b) * B2;
+          T* blk = sK + ke_block_offset(NN, ND, a, b);
           blk[i * ND + j] = v;
           if (a == b) blk[j * ND + i] = v;
         }
       }
-      for (int i = EU + te; i < EW; i += TPE) sK[i] = (T)0;
+      // padding values: the one between the halves (Hex20) and the tail
+      constexpr int SR = ke_split_row(NN, ND);
+      constexpr int PADS = SR < NN ? ke_block_offset(NN, ND, SR, SR) - 1 : -1;
+      for (int i = EU + (PADS >= 0 ? 1 : 0) + te; i < EW; i += TPE) sK[i] = (T)0;
+      if (PADS >= 0 && te == 0) sK[PADS] = (T)0;
     }
     __syncthreads();
     // epilogue 4: contiguous write of the element matrix in 16-byte pieces.  The trip count is a

@@ -267,7 +267,8 @@ class Config:
         # MI355X path selection (not in pmsim): linear solver and tangent precision
         self.lin_sol = "pcg"          # "pcg" (device FP64 block-Jacobi PCG) or "host" (host direct)
         self.tangent_precision = "f64"  # "f64", "f32" or "bf16" (inexact-Newton tangent)
-        self.tangent_kernel = "default"  # "default", "vector", "mfma" (matrix cores) or "tune" (fastest measured)
+        self.tangent_kernel = "default"  # "default", "vector", "mfma" (matrix cores), "tune" (fastest measured) or
+        # "vector-wg" (the vector kernel with one workgroup per group of elements, never the persistent Hex20 form)
         self.use_graphs = True  # replay captured HIP graphs for element passes and PCG chunks
         self.model_settings: Dict[int, Settings] = {}
         self.pcg_small_max = -1  # DOF limit of the single-workgroup PCG (-1: library default)
