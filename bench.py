@@ -388,7 +388,7 @@ def main():
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
-    ap.add_argument("--kernel", default="default", choices=["default", "vector", "vector-wg", "mfma", "tune"])
+    ap.add_argument("--kernel", default="default", choices=["default", "vector", "vector-wg", "mfma", "mfma-wg", "tune"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="hex20: leave out the served comparison")
     ap.add_argument("--serve-requests", type=int, default=24, help="hex20: requests of the served comparison")

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmsim_amd.so")
-SOURCES = ["pm_shapes.cpp", "pm_host.cpp", "pm_device.cpp", "pm_solver.cpp", "pm_capi.cpp", "pm_serve.cpp", "pm_kernels.hip", "pm_tangent_mfma.hip", "pm_tangent_hex20.hip", "pm_linalg.hip", "pm_pcg_batch.hip", "pm_noalloc.hip"]
+SOURCES = ["pm_shapes.cpp", "pm_host.cpp", "pm_device.cpp", "pm_solver.cpp", "pm_capi.cpp", "pm_serve.cpp", "pm_kernels.hip", "pm_tangent_mfma.hip", "pm_tangent_hex20.hip", "pm_tangent_hex20_mfma.hip", "pm_linalg.hip", "pm_pcg_batch.hip", "pm_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

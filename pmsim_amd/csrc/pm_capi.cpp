@@ -122,7 +122,7 @@ int pm_set_config(pm_fem* f, const pm_config* c) {
   if (c->dt <= 0.0) { f->err = "Δt is smaller than the allowed minimum"; return 1; }
   if (c->tangent_precision < 0 || c->tangent_precision > 2) { f->err = "unknown tangent precision"; return 1; }
   if (f->initialized && c->tangent_precision != f->cfg.tangent_precision) { f->err = "tangent precision must be set before pm_initialize"; return 1; }
-  if (c->tangent_kernel < 0 || c->tangent_kernel > 4) { f->err = "unknown tangent kernel"; return 1; }
+  if (c->tangent_kernel < 0 || c->tangent_kernel > 5) { f->err = "unknown tangent kernel"; return 1; }
   const bool retune = f->initialized && c->tangent_kernel != f->cfg.tangent_kernel;
   if (f->initialized) release_graphs(f);  // captured launches depend on the configuration
   f->cfg = *c;

@@ -102,7 +102,7 @@ int device_alloc(pm_fem* f) {
     return 1;
   const int64_t gn = (f->npoint + 255) / 256;
   if (dalloc(f, &f->d_red, std::max<int64_t>(4096 + 2 * std::min<int64_t>(gn, 2048), 1024) + 64) || dalloc(f, &f->d_scal, 32) || dalloc(f, &f->d_err, 4) ||
-      dalloc(f, &f->d_work, 64)) return 1;
+      dalloc(f, &f->d_work, (int64_t)kWorkCounters * kWorkStride)) return 1;
   std::vector<double> freev(n, 1.0);
   for (auto eq : f->prescribed_eq) freev[eq] = 0.0;
   if (upload(f, f->d_free, freev.data(), n)) return 1;

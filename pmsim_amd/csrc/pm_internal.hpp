@@ -69,6 +69,10 @@ struct PassEngine;
 // 946 values, each a whole number of 16-byte pieces in FP64.  Each element's storage is padded to a multiple
 // of 4 values so that every element starts on a 16-byte boundary in FP32 and FP64.
 constexpr int GATHER_CHUNK = 256;  // CSR blocks per gather workgroup (its block size)
+// Persistent tangents hand out elements from kWorkCounters counters, kWorkStride int32 apart (one 256-byte
+// line each): counter c owns elements c, c + n, c + 2n, ... (n counters in use), so that the device-scope
+// atomics of the wavefronts are spread over as many addresses.
+constexpr int kWorkCounters = 64, kWorkStride = 64;
 constexpr int packed_block(int nn, int a, int b) { return a * (2 * nn - a + 1) / 2 + (b - a); }
 constexpr int packed_ke_values(int nn, int nd) { return ((nn * (nn + 1) / 2 * nd * nd + 3) / 4) * 4; }
 constexpr int ke_split_row(int nn, int nd) { return nn == 20 && nd == 3 ? 6 : nn; }
@@ -221,7 +225,7 @@ struct pm_fem {
   double* d_red = nullptr;             // reduction partials
   double* d_scal = nullptr;            // device scalars
   int32_t* d_err = nullptr;            // device error flags and first failing IP (DevErr, raise_error)
-  int32_t* d_work = nullptr;           // work counter of the persistent Hex20 tangent (reset before each launch)
+  int32_t* d_work = nullptr;           // work counters of the persistent Hex20 tangents (reset before each launch)
   int64_t err_cell = -1;               // cell and Gauss point of the last device-side error (-1: none)
   int32_t err_gauss = -1;
   int64_t pcg_small_max = kPcgSmallMaxDefault;  // systems up to this many DOFs use the single-workgroup PCG
@@ -263,6 +267,8 @@ void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 bool launch_tangent_hex20(pm_fem* f, Group& g);
 bool tangent_hex20_fits(const pm_fem* f, const Group& g);
+bool launch_tangent_hex20_bf16(pm_fem* f, Group& g);
+bool tangent_hex20_bf16_fits(const pm_fem* f, const Group& g);
 // choose the tangent kernel (and work split) of every group, timing the candidates when asked to
 int tune_tangent(pm_fem* f);
 // drops every captured graph (configuration, tuning or buffers changed)

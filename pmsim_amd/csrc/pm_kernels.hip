@@ -870,16 +870,23 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
 
 // tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16, the vector
 // kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
-// cores, 4: vector kernel with one workgroup per element group (never the persistent form)
+// cores (for BF16 the persistent Hex20 form where it fits, tangent_hex20_bf16_fits), 4: vector kernel with one
+// workgroup per element group, 5: the per-group matrix-core kernel (never the persistent forms)
 static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   const int prec = f->cfg.tangent_precision;
-  if (prec == 2) return 2;  // BF16 runs on the matrix cores only
+  if (prec == 2) return g.tune_kernel == 5 ? 5 : 2;  // BF16 runs on the matrix cores only
   if (g.tune_kernel) return g.tune_kernel;
   return 1;  // FP64 and FP32: the vector kernel (measured faster than the FP32 matrix-core kernel)
 }
 // true when a group's tangent runs as the persistent Hex20 kernel
 static bool runs_hex20(const pm_fem* f, const Group& g) {
   return tangent_kernel_of(f, g) == 1 && g.tune_ch == 0 && tangent_hex20_fits(f, g);
+}
+// a group's matrix-core tangent: the persistent Hex20 BF16 kernel where it fits and is not excluded, else the
+// per-group kernel; false when neither has a variant for the group
+static bool launch_matrix_core(pm_fem* f, Group& g) {
+  if (tangent_kernel_of(f, g) == 2 && tangent_hex20_bf16_fits(f, g) && launch_tangent_hex20_bf16(f, g)) return true;
+  return launch_tangent_mfma(f, g, f->cfg.tangent_precision);
 }
 
 // IPs per chunk of a group's vector tangent: the tuned value, else the heuristic
@@ -900,7 +907,8 @@ static TanArgs<KT> tan_args(pm_fem* f, Group& g, int ch, KT* ke, size_t shm) {
 void launch_tangent(pm_fem* f, Group& g) {
   if (g.nel == 0) return;
   if (g.general_plasticity) { refuse(f, kGpStiffness); return; }
-  if (tangent_kernel_of(f, g) == 2 && launch_tangent_mfma(f, g, f->cfg.tangent_precision)) return;
+  const int kern = tangent_kernel_of(f, g);
+  if ((kern == 2 || kern == 5) && launch_matrix_core(f, g)) return;
   if (runs_hex20(f, g) && launch_tangent_hex20(f, g)) return;
   const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
   const int64_t grid = (g.nel + epw - 1) / epw;
@@ -925,7 +933,7 @@ void launch_tangent(pm_fem* f, Group& g) {
 int tune_tangent(pm_fem* f) {
   const int mode = f->cfg.tangent_kernel;
   release_graphs(f);
-  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2 || mode == 4) ? mode : 0; g.tune_ch = 0; }
+  for (auto& g : f->groups) { g.tune_kernel = (mode == 1 || mode == 2 || mode == 4 || mode == 5) ? mode : 0; g.tune_ch = 0; }
   if (mode != 3) return 0;
   struct Cand { int kern, ch; };
   hipEvent_t ev0, ev1;
@@ -936,6 +944,7 @@ int tune_tangent(pm_fem* f) {
     if (g.nel == 0 || g.general_plasticity) continue;  // no tangent to time
     std::vector<Cand> cands;
     cands.push_back({2, 0});
+    if (f->cfg.tangent_precision == 2 && tangent_hex20_bf16_fits(f, g)) cands.push_back({5, 0});
     if (f->cfg.tangent_precision != 2 && tangent_hex20_fits(f, g)) cands.push_back({1, 0});
     if (f->cfg.tangent_precision != 2)
       for (int ch : {1, 2, 3, 4, 6, 9, 14}) {
@@ -955,7 +964,8 @@ int tune_tangent(pm_fem* f) {
       (void)hipEventElapsedTime(&ms, ev0, ev1);
       if (f->cfg.verbose & 1)
         std::fprintf(stderr, "tangent tuning: kind %d, %s kernel, %d IPs per chunk: %.3f ms\n", g.kind,
-                     c.kern == 2 ? "matrix-core" : (c.kern == 1 ? "persistent vector" : "vector"), c.ch, ms / 3.0f);
+                     c.kern == 2 ? "matrix-core" : c.kern == 5 ? "per-group matrix-core" : (c.kern == 1 ? "persistent vector" : "vector"),
+                     c.ch, ms / 3.0f);
       if (ms < best) { best = ms; pick = c; }
     }
     g.tune_kernel = pick.kern; g.tune_ch = pick.ch;
@@ -1056,7 +1066,7 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
         if (!c.add(upd_args(f, g, J.do_update, epb), (g.nel + epb - 1) / epb)) return 1;
       }
       if (J.tangent) {
-        if (tangent_kernel_of(f, g) == 2) { mfma.push_back({f, &g}); continue; }
+        if (tangent_kernel_of(f, g) == 2 || tangent_kernel_of(f, g) == 5) { mfma.push_back({f, &g}); continue; }
         if (runs_hex20(f, g)) { h20.push_back({f, &g}); continue; }
         if (!ke_range_ok(g.ke_base, g.nel, g.kew, packed_ke_values(g.nnode, g.ndim), f->n_ke_values)) { f->launch_refused = true; continue; }
         const int ch = tangent_split(f, g), epw = tangent_epw(g.nnode);
@@ -1126,7 +1136,7 @@ int pass_batch_run(PassBatchBuf* buf, hipStream_t stream, std::vector<PassJob>& 
   for (auto& mg : mfma) {  // one launch per group, on the batch's stream (the handle's owner waits for the batch)
     hipStream_t own = mg.first->stream;
     mg.first->stream = stream;
-    (void)launch_tangent_mfma(mg.first, *mg.second, mg.first->cfg.tangent_precision);
+    (void)launch_matrix_core(mg.first, *mg.second);
     mg.first->stream = own;
   }
   for (auto& hg : h20) {  // likewise

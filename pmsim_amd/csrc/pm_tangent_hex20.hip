@@ -7,9 +7,9 @@
 // with the same terms summed in the same order per entry (its element matrices agree with that kernel's to
 // rounding: the two compile to different contractions in places).  The organisation is what differs:
 //  * one workgroup per CU of up to twelve wavefronts; every wavefront works through elements one at a time,
-//    its first one fixed and each next one taken from a work counter one element ahead (so wavefronts
-//    that get more issue cycles take more elements and all finish together), and never waits for another
-//    wavefront;
+//    each taken from a work counter one element ahead (so wavefronts that get more issue cycles take more
+//    elements and all finish together; the counters are spread over many addresses), and never waits for
+//    another wavefront;
 //  * the shape-function derivatives of the 27 IPs sit in LDS once per workgroup;
 //  * the IP geometry (J^-1, det J) and state of the next element are copied from HBM into the
 //    wavefront's LDS by direct global->LDS loads while the last IPs of the current element are
@@ -63,7 +63,8 @@ struct H20Args {
   int64_t nel;
   const uint8_t* rank; const double* geo; const double* tab; const double* st; KT* ke;
   H20Mat mp; double thickness; int32_t* err; uint64_t ekey;
-  int32_t* work;            // elements handed out beyond the first of each wavefront (zero at launch)
+  int32_t* work;            // work counters (zero at launch)
+  int ncnt;                 // counters in use (at most the number of workgroups)
 };
 
 __device__ __forceinline__ void lds_wait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
@@ -93,12 +94,10 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
     for (int i = tid; i < H_NIP; i += blockDim.x) sw[i] = tab[i];
   }
   __syncthreads();  // the only workgroup-wide wait
-  // elements: the wavefront's global index first, then nwt + (a work-counter ticket); every wavefront ends
-  // once its element index passes nel
-  const int64_t nwt = (int64_t)gridDim.x * nw;
-  int64_t cur = (int64_t)blockIdx.x * nw + wv;
-  if (cur >= nel) return;  // uniform over the wavefront
-  int32_t* work = A.work;
+  // elements: this workgroup's counter c hands out c, c + ncnt, c + 2 ncnt, ... (a ticket each); every
+  // wavefront ends once its element index passes nel
+  const int ncnt = A.ncnt, cidx = (int)(blockIdx.x % (unsigned)ncnt);
+  int32_t* work = A.work + cidx * kWorkStride;
   // a ticket: one lane's atomic add, read back (readfirstlane) only at the end of the element, when it has
   // long arrived.  The address passes through an empty asm so that the compiler does not turn the add into
   // a wavefront-wide one whose result it would wait for at once.
@@ -153,12 +152,16 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
     __builtin_amdgcn_global_load_lds(src + min(lane, H_NN / 4 - 1), (lds_void*)rrk, 4, 0, 0);
     order();
   };
+  int32_t tk = ticket();
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  int64_t cur = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
+  if (cur >= nel) return;  // uniform over the wavefront
   prefetch_state(cur);
   for (int k = 0; k < H_GEO_DW / 64; k++) prefetch_geo(cur, k);
   prefetch_rank(cur);
-  int32_t tk = ticket();
+  tk = ticket();
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  int64_t nxt = nwt + __builtin_amdgcn_readfirstlane(tk);
+  int64_t nxt = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
   static_assert(64 * (H_NCH - 1) + 64 <= 2 * H_GW * H_CH * H_NCH && H_GEO_DW / 64 == H_NCH + 1,
                 "geometry load k = chunk c overwrites only IPs staged by then; the tenth follows the last chunk");
 
@@ -377,12 +380,12 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
     else asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
     cur = nxt;
     asm volatile("" : "+v"(tk));  // read the ticket here, after the wait above
-    nxt = nwt + __builtin_amdgcn_readfirstlane(tk);
+    nxt = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
   }
 }
 
 __global__ void k_zero_work(int32_t* w) {
-  if (threadIdx.x == 0) w[0] = 0;
+  if (threadIdx.x < kWorkCounters) w[threadIdx.x * kWorkStride] = 0;
 }
 
 }  // namespace
@@ -430,10 +433,12 @@ bool launch_tangent_hex20(pm_fem* f, Group& g) {
   }
   hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, f->stream, f->d_work);  // a kernel, not a memset (graph replays)
   if (f->cfg.tangent_precision == 0) {
-    H20Args<double> A{g.nel, g.d_rank, g.d_geo, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp, f->thickness, f->d_err, ekey, f->d_work};
+    H20Args<double> A{g.nel, g.d_rank, g.d_geo, g.d_tab, g.d_state, f->d_ke + g.ke_base, mp, f->thickness, f->d_err, ekey, f->d_work,
+                      (int)std::min<int64_t>(kWorkCounters, grid)};
     hipLaunchKernelGGL((k_tangent_hex20<double>), dim3((unsigned)grid), dim3(64 * nw), shm, f->stream, A);
   } else {
-    H20Args<float> A{g.nel, g.d_rank, g.d_geo, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp, f->thickness, f->d_err, ekey, f->d_work};
+    H20Args<float> A{g.nel, g.d_rank, g.d_geo, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp, f->thickness, f->d_err, ekey, f->d_work,
+                     (int)std::min<int64_t>(kWorkCounters, grid)};
     hipLaunchKernelGGL((k_tangent_hex20<float>), dim3((unsigned)grid), dim3(64 * nw), shm, f->stream, A);
   }
   return true;

@@ -268,7 +268,8 @@ class Config:
         self.lin_sol = "pcg"          # "pcg" (device FP64 block-Jacobi PCG) or "host" (host direct)
         self.tangent_precision = "f64"  # "f64", "f32" or "bf16" (inexact-Newton tangent)
         self.tangent_kernel = "default"  # "default", "vector", "mfma" (matrix cores), "tune" (fastest measured) or
-        # "vector-wg" (the vector kernel with one workgroup per group of elements, never the persistent Hex20 form)
+        # "vector-wg" / "mfma-wg" (the vector / matrix-core kernel with workgroups per group of elements, never the
+        # persistent Hex20 forms)
         self.use_graphs = True  # replay captured HIP graphs for element passes and PCG chunks
         self.model_settings: Dict[int, Settings] = {}
         self.pcg_small_max = -1  # DOF limit of the single-workgroup PCG (-1: library default)

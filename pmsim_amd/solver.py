@@ -140,7 +140,7 @@ class PmError(RuntimeError):
 
 
 PRECISION = {"f64": 0, "f32": 1, "bf16": 2}
-TANGENT_KERNEL = {"default": 0, "vector": 1, "mfma": 2, "tune": 3, "vector-wg": 4}
+TANGENT_KERNEL = {"default": 0, "vector": 1, "mfma": 2, "tune": 3, "vector-wg": 4, "mfma-wg": 5}
 
 
 def make_config(d: dict, cfg: Optional[Config] = None, keep_history=True) -> PmConfig:
