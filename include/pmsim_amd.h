@@ -47,12 +47,13 @@ typedef struct pm_config {
   int32_t pcg_max_iter;             /* default 20000 */
   int32_t keep_history;             /* store u and IP state of every output step on the host */
   double tol_rr_scaled;             /* > 0 adds ||R||inf <= tol_rr_scaled * ||F_int||inf as a convergence test */
-  int32_t tangent_kernel;           /* 0: default (vector kernel for FP64 / FP32, matrix cores for BF16), 1: node-block vector kernel (FP64 arithmetic;
-                                       persistent form for Hex20 with 27 IPs, same terms, agrees to rounding), 2: matrix-core (MFMA) kernel in the tangent precision,
+  int32_t tangent_kernel;           /* 0: default (vector kernel for FP64; matrix cores for BF16, and for FP32 where the persistent Hex20
+                                       matrix-core kernel fits, else the vector kernel), 1: node-block vector kernel (FP64 arithmetic;
+                                       persistent form for Hex20 with 27 IPs, same terms, agrees to rounding), 2: matrix-core (MFMA) kernel in the tangent precision
+                                       (FP32 / BF16: persistent form for Hex20 with 27 IPs and Poisson ratio >= 0),
                                        3: tune at pm_initialize (fastest measured; FP64 variants agree to rounding, not bitwise),
                                        4: node-block vector kernel with one workgroup per group of elements (never the persistent form),
-                                       5: matrix-core kernel per group of elements (never the persistent Hex20 BF16 form, which 2 and the
-                                       BF16 default use for Hex20 with 27 IPs) */
+                                       5: matrix-core kernel per group of elements (never the persistent Hex20 form) */
   int32_t use_graphs;               /* 1 (default): element passes and 16-iteration PCG chunks replay captured HIP graphs */
   int32_t pcg_small_max;            /* systems with at most this many DOFs run the whole PCG in one workgroup; < 0: default */
   int32_t verbose;                  /* to stderr; bit 0: a line per converged step, bit 1: a line per Newton iteration

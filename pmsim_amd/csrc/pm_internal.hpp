@@ -267,8 +267,8 @@ void cell_ranks(const pm_fem* f, int64_t e, int* rk);
 bool launch_tangent_mfma(pm_fem* f, Group& g, int prec);
 bool launch_tangent_hex20(pm_fem* f, Group& g);
 bool tangent_hex20_fits(const pm_fem* f, const Group& g);
-bool launch_tangent_hex20_bf16(pm_fem* f, Group& g);
-bool tangent_hex20_bf16_fits(const pm_fem* f, const Group& g);
+bool launch_tangent_hex20_mfma(pm_fem* f, Group& g);
+bool tangent_hex20_mfma_fits(const pm_fem* f, const Group& g);
 // choose the tangent kernel (and work split) of every group, timing the candidates when asked to
 int tune_tangent(pm_fem* f);
 // drops every captured graph (configuration, tuning or buffers changed)

@@ -868,24 +868,27 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
   PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
 }
 
-// tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16, the vector
-// kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
-// cores (for BF16 the persistent Hex20 form where it fits, tangent_hex20_bf16_fits), 4: vector kernel with one
-// workgroup per element group, 5: the per-group matrix-core kernel (never the persistent forms)
+// tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16 and for FP32
+// groups that the persistent Hex20 matrix-core kernel fits, the vector kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
+// cores (for FP32 and BF16 the persistent Hex20 form where it fits, tangent_hex20_mfma_fits), 4: vector kernel
+// with one workgroup per element group, 5: the per-group matrix-core kernel (never the persistent forms)
 static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   const int prec = f->cfg.tangent_precision;
   if (prec == 2) return g.tune_kernel == 5 ? 5 : 2;  // BF16 runs on the matrix cores only
   if (g.tune_kernel) return g.tune_kernel;
-  return 1;  // FP64 and FP32: the vector kernel (measured faster than the FP32 matrix-core kernel)
+  // FP32: the persistent Hex20 matrix-core kernel where it fits (measured faster than the vector kernel),
+  // else the vector kernel (faster than the per-group FP32 matrix-core kernel); FP64: the vector kernel
+  if (prec == 1 && tangent_hex20_mfma_fits(f, g)) return 2;
+  return 1;
 }
 // true when a group's tangent runs as the persistent Hex20 kernel
 static bool runs_hex20(const pm_fem* f, const Group& g) {
   return tangent_kernel_of(f, g) == 1 && g.tune_ch == 0 && tangent_hex20_fits(f, g);
 }
-// a group's matrix-core tangent: the persistent Hex20 BF16 kernel where it fits and is not excluded, else the
+// a group's matrix-core tangent: the persistent Hex20 kernel where it fits and is not excluded, else the
 // per-group kernel; false when neither has a variant for the group
 static bool launch_matrix_core(pm_fem* f, Group& g) {
-  if (tangent_kernel_of(f, g) == 2 && tangent_hex20_bf16_fits(f, g) && launch_tangent_hex20_bf16(f, g)) return true;
+  if (tangent_kernel_of(f, g) == 2 && tangent_hex20_mfma_fits(f, g) && launch_tangent_hex20_mfma(f, g)) return true;
   return launch_tangent_mfma(f, g, f->cfg.tangent_precision);
 }
 
@@ -944,7 +947,7 @@ int tune_tangent(pm_fem* f) {
     if (g.nel == 0 || g.general_plasticity) continue;  // no tangent to time
     std::vector<Cand> cands;
     cands.push_back({2, 0});
-    if (f->cfg.tangent_precision == 2 && tangent_hex20_bf16_fits(f, g)) cands.push_back({5, 0});
+    if (f->cfg.tangent_precision != 0 && tangent_hex20_mfma_fits(f, g)) cands.push_back({5, 0});
     if (f->cfg.tangent_precision != 2 && tangent_hex20_fits(f, g)) cands.push_back({1, 0});
     if (f->cfg.tangent_precision != 2)
       for (int ch : {1, 2, 3, 4, 6, 9, 14}) {

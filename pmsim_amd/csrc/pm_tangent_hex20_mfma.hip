@@ -1,5 +1,6 @@
-// Persistent matrix-core tangent for Hex20 elements with 27 integration points: BF16 operands, FP32
-// accumulation (the BF16 inexact-Newton tangent).
+// Persistent matrix-core tangent for Hex20 elements with 27 integration points: FP32 operands (16x16x4
+// MFMA) or BF16 operands (16x16x16), FP32 accumulation, element matrices stored in FP32 (the FP32 and BF16
+// inexact-Newton tangents).
 //
 // For D = alpha I + beta I(x)I + gamma s(x)s (von_mises.rs:124-167) the element matrix is
 //   K_ab = M_ab^T + tr(M_ab) 1 + C_ab,
@@ -9,8 +10,8 @@
 // (alpha > 0, beta >= 0 for nu >= 0), and the gamma term, whose weight is never positive, is subtracted
 // by negating its B fragments in registers.  Node columns are padded to four (80 columns: 5x5 tiles of
 // 16x16, 15 upper tiles), so that no 3x3 node block crosses a tile: the block transposes of M and its
-// traces stay inside a tile, between the four lanes of a quad (DPP).  The IPs are taken in two chunks of 16
-// (the MFMA depth), the rows of a chunk staged in LDS as BF16.
+// traces stay inside a tile, between the four lanes of a quad (DPP).  The IPs are taken in chunks (16 for
+// BF16, the MFMA depth; 8 for FP32, two MFMA depths), the rows of a chunk staged in LDS in the operand type.
 //
 // Organisation as the persistent vector kernel (pm_tangent_hex20.hip): one workgroup per CU of up to eight
 // wavefronts, one element per wavefront at a time from work counters, the shape-function derivatives in
@@ -32,23 +33,15 @@ constexpr double ISQ2_M = 0.70710678118654752440;
 constexpr int M_NIP = 27, M_NN = 20, M_ND = 3, M_NS = 6;
 constexpr int M_NC = 80;                 // columns: 20 nodes x 4 (three dofs and a zero)
 constexpr int M_T = 5;                   // 16-column tiles per dimension
-constexpr int M_KC = 16;                 // IPs per chunk (MFMA depth)
-constexpr int M_NCHK = 2;                // chunks (IPs 27..31 are zero rows)
-constexpr int M_RS = 20;                 // BF16 values per column row in LDS (16 used; 40 B: conflict-free reads)
-constexpr int M_KIND = M_NC * M_RS;      // BF16 values per row kind
-constexpr int M_QW = 9;                  // per IP: sqrt(c alpha/2), sqrt(c beta), sqrt(-c gamma), deviator (6)
 constexpr int M_GW = 11;                 // per IP in the geometry array: J^-1 (9), det J, radius
 constexpr int M_GEO_DW = 640, M_ST_DW = 448, M_RK_DW = 64;  // LDS copies of one element (dwords)
 constexpr int M_RAW = (M_GEO_DW + M_ST_DW + M_RK_DW) / 2;    // doubles
 constexpr int M_EW = packed_ke_values(M_NN, M_ND);           // 1892 values per element matrix
-constexpr int M_ROWB = 3 * M_KIND * 2;                       // bytes of the three row kinds of a chunk
 constexpr int M_STG = M_EW * 4;                              // output staging (FP32)
-constexpr int M_UB = ((M_ROWB > M_STG ? M_ROWB : M_STG) + 15) / 16 * 16;  // union bytes
-constexpr int M_WAVE = M_RAW + M_NIP * M_QW + 1 + M_UB / 8;  // doubles per wavefront (even)
-constexpr int M_TAB = M_NIP * M_NN * M_ND + M_NIP + 1;       // dN, w, padding (doubles)
+constexpr int M_PW = 20;                 // floats per IP in the row-factor table: A (9), T (9), b/a ratio, zero
+constexpr int M_QP = 32;                 // IPs in the row-factor table (IPs 27..31: zero rows for the padded chunk)
+constexpr int M_TAB = M_NIP * M_NN * 2 + M_NIP + 1;          // dN (float4 per IP and node), w, padding (doubles)
 constexpr int M_WMAX = 8;
-static_assert(M_WAVE % 2 == 0 && M_TAB % 2 == 0, "wavefront regions start 16-byte aligned");
-static_assert(M_NIP <= M_KC * M_NCHK, "two chunks cover the IPs");
 
 typedef __attribute__((address_space(3))) void lds_void_m;
 typedef __attribute__((address_space(1))) int32_t gint_m;
@@ -57,6 +50,32 @@ typedef short s4_m __attribute__((ext_vector_type(4)));
 typedef float V4 __attribute__((ext_vector_type(4)));
 typedef float f2_m __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2_m __attribute__((ext_vector_type(2)));  // round to nearest even (v_cvt_pk_bf16_f32)
+
+// operand precision P: 1 FP32 (v_mfma_f32_16x16x4_f32), 2 BF16 (v_mfma_f32_16x16x16_bf16)
+template <int P> struct MOp;
+template <> struct MOp<1> {
+  typedef float E;                     // row element in LDS
+  typedef float Frag;                  // one lane's MFMA operand: one k
+  static constexpr int KS = 4;         // IPs per MFMA
+  static constexpr int KC = 8;         // IPs per chunk
+  static constexpr int RS = 9;         // elements per column row (8 used; odd: conflict-free single-k reads)
+};
+template <> struct MOp<2> {
+  typedef uint16_t E;
+  typedef s4_m Frag;                   // four consecutive k
+  static constexpr int KS = 16;
+  static constexpr int KC = 16;
+  static constexpr int RS = 20;        // 16 used; 40 B: conflict-free 8-byte reads
+};
+template <int P> struct MShp {
+  static constexpr int NCHK = (M_NIP + MOp<P>::KC - 1) / MOp<P>::KC;   // chunks (IPs past 27 are zero rows)
+  static constexpr int KIND = M_NC * MOp<P>::RS;                       // elements per row kind
+  static constexpr int ROWB = 3 * KIND * (int)sizeof(typename MOp<P>::E);  // bytes of a chunk's rows
+  static constexpr int UB = ((ROWB > M_STG ? ROWB : M_STG) + 15) / 16 * 16;  // union bytes
+  static constexpr int WAVE = M_RAW + M_QP * M_PW / 2 + UB / 8;        // doubles per wavefront (even)
+  static_assert(WAVE % 2 == 0 && M_TAB % 2 == 0, "wavefront regions start 16-byte aligned");
+  static_assert(MOp<P>::KC % MOp<P>::KS == 0 && MOp<P>::KC % 2 == 0, "chunks of whole MFMA depths and IP pairs");
+};
 
 struct MMat {
   int model;
@@ -84,7 +103,15 @@ __device__ __forceinline__ float pick4(const float* x, int i) {
   return i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : x[3];
 }
 
-__global__ __launch_bounds__(64 * M_WMAX) void k_tangent_hex20_bf16(MArgs A) {
+template <int P>
+__global__ __launch_bounds__(64 * M_WMAX) void k_tangent_hex20_mfma(MArgs A) {
+  using O = MOp<P>;
+  using S = MShp<P>;
+  using E = typename O::E;
+  using Frag = typename O::Frag;
+  constexpr int KC = O::KC, RS = O::RS, KIND = S::KIND;
+  constexpr int ITEMS = (KC / 2) * M_NN;            // staging items per chunk: (IP pair, node)
+  constexpr int SR = (ITEMS + 63) / 64;             // staging rounds per chunk
   extern __shared__ __attribute__((aligned(16))) double sh[];
   const int64_t nel = A.nel;
   const int64_t nipg = nel * M_NIP;
@@ -97,27 +124,33 @@ __global__ __launch_bounds__(64 * M_WMAX) void k_tangent_hex20_bf16(MArgs A) {
   int32_t* err = A.err;
   const uint64_t ekey = A.ekey;
   const int tid = threadIdx.x;
-  const int nw = blockDim.x >> 6;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  double* sdN = sh;
-  double* sw = sh + M_NIP * M_NN * M_ND;
+  V4* sdN = reinterpret_cast<V4*>(sh);                // [27][20]: dN/dxi of the node (three and a zero)
+  double* sw = sh + M_NIP * M_NN * 2;
   {
     const double* __restrict__ tab = A.tab;
-    for (int i = tid; i < M_NIP * M_NN * M_ND; i += blockDim.x) sdN[i] = tab[M_NIP + M_NIP * M_NN + i];
+    for (int i = tid; i < M_NIP * M_NN; i += blockDim.x) {
+      const double* d = tab + M_NIP + M_NIP * M_NN + i * M_ND;
+      sdN[i] = V4{(float)d[0], (float)d[1], (float)d[2], 0.0f};
+    }
     for (int i = tid; i < M_NIP; i += blockDim.x) sw[i] = tab[i];
   }
-  __syncthreads();  // the only workgroup-wide wait
   const int ncnt = A.ncnt, cidx = (int)(blockIdx.x % (unsigned)ncnt);
 
-  double* raw = sh + M_TAB + (size_t)wv * M_WAVE;
+  double* raw = sh + M_TAB + (size_t)wv * S::WAVE;
   double* rgeo = raw;
   double* rst = raw + M_GEO_DW / 2;
   double* rrk = rst + M_ST_DW / 2;
   const uint8_t* srk = reinterpret_cast<const uint8_t*>(rrk);
-  double* sQ = raw + M_RAW;                                            // [27][9]
-  char* uni = reinterpret_cast<char*>(sQ + M_NIP * M_QW + 1);          // rows of a chunk | output staging
-  uint16_t* rows = reinterpret_cast<uint16_t*>(uni);                   // [3 kinds][80 columns][M_RS]
+  float* sP = reinterpret_cast<float*>(raw + M_RAW);                   // [32][20] row factors per IP
+  char* uni = reinterpret_cast<char*>(sP + M_QP * M_PW);               // rows of a chunk | output staging
+  E* rows = reinterpret_cast<E*>(uni);                                 // [3 kinds][80 columns][RS]
   float* stage = reinterpret_cast<float*>(uni);
+  {
+    const int lane = tid & 63;
+    for (int i = M_NIP * M_PW + lane; i < M_QP * M_PW; i += 64) sP[i] = 0.0f;  // padding IPs: zero rows
+  }
+  __syncthreads();  // the only workgroup-wide wait
 
   int32_t* work = A.work + cidx * kWorkStride;
   auto ticket = [&]() -> int32_t {
@@ -127,51 +160,53 @@ __global__ __launch_bounds__(64 * M_WMAX) void k_tangent_hex20_bf16(MArgs A) {
     if ((threadIdx.x & 63) == 0) t = __hip_atomic_fetch_add(wp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return t;
   };
-  auto prefetch_state = [&](int64_t e) {
+  // the next element's state (sigma, lambda, elastic flag: one component of 27 doubles per load), geometry and
+  // node ranks, copied into this wavefront's LDS: uniform bases, lane offsets of four bytes
+  auto prefetch = [&](int64_t e) {
     int lane = tid & 63;
     asm volatile("" : "+v"(lane));
+    const uint32_t lo = 4u * (uint32_t)lane;
+    if (lane < 2 * M_NIP) {
 #pragma unroll
-    for (int k = 0; k < M_ST_DW / 64; k++) {
-      const int d = min(64 * k + lane, 8 * M_NIP * 2 - 1);
-      const int c = d / (2 * M_NIP), o = d - c * (2 * M_NIP);
-      const int cg = c < M_NS ? c : c + 1;
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(st + cg * nipg + e * M_NIP) + o;
-      __builtin_amdgcn_global_load_lds(src, (lds_void_m*)(rst + 32 * k), 4, 0, 0);
+      for (int c = 0; c < M_NS + 2; c++) {
+        const int cg = c < M_NS ? c : c + 1;
+        const char* src = reinterpret_cast<const char*>(st + cg * nipg + e * M_NIP);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(src + lo), (lds_void_m*)(rst + M_NIP * c), 4, 0, 0);
+      }
     }
-    order_m();
-  };
-  auto prefetch_geo = [&](int64_t e, int k) {
-    int lane = tid & 63;
-    asm volatile("" : "+v"(lane));
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(geo + e * (M_NIP * M_GW));
-    const int d = min(64 * k + lane, M_NIP * M_GW * 2 - 1);
-    __builtin_amdgcn_global_load_lds(src + d, (lds_void_m*)(rgeo + 32 * k), 4, 0, 0);
-    order_m();
-  };
-  auto prefetch_rank = [&](int64_t e) {
-    int lane = tid & 63;
-    asm volatile("" : "+v"(lane));
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(rank + e * M_NN);
-    __builtin_amdgcn_global_load_lds(src + min(lane, M_NN / 4 - 1), (lds_void_m*)rrk, 4, 0, 0);
+    const char* gsrc = reinterpret_cast<const char*>(geo + e * (M_NIP * M_GW));
+#pragma unroll
+    for (int k = 0; k < M_GEO_DW / 64 - 1; k++)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(gsrc + lo + 256 * k), (lds_void_m*)(rgeo + 32 * k), 4, 0, 0);
+    if (lane < M_NIP * M_GW * 2 - (M_GEO_DW - 64))
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(gsrc + lo + 4 * (M_GEO_DW - 64)), (lds_void_m*)(rgeo + M_GEO_DW / 2 - 32), 4, 0, 0);
+    if (lane < M_NN / 4)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(rank + e * M_NN) + lane, (lds_void_m*)rrk, 4, 0, 0);
     order_m();
   };
   int32_t tk = ticket();
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   int64_t cur = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
   if (cur >= nel) return;  // uniform over the wavefront
-  prefetch_state(cur);
-  for (int k = 0; k < M_GEO_DW / 64; k++) prefetch_geo(cur, k);
-  prefetch_rank(cur);
+  prefetch(cur);
   tk = ticket();
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   int64_t nxt = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
+
+  // epilogue lane roles: column j = 4 bp + p of a tile, rows 4 g .. 4 g + 3
+  int lane0 = tid & 63;
+  asm volatile("" : "+v"(lane0));
+  const int p = lane0 & 3;
+  const bool p0 = (p & 1) != 0, p1 = (p & 2) != 0;
+  const float d0 = p == 0 ? 1.0f : 0.0f, d1 = p == 1 ? 1.0f : 0.0f, d2 = p == 2 ? 1.0f : 0.0f;
 
   while (cur < nel) {
     const int64_t e = cur;
     int lane = tid & 63;
     asm volatile("" : "+v"(lane));
-    // per IP: D coefficients (von_mises.rs:124-167) and deviator, one lane per IP; the Gram rows carry the
-    // square roots of c alpha/2, c beta and -c gamma
+    // per IP, one lane per IP: D coefficients (von_mises.rs:124-167), then the row factors of the IP:
+    // A = sqrt(c alpha/2) J^-T, T = sqrt(-c gamma) s J^-T (s the deviator as a symmetric 3x3 matrix) and
+    // sqrt(c beta) / sqrt(c alpha/2), so that the rows of node m are A dN_m, ratio * A dN_m and T dN_m
     if (lane < M_NIP) {
       const int q = lane;
       const int64_t ip = e * M_NIP + q;
@@ -201,132 +236,154 @@ __global__ __launch_bounds__(64 * M_WMAX) void k_tangent_hex20_bf16(MArgs A) {
         gamma = 6.0 * mp.G * mp.G * (lam / sdtr - 1.0 / d) / (norm_s * norm_s);
         beta = mp.K - alpha / 3.0;
       }
-      double* Q = sQ + q * M_QW;
-      Q[0] = sqrt(coef * alpha * 0.5);
-      Q[1] = sqrt(fmax(coef * beta, 0.0));
-      Q[2] = sqrt(fmax(-coef * gamma, 0.0));
+      const float wa = sqrtf((float)(coef * alpha * 0.5));
+      const float wb = sqrtf(fmaxf((float)(coef * beta), 0.0f));
+      const float wg = sqrtf(fmaxf((float)(-coef * gamma), 0.0f));
+      float Ji[9];
 #pragma unroll
-      for (int c = 0; c < 6; c++) Q[3 + c] = s[c];
+      for (int k = 0; k < 9; k++) Ji[k] = (float)rgeo[q * M_GW + k];
+      const float sxy = (float)(s[3] * ISQ2_M), syz = (float)(s[4] * ISQ2_M), sxz = (float)(s[5] * ISQ2_M);
+      const float Sm[3][3] = {{(float)s[0], sxy, sxz}, {sxy, (float)s[1], syz}, {sxz, syz, (float)s[2]}};
+      float* Pq = sP + q * M_PW;
+      float out[M_PW];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          out[3 * i + k] = wa * Ji[3 * k + i];
+          out[9 + 3 * i + k] = wg * (Sm[i][0] * Ji[3 * k] + Sm[i][1] * Ji[3 * k + 1] + Sm[i][2] * Ji[3 * k + 2]);
+        }
+      out[18] = wa > 0.0f ? wb / wa : 0.0f;
+      out[19] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < M_PW / 4; k++)
+        reinterpret_cast<V4*>(Pq)[k] = V4{out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]};
+    }
+    // column of each staging round's node (the node ranks are read before the next element's replace them)
+    int colr[SR];
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+      const int w = min(lane + 64 * r, ITEMS - 1);
+      colr[r] = 4 * srk[w % M_NN];
     }
     int32_t tk2 = ticket();  // the element after nxt, read at the end of this element
     const bool next = nxt < nel;  // uniform
-    if (next) {
-      lds_wait_m();  // the state of this element has been read
-      prefetch_state(nxt);
-    }
+    lds_wait_m();  // this element's state, geometry and ranks have been read; the row factors are written
+    if (next) prefetch(nxt);
     f4_m accM[15], accC[15];
+    // Gram products of a chunk on the upper tiles: A fragment = 16 columns of tile I x the k of this lane's
+    // group (four consecutive IPs for BF16, one for FP32), B fragment likewise for tile J; the gamma Gram is
+    // subtracted by negating its A fragments.  The first products of an element start from zero.
+    auto grams = [&](auto first) __attribute__((always_inline)) {
 #pragma unroll
-    for (int t = 0; t < 15; t++) { accM[t] = f4_m{0, 0, 0, 0}; accC[t] = f4_m{0, 0, 0, 0}; }
-#pragma unroll 1
-    for (int h = 0; h < M_NCHK; h++) {
-      lds_wait_m();  // coefficients written (first chunk); the previous chunk's rows consumed
-      // rows of IPs 16h + 2p, 16h + 2p + 1 for node m (item p * 20 + m): three kinds x three dofs, the fourth
-      // column of the node zero; IPs past 27 are zero rows
-#pragma unroll 1
-      for (int w = lane; w < (M_KC / 2) * M_NN; w += 64) {
-        const int p = w / M_NN, m = w - p * M_NN;
-        // the nine weighted row values (three kinds x three dofs) of each of the two IPs, in FP32; the products
-        // of the weights with G and ts are formed in FP32 and rounded once to BF16 (two IPs per conversion)
-        float v[2][9];
+      for (int ks = 0; ks < KC / O::KS; ks++) {
+        const int fo = (lane & 15) * RS + (P == 2 ? 4 * (lane >> 4) : O::KS * ks + (lane >> 4));
+        const bool z = decltype(first)::value && ks == 0;
+        int t = 0;
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          const int q = M_KC * h + 2 * p + t;
+        for (int I = 0; I < M_T; I++) {
+          const Frag a1 = *reinterpret_cast<const Frag*>(rows + 0 * KIND + 16 * I * RS + fo);
+          const Frag a2 = *reinterpret_cast<const Frag*>(rows + 1 * KIND + 16 * I * RS + fo);
+          Frag a3 = *reinterpret_cast<const Frag*>(rows + 2 * KIND + 16 * I * RS + fo);
+          if constexpr (P == 2) a3 ^= (short)0x8000;  // -sqrt(-c gamma) ts
+          else a3 = -a3;
 #pragma unroll
-          for (int z = 0; z < 9; z++) v[t][z] = 0.0f;
-          if (q < M_NIP) {
-            const double* Q = sQ + q * M_QW;
-            const double* dN = sdN + (q * M_NN + m) * M_ND;
-            const double* Ji = rgeo + q * M_GW;
-            double G[3];
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-              double sum = 0.0;
-#pragma unroll
-              for (int k = 0; k < 3; k++) sum += dN[k] * Ji[k * 3 + j];
-              G[j] = sum;
+          for (int J = I; J < M_T; J++, t++) {
+            const Frag b1 = *reinterpret_cast<const Frag*>(rows + 0 * KIND + 16 * J * RS + fo);
+            const Frag b2 = *reinterpret_cast<const Frag*>(rows + 1 * KIND + 16 * J * RS + fo);
+            const Frag b3 = *reinterpret_cast<const Frag*>(rows + 2 * KIND + 16 * J * RS + fo);
+            const f4_m zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (P == 2) {
+              accM[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, z ? zero : accM[t], 0, 0, 0);
+              accC[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b2, z ? zero : accC[t], 0, 0, 0);
+              accC[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, b3, accC[t], 0, 0, 0);
+            } else {
+              accM[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, z ? zero : accM[t], 0, 0, 0);
+              accC[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, z ? zero : accC[t], 0, 0, 0);
+              accC[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, b3, accC[t], 0, 0, 0);
             }
-            const double* sv = Q + 3;
-            const double sxy = sv[3] * ISQ2_M, syz = sv[4] * ISQ2_M, sxz = sv[5] * ISQ2_M;
-            const float ts0 = (float)(sv[0] * G[0] + sxy * G[1] + sxz * G[2]);
-            const float ts1 = (float)(sxy * G[0] + sv[1] * G[1] + syz * G[2]);
-            const float ts2 = (float)(sxz * G[0] + syz * G[1] + sv[2] * G[2]);
-            const float wa = (float)Q[0], wb = (float)Q[1], wg = (float)Q[2];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-              const float gi = (float)G[i];
-              v[t][i] = wa * gi;
-              v[t][3 + i] = wb * gi;
-            }
-            v[t][6] = wg * ts0; v[t][7] = wg * ts1; v[t][8] = wg * ts2;
           }
         }
-        const int col = 4 * srk[m];
+      }
+    };
+#pragma unroll 1
+    for (int h = 0; h < S::NCHK; h++) {
+      if (h > 0) lds_wait_m();  // the previous chunk's rows consumed
+      // rows of IPs KC h + 2 pr, KC h + 2 pr + 1 for node m (item pr * 20 + m): three kinds x three dofs, the
+      // fourth column of the node zero; IPs past 27 have zero row factors
 #pragma unroll
-        for (int z = 0; z < 3; z++) {
-          uint32_t* r32 = reinterpret_cast<uint32_t*>(rows + z * M_KIND + col * M_RS + 2 * p);
+      for (int r = 0; r < SR; r++) {
+        const int w = lane + 64 * r;
+        if (ITEMS % 64 == 0 || r < SR - 1 || w < ITEMS) {
+          const int pr = w / M_NN, m = w - pr * M_NN;
+          float v[2][9];
 #pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const f2_m pr = {v[0][3 * z + i], v[1][3 * z + i]};
-            r32[i * (M_RS / 2)] = __builtin_bit_cast(uint32_t, __builtin_convertvector(pr, bf2_m));
+          for (int t = 0; t < 2; t++) {
+            const int q = KC * h + 2 * pr + t;
+            const V4 dn = sdN[min(q, M_NIP - 1) * M_NN + m];
+            const V4* Pq = reinterpret_cast<const V4*>(sP + q * M_PW);
+            const V4 f0 = Pq[0], f1 = Pq[1], f2 = Pq[2], f3 = Pq[3], f4 = Pq[4];
+            const float Pf[M_PW] = {f0[0], f0[1], f0[2], f0[3], f1[0], f1[1], f1[2], f1[3], f2[0], f2[1],
+                                    f2[2], f2[3], f3[0], f3[1], f3[2], f3[3], f4[0], f4[1], f4[2], f4[3]};
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+              const float va = fmaf(Pf[3 * i], dn[0], fmaf(Pf[3 * i + 1], dn[1], Pf[3 * i + 2] * dn[2]));
+              v[t][i] = va;
+              v[t][3 + i] = Pf[18] * va;
+              v[t][6 + i] = fmaf(Pf[9 + 3 * i], dn[0], fmaf(Pf[9 + 3 * i + 1], dn[1], Pf[9 + 3 * i + 2] * dn[2]));
+            }
           }
-          r32[3 * (M_RS / 2)] = 0u;
+#pragma unroll
+          for (int z = 0; z < 3; z++) {
+            E* rr = rows + z * KIND + colr[r] * RS + 2 * pr;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+              if constexpr (P == 2) {
+                const f2_m pv = {v[0][3 * z + i], v[1][3 * z + i]};
+                *reinterpret_cast<uint32_t*>(rr + i * RS) = __builtin_bit_cast(uint32_t, __builtin_convertvector(pv, bf2_m));
+              } else {
+                rr[i * RS] = v[0][3 * z + i];
+                rr[i * RS + 1] = v[1][3 * z + i];
+              }
+            }
+            if constexpr (P == 2) *reinterpret_cast<uint32_t*>(rr + 3 * RS) = 0u;
+            else { rr[3 * RS] = 0.0f; rr[3 * RS + 1] = 0.0f; }
+          }
         }
       }
       lds_wait_m();
-      if (next) {  // the J^-1 of this chunk's IPs has been read (and, after the last chunk, the ranks)
-        if (h == 0) { for (int k = 0; k < 5; k++) prefetch_geo(nxt, k); }
-        else { for (int k = 5; k < M_GEO_DW / 64; k++) prefetch_geo(nxt, k); prefetch_rank(nxt); }
-      }
-      // Gram products of the chunk on the upper tiles: A fragment = 16 columns of tile I x 4 IPs of this
-      // lane's group, B fragment likewise for tile J (negated for the gamma term)
-      const int fo = (lane & 15) * M_RS + 4 * (lane >> 4);
-      int t = 0;
-#pragma unroll
-      for (int I = 0; I < M_T; I++) {
-        const s4_m a1 = *reinterpret_cast<const s4_m*>(rows + 0 * M_KIND + 16 * I * M_RS + fo);
-        const s4_m a2 = *reinterpret_cast<const s4_m*>(rows + 1 * M_KIND + 16 * I * M_RS + fo);
-        const s4_m a3 = *reinterpret_cast<const s4_m*>(rows + 2 * M_KIND + 16 * I * M_RS + fo);
-#pragma unroll
-        for (int J = I; J < M_T; J++, t++) {
-          const s4_m b1 = *reinterpret_cast<const s4_m*>(rows + 0 * M_KIND + 16 * J * M_RS + fo);
-          const s4_m b2 = *reinterpret_cast<const s4_m*>(rows + 1 * M_KIND + 16 * J * M_RS + fo);
-          s4_m b3 = *reinterpret_cast<const s4_m*>(rows + 2 * M_KIND + 16 * J * M_RS + fo);
-          b3 ^= (short)0x8000;  // -sqrt(-c gamma) ts: subtracts the gamma Gram
-          accM[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, accM[t], 0, 0, 0);
-          accC[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b2, accC[t], 0, 0, 0);
-          accC[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, b3, accC[t], 0, 0, 0);
-        }
-      }
+      if (h == 0) grams(std::true_type{});
+      else grams(std::false_type{});
       order_m();
     }
     // epilogue per tile: K = M^T (per node block) + tr(M) 1 + C into the staging at the stored offsets of the
-    // upper node blocks.  Lane: column j = 4 bp + p of tile J, rows 4g..4g+3 of tile I, so the four lanes of a
-    // quad hold one 4x4 node block of M by columns; its transpose and trace move between them by DPP (the
-    // padding row and column of a node are zero, so the trace over four is the trace over three).
+    // upper node blocks.  The four lanes of a quad hold one 4x4 node block of M by columns (lane p: column p);
+    // a two-step butterfly over the quad (exchange lane bit 0 with register bit 0, then bit 1 with bit 1)
+    // leaves row p in lane p, i.e. column p of M^T.  The padding row and column of a node are zero, so the
+    // trace over four is the trace over three.
     lds_wait_m();  // the rows are consumed (the staging overlays them)
     {
-      const int j = lane & 15, g = lane >> 4, bp = j >> 2, p = j & 3;
+      const int j = lane & 15, g = lane >> 4, bp = j >> 2;
       int t = 0;
 #pragma unroll
       for (int I = 0; I < M_T; I++)
 #pragma unroll
         for (int J = I; J < M_T; J++, t++) {
-          const float x[4] = {accM[t][0], accM[t][1], accM[t][2], accM[t][3]};
-          float T[4];  // T[s] = M(row 4g + p, col 4bp + (p ^ s)): entry (p, p ^ s) of the block, transposed
-          T[0] = pick4(x, p);
-          T[1] = quad_xor<1>(pick4(x, p ^ 1));
-          T[2] = quad_xor<2>(pick4(x, p ^ 2));
-          T[3] = quad_xor<3>(pick4(x, p ^ 3));
-          const float t1 = T[0] + quad_xor<1>(T[0]);
+          const float x0 = accM[t][0], x1 = accM[t][1], x2 = accM[t][2], x3 = accM[t][3];
+          float ra = quad_xor<1>(p0 ? x0 : x1), rb = quad_xor<1>(p0 ? x2 : x3);
+          const float y0 = p0 ? ra : x0, y1 = p0 ? x1 : ra, y2 = p0 ? rb : x2, y3 = p0 ? x3 : rb;
+          ra = quad_xor<2>(p1 ? y0 : y2);
+          rb = quad_xor<2>(p1 ? y1 : y3);
+          const float z0 = p1 ? ra : y0, z1 = p1 ? rb : y1, z2 = p1 ? y2 : ra;  // z_i = M(p, i)
+          const float dg = fmaf(x2, d2, fmaf(x1, d1, x0 * d0));          // M(p, p)
+          const float t1 = dg + quad_xor<1>(dg);
           const float tr = t1 + quad_xor<2>(t1);
           const int a = 4 * I + g, b = 4 * J + bp;
           if (p < 3 && (I < J || g <= bp)) {
-            const int o = ke_block_offset(M_NN, M_ND, a, b);
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-              const float v = accC[t][i] + pick4(T, i ^ p);
-              stage[o + i * 3 + p] = i == p ? v + tr : v;
-            }
+            float* o = stage + ke_block_offset(M_NN, M_ND, a, b) + p;
+            o[0] = fmaf(tr, d0, accC[t][0] + z0);
+            o[3] = fmaf(tr, d1, accC[t][1] + z1);
+            o[6] = fmaf(tr, d2, accC[t][2] + z2);
           }
         }
       if (lane == 0) { stage[ke_block_offset(M_NN, M_ND, 6, 6) - 1] = 0.0f; stage[M_EW - 1] = 0.0f; }
@@ -357,38 +414,36 @@ __global__ void k_zero_work_m(int32_t* w) {
   if (threadIdx.x < kWorkCounters) w[threadIdx.x * kWorkStride] = 0;
 }
 
-static size_t bf16_lds(int nw) { return sizeof(double) * ((size_t)M_TAB + (size_t)nw * M_WAVE); }
+template <int P>
+size_t mfma20_lds(int nw) { return sizeof(double) * ((size_t)M_TAB + (size_t)nw * MShp<P>::WAVE); }
 
-static int bf16_waves(const pm_fem* f) {
+template <int P>
+int mfma20_waves(const pm_fem* f) {
   int nw = M_WMAX;
-  while (nw > 2 && (int64_t)bf16_lds(nw) > f->lds_per_block) nw--;
-  return (int64_t)bf16_lds(nw) <= f->lds_per_block ? nw : 0;
+  while (nw > 2 && (int64_t)mfma20_lds<P>(nw) > f->lds_per_block) nw--;
+  return (int64_t)mfma20_lds<P>(nw) <= f->lds_per_block ? nw : 0;
 }
 
 }  // namespace
 
-// true when a group's BF16 tangent runs as the persistent Hex20 matrix-core kernel: Hex20 cells with 27 IPs,
-// a Poisson ratio >= 0 (beta >= 0, so that every Gram weight has a real square root), and a device whose
-// workgroups may hold its LDS
-bool tangent_hex20_bf16_fits(const pm_fem* f, const Group& g) {
-  return g.kind == HEX20 && g.nip == M_NIP && !f->axisym && !g.plane_stress && f->cfg.tangent_precision == 2 &&
-         g.param.poisson >= 0.0 && bf16_waves(f) > 0;
+// true when a group's reduced-precision tangent can run as the persistent Hex20 matrix-core kernel: Hex20 cells
+// with 27 IPs, an FP32 or BF16 tangent, a Poisson ratio >= 0 (beta >= 0, so that every Gram weight has a real
+// square root), and a device whose workgroups may hold its LDS
+bool tangent_hex20_mfma_fits(const pm_fem* f, const Group& g) {
+  const int prec = f->cfg.tangent_precision;
+  if (g.kind != HEX20 || g.nip != M_NIP || f->axisym || g.plane_stress || g.param.poisson < 0.0) return false;
+  return prec == 1 ? mfma20_waves<1>(f) > 0 : prec == 2 ? mfma20_waves<2>(f) > 0 : false;
 }
 
-bool launch_tangent_hex20_bf16(pm_fem* f, Group& g) {
-  if (!tangent_hex20_bf16_fits(f, g)) return false;
-  if (g.nel == 0) return true;
-  const int nw = bf16_waves(f);
-  const size_t shm = bf16_lds(nw);
+template <int P>
+static void launch_hex20_mfma(pm_fem* f, Group& g) {
+  const int nw = mfma20_waves<P>(f);
+  const size_t shm = mfma20_lds<P>(nw);
   const int per_cu = std::max<int>(1, (int)(f->lds_per_cu / (int64_t)shm));
   const int64_t grid = std::min<int64_t>((int64_t)f->n_cu * per_cu, (g.nel + nw - 1) / nw);
-  if (!ke_range_ok(g.ke_base, g.nel, g.kew, M_EW, f->n_ke_values)) {
-    f->launch_refused = true;  // reported by check_device_error; never launched out of range
-    return true;
-  }
   static thread_local bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tangent_hex20_bf16), hipFuncAttributeMaxDynamicSharedMemorySize, (int)f->lds_per_block);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tangent_hex20_mfma<P>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)f->lds_per_block);
     (void)hipGetLastError();
     attr_set = true;
   }
@@ -398,7 +453,19 @@ bool launch_tangent_hex20_bf16(pm_fem* f, Group& g) {
   hipLaunchKernelGGL(k_zero_work_m, dim3(1), dim3(64), 0, f->stream, f->d_work);  // a kernel, not a memset (graph replays)
   MArgs A{g.nel, g.d_rank, g.d_geo, g.d_tab, g.d_state, f->d_ke32 + g.ke_base, mp, f->thickness, f->d_err, ekey, f->d_work,
           (int)std::min<int64_t>(kWorkCounters, grid)};
-  hipLaunchKernelGGL(k_tangent_hex20_bf16, dim3((unsigned)grid), dim3(64 * nw), shm, f->stream, A);
+  hipLaunchKernelGGL((k_tangent_hex20_mfma<P>), dim3((unsigned)grid), dim3(64 * nw), shm, f->stream, A);
+}
+
+// Launches the persistent Hex20 matrix-core tangent of one group; false when it does not fit the group
+bool launch_tangent_hex20_mfma(pm_fem* f, Group& g) {
+  if (!tangent_hex20_mfma_fits(f, g)) return false;
+  if (g.nel == 0) return true;
+  if (!ke_range_ok(g.ke_base, g.nel, g.kew, M_EW, f->n_ke_values)) {
+    f->launch_refused = true;  // reported by check_device_error; never launched out of range
+    return true;
+  }
+  if (f->cfg.tangent_precision == 1) launch_hex20_mfma<1>(f, g);
+  else launch_hex20_mfma<2>(f, g);
   return true;
 }
 

@@ -5,7 +5,8 @@ differ in the last bit), in FP64 and FP32 storage, on meshes whose element count
 wavefronts with zero, one, a few and about forty elements each (the next element's data is prefetched into
 LDS while the current one is accumulated, so more than one element per wavefront exercises the prefetch; a
 stale or half-written prefetch would show as an O(1) difference).  The persistent kernel is deterministic
-(two passes bitwise equal) and matches the oracle at 1e-12."""
+(two passes bitwise equal) and matches the oracle at 1e-12.  The persistent matrix-core kernels (FP32 and BF16
+operands) are checked against the FP64 K and against the per-group matrix-core kernel."""
 import numpy as np
 import pytest
 
@@ -89,9 +90,9 @@ def test_persistent_hex20_tangent_full_size(prec):
     assert np.array_equal(fp.apply_kk(v), kp)
 
 
-def _bf16_pair(n, seed=7):
-    """FP64 handle and the two BF16 matrix-core kernels (persistent Hex20 form, per-group form) after the same
-    partly plastic increment."""
+def _mfma_pair(n, prec, seed=7):
+    """FP64 handle and the two reduced-precision matrix-core kernels (persistent Hex20 form, per-group form)
+    after the same partly plastic increment."""
     mesh = G.hex20_block(*n, 1.0, 1.0, 1.0, seed=20261203, jitter=0.05)
     base = FemBase(mesh, [(1, ParamSolid(1.0, StressStrain.VonMises(2e5, 0.3, 1e3, 250.0), 27))])
     X = mesh.coords
@@ -100,8 +101,8 @@ def _bf16_pair(n, seed=7):
     u0[:, 2] = -2.5e-3 * X[:, 0] * X[:, 2]
     u0 = u0.ravel() + rng.uniform(-2e-6, 2e-6, X.size)
     out = []
-    for prec, kern in (("f64", "default"), ("bf16", "mfma"), ("bf16", "mfma-wg")):
-        cfg = Config(mesh).set_tangent_precision(prec).set_tangent_kernel(kern)
+    for p, kern in (("f64", "default"), (prec, "mfma"), (prec, "mfma-wg")):
+        cfg = Config(mesh).set_tangent_precision(p).set_tangent_kernel(kern)
         f = Fem(describe(mesh, base, cfg, Essential(), Natural()), cfg, keep_history=False)
         f.backup_secondary_values()
         f.set_ddu(u0)
@@ -110,18 +111,26 @@ def _bf16_pair(n, seed=7):
     return mesh, out
 
 
+# bars relative to the largest |K|: the matrix-core kernels' bars (FP32 operands and accumulation 2e-5, BF16
+# operands with FP32 accumulation 3e-2), and the slack over the per-group kernel's own distance from the FP64 K
+MFMA_TOL = {"f32": (2e-5, 5e-6), "bf16": (3e-2, 1e-2)}
+KV_TOL = {"f32": 1e-5, "bf16": 1e-2}
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
 @pytest.mark.parametrize("n", [(3, 2, 2), (5, 7, 3), (20, 20, 10)])
-def test_persistent_hex20_bf16_tangent(n):
-    """BF16 operands with FP32 accumulation: K of the persistent kernel within the BF16 bar of the FP64 K
-    (3e-2 relative to the largest entry, the bar of the per-group matrix-core kernel) and no further from it
-    than the per-group kernel's K plus the BF16 rounding of either; diagonal node blocks exactly symmetric;
-    f_int bitwise the FP64 handle's (the stress update and B^T sigma stay FP64); repeatable."""
-    mesh, (f64, fp, fw) = _bf16_pair(n)
+def test_persistent_hex20_mfma_tangent(n, prec):
+    """FP32 or BF16 operands with FP32 accumulation: K of the persistent kernel within the matrix-core bar of
+    the FP64 K and no further from it than the per-group kernel's K plus the operand rounding of either;
+    diagonal node blocks exactly symmetric; f_int bitwise the FP64 handle's (the stress update and B^T sigma
+    stay FP64); repeatable."""
+    mesh, (f64, fp, fw) = _mfma_pair(n, prec)
     v64 = f64.get_csr()[2]
     vp, vw = fp.get_csr()[2], fw.get_csr()[2]
     ep, ew = rel_err(vp, v64), rel_err(vw, v64)
-    print(f"{n} bf16: |K - K64| / max|K64|: persistent {ep:.2e}, per-group {ew:.2e}", flush=True)
-    assert ep < 3e-2 and ep < ew + 1e-2
+    print(f"{n} {prec}: |K - K64| / max|K64|: persistent {ep:.2e}, per-group {ew:.2e}", flush=True)
+    bar, slack = MFMA_TOL[prec]
+    assert ep < bar and ep < ew + slack
     assert np.array_equal(fp.get_f_int(), f64.get_f_int())
     for cell in sorted({0, mesh.ncell // 2, mesh.ncell - 1}):
         k = fp.element_kk(cell)
@@ -132,14 +141,15 @@ def test_persistent_hex20_bf16_tangent(n):
     assert np.array_equal(fp.get_csr()[2], vp)
 
 
-def test_persistent_hex20_bf16_tangent_full_size():
-    """64x64x32 (131,072 elements): K v of the persistent BF16 kernel within 1e-2 of the FP64 K v (relative,
-    max norm) and repeatable bit for bit."""
-    mesh, (f64, fp, fw) = _bf16_pair((64, 64, 32))
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_persistent_hex20_mfma_tangent_full_size(prec):
+    """64x64x32 (131,072 elements): K v of the persistent matrix-core kernel within the bar of the FP64 K v
+    (relative, max norm: 1e-5 FP32, 1e-2 BF16) and repeatable bit for bit."""
+    mesh, (f64, fp, fw) = _mfma_pair((64, 64, 32), prec)
     v = np.random.default_rng(3).normal(size=f64.ndof)
     k64, kp, kw = f64.apply_kk(v), fp.apply_kk(v), fw.apply_kk(v)
     ep, ew = rel_err(kp, k64), rel_err(kw, k64)
-    print(f"64x64x32 bf16: |K v - K64 v| / max|K64 v|: persistent {ep:.2e}, per-group {ew:.2e}", flush=True)
-    assert ep < 1e-2
+    print(f"64x64x32 {prec}: |K v - K64 v| / max|K64 v|: persistent {ep:.2e}, per-group {ew:.2e}", flush=True)
+    assert ep < KV_TOL[prec]
     fp.element_pass()
     assert np.array_equal(fp.apply_kk(v), kp)
