@@ -34,7 +34,7 @@ constexpr int H_NIP = 27, H_NN = 20, H_ND = 3, H_NS = 6;
 constexpr int H_NH = 10, H_NT = 55;     // node pairs, upper 2x2 tiles of node blocks
 constexpr int H_PW = 7;                 // node vector: G (3), unused, ts (3): odd stride
 constexpr int H_CH = 3, H_NCH = 9;      // IPs per chunk (60 staging lanes), chunks
-constexpr int H_QW = 9;                 // per IP: c alpha/2, c beta, c gamma, deviator (6)
+constexpr int H_QW = 19;                // per IP slot: coefficients (3), deviator (6), J^-1 (9), IP index
 constexpr int H_GW = 11;                // per IP in the geometry array: J^-1 (9), det J, radius
 constexpr int H_GEO_DW = 640;           // geometry of one element in LDS, dwords (594 used, 10 loads)
 constexpr int H_ST_DW = 448;            // state of one element in LDS, dwords (8 x 27 doubles used, 7 loads)
@@ -114,56 +114,47 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
   double* rst = raw + H_GEO_DW / 2;                 // sigma (6), lambda, elastic flag; 27 each
   double* rrk = rst + H_ST_DW / 2;                  // ranks of the 20 nodes (bytes)
   const uint8_t* srk = reinterpret_cast<const uint8_t*>(rrk);
-  double* sQ = raw + H_RAW;                         // [27][9] per-IP coefficients and deviator
+  double* sQ = raw + H_RAW;                         // [27 slots][19] per-IP coefficients, deviator, J^-1
   double* sP = sQ + H_NIP * H_QW;                   // [3 IPs][20 nodes][7] node vectors of a chunk
   double* sU = sQ;                                  // output staging (over the per-IP data)
 
-  // Direct global->LDS copies of element e's data into this wavefront's region: the state (components 0-5,
-  // 7, 8 of the state arrays, 54 dwords each) in seven loads, and the geometry (594 contiguous dwords) in
-  // ten loads of 64 dwords, load k being issued once the IPs whose J^-1 it overwrites have been staged;
-  // surplus lanes re-read the last dword into padding.  The lane index passes through an empty asm so that
-  // no per-lane address is kept live (and spilled) across elements.
-  auto prefetch_state = [&](int64_t e) {
-    int lane = tid & 63;
-    asm volatile("" : "+v"(lane));
-#pragma unroll
-    for (int k = 0; k < H_ST_DW / 64; k++) {
-      const int d = min(64 * k + lane, 8 * H_NIP * 2 - 1);
-      const int c = d / (2 * H_NIP), o = d - c * (2 * H_NIP);
-      const int cg = c < H_NS ? c : c + 1;  // z (component 6) is not needed
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(st + cg * nipg + e * H_NIP) + o;
-      __builtin_amdgcn_global_load_lds(src, (lds_void*)(rst + 32 * k), 4, 0, 0);
-    }
-    order();
-  };
-  auto prefetch_geo = [&](int64_t e, int k) {
-    int lane = tid & 63;
-    asm volatile("" : "+v"(lane));
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(geo + e * (H_NIP * H_GW));
-    const int d = min(64 * k + lane, H_NIP * H_GW * 2 - 1);
-    __builtin_amdgcn_global_load_lds(src + d, (lds_void*)(rgeo + 32 * k), 4, 0, 0);
-    order();
-  };
+  // Direct global->LDS copies of element e's data into this wavefront's region, in one burst: the state
+  // (components 0-5, 7, 8 of the state arrays: 54 dwords each, one load per component), the geometry (594
+  // contiguous dwords in ten loads) and the ranks (5 dwords); uniform bases, lane offsets of four bytes.
   const uint8_t* __restrict__ rank = A.rank;
-  auto prefetch_rank = [&](int64_t e) {  // 20 bytes = 5 dwords (the element's ranks start 4-byte aligned)
+  auto prefetch = [&](int64_t e) {
     int lane = tid & 63;
     asm volatile("" : "+v"(lane));
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(rank + e * H_NN);
-    __builtin_amdgcn_global_load_lds(src + min(lane, H_NN / 4 - 1), (lds_void*)rrk, 4, 0, 0);
+    const uint32_t lo = 4u * (uint32_t)lane;
+    if (lane < 2 * H_NIP) {
+#pragma unroll
+      for (int c = 0; c < H_NS + 2; c++) {
+        const int cg = c < H_NS ? c : c + 1;  // z (component 6) is not needed
+        const char* src = reinterpret_cast<const char*>(st + cg * nipg + e * H_NIP);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(src + lo), (lds_void*)(rst + H_NIP * c), 4, 0, 0);
+      }
+    }
+    const char* gsrc = reinterpret_cast<const char*>(geo + e * (H_NIP * H_GW));
+#pragma unroll
+    for (int k = 0; k < H_GEO_DW / 64 - 1; k++)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(gsrc + lo + 256 * k), (lds_void*)(rgeo + 32 * k), 4, 0, 0);
+    if (lane < H_NIP * H_GW * 2 - (H_GEO_DW - 64))
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(gsrc + lo + 4 * (H_GEO_DW - 64)), (lds_void*)(rgeo + H_GEO_DW / 2 - 32), 4, 0, 0);
+    if (lane < H_NN / 4)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(rank + e * H_NN) + lane, (lds_void*)rrk, 4, 0, 0);
     order();
   };
   int32_t tk = ticket();
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   int64_t cur = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
   if (cur >= nel) return;  // uniform over the wavefront
-  prefetch_state(cur);
-  for (int k = 0; k < H_GEO_DW / 64; k++) prefetch_geo(cur, k);
-  prefetch_rank(cur);
+  prefetch(cur);
   tk = ticket();
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   int64_t nxt = cidx + (int64_t)ncnt * __builtin_amdgcn_readfirstlane(tk);
-  static_assert(64 * (H_NCH - 1) + 64 <= 2 * H_GW * H_CH * H_NCH && H_GEO_DW / 64 == H_NCH + 1,
-                "geometry load k = chunk c overwrites only IPs staged by then; the tenth follows the last chunk");
+  // elastic IPs: D = 2G I + lambda I(x)I, so their part of K_ab is G (Y_ab^T + tr(Y_ab) 1) + lambda Y_ab with
+  // Y = sum_q c_q G_a G_b^T, one Gram; they are taken first and Y converted before the plastic IPs
+  const double Gm = mp.G, lamb = mp.K - 2.0 * mp.G / 3.0;
 
   while (cur < nel) {
     const int64_t e = cur;
@@ -181,16 +172,22 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
     // this lane's node vector in a chunk's staging: IP ql of the chunk, node m
     const bool stager = lane < H_CH * H_NN;
     const int sql = lane / H_NN, sm = lane - sql * H_NN;
-    // per IP: D coefficients (von_mises.rs:124-167) and deviator, one lane per IP
+    // per IP: D coefficients (von_mises.rs:124-167) and deviator, one lane per IP, written to the IP's slot:
+    // the elastic IPs first (in IP order), then the plastic ones
+    const bool el = lane < H_NIP && (mp.model == 0 || rst[7 * H_NIP + (lane < H_NIP ? lane : 0)] != 0.0);
+    const uint64_t emask = __ballot(el);
+    const int n_el = __builtin_popcountll(emask);  // uniform
     if (lane < H_NIP) {
       const int q = lane;
+      const uint64_t below = (1ull << q) - 1ull;
+      const int slot = el ? __builtin_popcountll(emask & below) : n_el + __builtin_popcountll(~emask & below);
       const int64_t ip = e * H_NIP + q;
       const double det = rgeo[q * H_GW + 9];
       if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)ip);
       double coef = det * sw[q] * thickness * 1.0;
       double alpha, beta, gamma = 0.0;
       double s[6] = {0, 0, 0, 0, 0, 0};
-      if (mp.model == 0 || rst[7 * H_NIP + q] != 0.0) {
+      if (el) {
         alpha = 2.0 * mp.G;
         beta = mp.K - alpha / 3.0;
       } else {
@@ -210,19 +207,21 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
         gamma = 6.0 * mp.G * mp.G * (lam / sdtr - 1.0 / d) / (norm_s * norm_s);
         beta = mp.K - alpha / 3.0;
       }
-      double* Q = sQ + q * H_QW;
-      Q[0] = coef * alpha * 0.5;
+      double* Q = sQ + slot * H_QW;
+      Q[0] = el ? coef : coef * alpha * 0.5;  // an elastic slot keeps c (its Gram weight)
       Q[1] = coef * beta;
       Q[2] = coef * gamma;
 #pragma unroll
       for (int c = 0; c < 6; c++) Q[3 + c] = s[c];
+#pragma unroll
+      for (int k = 0; k < 9; k++) Q[9 + k] = rgeo[q * H_GW + k];
+      Q[18] = (double)q;
     }
+    const int myrank = srk[lane % H_NN];  // rank of this lane's staging node
     tk = ticket();  // the element after nxt, read at the end of this element (it has long arrived)
     const bool next = nxt < nel;  // uniform
-    if (next) {
-      lds_wait();  // the state of this element has been read
-      prefetch_state(nxt);
-    }
+    lds_wait();  // this element's state, geometry and ranks have been read (and its slots written)
+    if (next) prefetch(nxt);
     double K[2][2][3][3];
     double Dg[2][2];
 #pragma unroll
@@ -235,15 +234,33 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
 #pragma unroll
           for (int j = 0; j < 3; j++) K[u][v][i][j] = 0.0;
       }
+    // the elastic IPs' Gram Y (held in K) to their part of K: G Y^T + lambda Y, and G tr(Y) on the diagonal
+    auto convert_elastic = [&]() __attribute__((always_inline)) {
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+          double Y[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) Y[i][j] = K[u][v][i][j];
+          Dg[u][v] = Gm * (Y[0][0] + Y[1][1] + Y[2][2]);
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) K[u][v][i][j] = fma(Gm, Y[j][i], lamb * Y[i][j]);
+        }
+    };
 #pragma unroll 1
     for (int c = 0; c < H_NCH; c++) {
       const int q0 = c * H_CH;
       lds_wait();  // coefficients (first chunk) written; the previous chunk's node vectors consumed
       if (stager) {
-        const int q = q0 + sql;
-        const double* Q = sQ + q * H_QW;
+        const double* Q = sQ + (q0 + sql) * H_QW;
+        const int q = (int)Q[18];
         const double* dN = sdN + (q * H_NN + sm) * H_ND;
-        const double* Ji = rgeo + q * H_GW;
+        const double* Ji = Q + 9;
         double G[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -253,7 +270,7 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
           G[j] = sum;
         }
         const double* sv = Q + 3;
-        double* P = sP + (sql * H_NN + srk[sm]) * H_PW;  // node m in the slot of its rank
+        double* P = sP + (sql * H_NN + myrank) * H_PW;  // node m in the slot of its rank
         const double sxy = sv[3] * ISQ2_H, syz = sv[4] * ISQ2_H, sxz = sv[5] * ISQ2_H;
         P[0] = G[0]; P[1] = G[1]; P[2] = G[2];
         P[4] = sv[0] * G[0] + sxy * G[1] + sxz * G[2];
@@ -261,13 +278,35 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
         P[6] = sxz * G[0] + syz * G[1] + sv[2] * G[2];
       }
       lds_wait();
-      if (next) {  // the J^-1 of this chunk's IPs has been read (and, after the last chunk, the ranks)
-        prefetch_geo(nxt, c);
-        if (c == H_NCH - 1) { prefetch_geo(nxt, H_NCH); prefetch_rank(nxt); }
-      }
+      // the chunk's elastic slots first, then (after converting Y where the elastic slots end) its plastic ones
+      const int nel_c = min(max(n_el - q0, 0), H_CH);  // uniform
       if (active) {
 #pragma unroll 1
-        for (int ql = 0; ql < H_CH; ql++) {
+        for (int ql = 0; ql < nel_c; ql++) {  // elastic: Y += c G_a G_b^T
+          const double cq = sQ[(q0 + ql) * H_QW];
+          const double* Pq = sP + ql * H_NN * H_PW;
+          double Ga[2][3];
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) Ga[u][i] = Pq[(2 * I + u) * H_PW + i];
+#pragma unroll
+          for (int v = 0; v < 2; v++) {
+            const double* Pb = Pq + (2 * J + v) * H_PW;
+            double ub[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) ub[i] = cq * Pb[i];
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+              for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) K[u][v][i][j] = fma(Ga[u][i], ub[j], K[u][v][i][j]);
+          }
+        }
+        if (nel_c < H_CH && n_el >= q0) convert_elastic();  // the first plastic slot is in this chunk
+#pragma unroll 1
+        for (int ql = nel_c; ql < H_CH; ql++) {
           const double* Q = sQ + (q0 + ql) * H_QW;
           const double ha = Q[0], be = Q[1], ga = Q[2];
           const double* Pq = sP + ql * H_NN * H_PW;
@@ -326,6 +365,7 @@ __global__ __launch_bounds__(64 * H_WMAX) void k_tangent_hex20(H20Args<KT> A) {
       }
       order();
     }
+    if (n_el == H_NIP) convert_elastic();  // every IP elastic (uniform)
 #pragma unroll
     for (int u = 0; u < 2; u++)
 #pragma unroll
