@@ -133,14 +133,17 @@ template <typename KT>
 struct ScatArgs { int64_t nnzb; const int32_t* korder; const int32_t* kptr; const int32_t* kmap; const KT* ke; double* kval; float* kval32; };
 struct RgArgs { int64_t np; const int32_t* fptr; const int32_t* fmap; const double* fe; double* fint; };
 
+typedef __attribute__((address_space(3))) void lds_void_k;
+
 // ------------------------------------------------------------------------------------------------
 // update kernel: gather of the displacement increments -> strain increment -> stress update from the
 // backup state -> element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
 // One IP per thread, epb elements per workgroup.  The workgroup's IP geometry (contiguous in HBM) and the
-// shape-function derivatives are copied into LDS with coalesced loads (the derivatives with an odd stride
-// per IP, so that the lanes of one element, which read different IPs, hit different banks).  For the force
-// loop each IP leaves T = c sigma J^-T (and the hoop term c sigma_zz / r, axisymmetric) in place of its
-// geometry, so f_e[m] = sum_q T_q dN_mq: one thread per element node, ND*ND FMA per IP.
+// shape-function derivatives are copied into LDS by direct global->LDS loads, all issued before the first
+// wait (the derivatives row by row, with an odd stride per IP, so that the lanes of one element, which read
+// different IPs, hit different banks).  The displacement gradient is H J^-1 with H = sum_m u_m dN_m^T.  For
+// the force loop each IP leaves T = c sigma J^-T (and the hoop term c sigma_zz / r, axisymmetric) in place
+// of its geometry, so f_e[m] = sum_q T_q dN_mq: one thread per element node, ND*ND FMA per IP.
 // ------------------------------------------------------------------------------------------------
 template <int ND, int NN>
 __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
@@ -184,15 +187,31 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   double z = mine ? st_in[NS * nipg + ip] : 0.0, lam = mine ? st_in[(NS + 1) * nipg + ip] : 0.0;
   double el_ = mine ? st_in[(NS + 2) * nipg + ip] : 0.0, ep = mine ? st_in[(NS + 3) * nipg + ip] : 0.0;
   {
-    const double* __restrict__ gsrc = geo + e0 * nip * GW;
-    const int ng = nloc * nip * GW;
-    for (int i = tid; i < ng; i += blockDim.x) sG[i] = gsrc[i];
-    for (int i = tid; i < nip * NN * ND; i += blockDim.x) sdN[i + i / (NN * ND)] = tdN[i];
+    // geometry: 2 ng dwords in loads of one dword per lane (TPB per round); a wavefront's load lands at its
+    // own 64-dword slice of the round
+    const uint32_t* __restrict__ gsrc = reinterpret_cast<const uint32_t*>(geo + e0 * nip * GW);
+    uint32_t* gdst = reinterpret_cast<uint32_t*>(sG);
+    const int nd2 = 2 * nloc * nip * GW, wv = tid >> 6, ln = tid & 63;
+    static_assert(TPB % 64 == 0, "whole wavefronts");
+    for (int r = 0; r < 2 * GW; r++) {  // 2 ng <= 2 TPB GW dwords (epb nip <= TPB)
+      const int base = r * TPB + wv * 64;  // uniform per wavefront
+      if (r * TPB >= nd2) break;
+      if (base + ln < nd2) __builtin_amdgcn_global_load_lds(gsrc + base + ln, (lds_void_k*)(gdst + base), 4, 0, 0);
+    }
+    // derivatives: IP q's row of NN ND doubles to sdN + q DS, in slices of 64 dwords
+    constexpr int RW = 2 * NN * ND, RS = (RW + 63) / 64;  // dwords per row, slices per row
+    const uint32_t* __restrict__ dsrc = reinterpret_cast<const uint32_t*>(tdN);
+    uint32_t* ddst = reinterpret_cast<uint32_t*>(sdN);
+    for (int t = wv; t < nip * RS; t += TPB / 64) {  // uniform per wavefront
+      const int q = t / RS, o = (t - q * RS) * 64;
+      if (o + ln < RW) __builtin_amdgcn_global_load_lds(dsrc + q * RW + o + ln, (lds_void_k*)(ddst + 2 * q * DS + o), 4, 0, 0);
+    }
     for (int i = tid; i < nloc * NN; i += blockDim.x) {
       const int64_t p = conn[e0 * NN + i];
 #pragma unroll
       for (int d = 0; d < ND; d++) sU[i * ND + d] = ddu[p * ND + d];
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the direct loads (not tracked by the compiler)
   }
   __syncthreads();
   if (mine) {
@@ -207,28 +226,32 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
     const double* dN = sdN + q * DS;
     const double* Ue = sU + el * NN * ND;
     if (!(det > 0.0)) raise_error(err, ERR_DETJ, ekey + (uint64_t)ip);
-    // strain increment (calculate_strain.rs:18-60), tensor components then Mandel
-    // shear sums are accumulated as 2 e_ij (one FMA per term) and halved once
-    double exx = 0, eyy = 0, ezz = 0, exy = 0, eyz = 0, exz = 0;
+    // strain increment (calculate_strain.rs:18-60): displacement gradient L = H J^-1 with
+    // H = sum_m u_m dN_m^T (dN with respect to the reference coordinates), then the tensor components and
+    // Mandel; the shear sums 2 e_ij = L_ij + L_ji are halved once
+    double H[ND][ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int k = 0; k < ND; k++) H[i][k] = 0.0;
 #pragma unroll 4
-    for (int m = 0; m < NN; m++) {
-      double G[ND];
+    for (int m = 0; m < NN; m++)
+#pragma unroll
+      for (int i = 0; i < ND; i++)
+#pragma unroll
+        for (int k = 0; k < ND; k++) H[i][k] = fma(Ue[m * ND + i], dN[m * ND + k], H[i][k]);
+    double L[ND][ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++)
 #pragma unroll
       for (int j = 0; j < ND; j++) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < ND; k++) s += dN[m * ND + k] * Ji[k][j];
-        G[j] = s;
+        for (int k = 0; k < ND; k++) s = fma(H[i][k], Ji[k][j], s);
+        L[i][j] = s;
       }
-      if constexpr (ND == 2) {
-        double ux = Ue[m * 2], uy = Ue[m * 2 + 1];
-        exx += ux * G[0]; eyy += uy * G[1]; exy += ux * G[1]; exy += uy * G[0];
-      } else {
-        double ux = Ue[m * 3], uy = Ue[m * 3 + 1], uz = Ue[m * 3 + 2];
-        exx += ux * G[0]; eyy += uy * G[1]; ezz += uz * G[2];
-        exy += ux * G[1]; exy += uy * G[0]; eyz += uy * G[2]; eyz += uz * G[1]; exz += ux * G[2]; exz += uz * G[0];
-      }
-    }
+    double exx = L[0][0], eyy = L[1][1], ezz = 0, exy = L[0][1] + L[1][0], eyz = 0, exz = 0;
+    if constexpr (ND == 3) { ezz = L[2][2]; eyz = L[1][2] + L[2][1]; exz = L[0][2] + L[2][0]; }
     if (axisym) for (int m = 0; m < NN; m++) ezz += Ue[m * ND] * tN[q * NN + m] / r;
     double de[NS];
     de[0] = exx; de[1] = eyy; de[2] = ezz; de[3] = (exy * 0.5) * SQ2;
