@@ -56,7 +56,7 @@ def _profile(nx, ny, nz, precision, kernel, what):
     if (nx, ny, nz) != (64, 64, 32):
         return None
     tag = {("f64", "default"): "f64_vector", ("f64", "vector"): "f64_vector", ("f64", "mfma"): "f64_mfma",
-           ("f32", "default"): "f32_vector", ("f32", "vector"): "f32_vector", ("f32", "mfma"): "f32_mfma",
+           ("f32", "default"): "f32_mfma", ("f32", "vector"): "f32_vector", ("f32", "mfma"): "f32_mfma",
            ("bf16", "default"): "bf16_mfma", ("bf16", "mfma"): "bf16_mfma"}.get((precision, kernel))
     path = os.path.join(ROOT, "profiles", f"{what}_hex20_64x64x32_{tag}.json")
     if tag is None or not os.path.exists(path):
