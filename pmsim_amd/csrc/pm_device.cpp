@@ -81,6 +81,8 @@ int device_alloc(pm_fem* f) {
   if (dalloc(f, &f->d_lidx, nnzl) || upload(f, f->d_lidx, f->h_lidx.data(), nnzl)) return 1;
   if (dalloc(f, &f->d_kmap_ptr, f->nnzu + 1) || upload(f, f->d_kmap_ptr, f->h_kmap_ptr.data(), f->nnzu + 1)) return 1;
   if (dalloc(f, &f->d_korder, (int64_t)f->h_korder.size()) || upload(f, f->d_korder, f->h_korder.data(), (int64_t)f->h_korder.size())) return 1;
+  f->nkwave = (int64_t)f->h_kwave.size() - 1;
+  if (dalloc(f, &f->d_kwave, (int64_t)f->h_kwave.size()) || upload(f, f->d_kwave, f->h_kwave.data(), (int64_t)f->h_kwave.size())) return 1;
   if (dalloc(f, &f->d_kmap, (int64_t)f->h_kmap.size()) || upload(f, f->d_kmap, f->h_kmap.data(), (int64_t)f->h_kmap.size())) return 1;
   if (dalloc(f, &f->d_fmap_ptr, f->npoint + 1) || upload(f, f->d_fmap_ptr, f->h_fmap_ptr.data(), f->npoint + 1)) return 1;
   if (dalloc(f, &f->d_fmap, (int64_t)f->h_fmap.size()) || upload(f, f->d_fmap, f->h_fmap.data(), (int64_t)f->h_fmap.size())) return 1;
@@ -120,6 +122,7 @@ int device_alloc(pm_fem* f) {
   std::vector<int32_t>().swap(f->h_kmap);
   std::vector<int32_t>().swap(f->h_kmap_ptr);
   std::vector<int32_t>().swap(f->h_korder);
+  std::vector<int32_t>().swap(f->h_kwave);
   std::vector<int32_t>().swap(f->h_fmap);
   std::vector<int32_t>().swap(f->h_fmap_ptr);
   return 0;
@@ -159,7 +162,7 @@ void device_free(pm_fem* f) {
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   for (auto& g : f->groups) { fr(g.d_conn); fr(g.d_rank); g.d_rank = nullptr; fr(g.d_tab); fr(g.d_state); fr(g.d_backup); fr(g.d_geo); g.d_conn = nullptr; g.d_tab = g.d_state = g.d_backup = g.d_geo = nullptr; }
   fr(f->d_X); fr(f->d_urp); fr(f->d_ucol); fr(f->d_lrp); fr(f->d_lcol); fr(f->d_lidx); fr(f->d_kval); fr(f->d_kval32); fr(f->d_ke); fr(f->d_ke32);
-  fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
+  fr(f->d_kmap_ptr); fr(f->d_kmap); fr(f->d_korder); fr(f->d_kwave); fr(f->d_fe); fr(f->d_fmap_ptr); fr(f->d_fmap);
   fr(f->d_u); fr(f->d_ddu); fr(f->d_fint); fr(f->d_fext); fr(f->d_rr); fr(f->d_mdu); fr(f->d_mdu0); fr(f->d_lam);
   fr(f->d_y); fr(f->d_cval); fr(f->d_peq); fr(f->d_free); fr(f->d_pcg); fr(f->d_binv); fr(f->d_red); fr(f->d_scal); fr(f->d_err); fr(f->d_work);
   fr(f->d_arc); f->d_arc = nullptr;
@@ -168,7 +171,7 @@ void device_free(pm_fem* f) {
   if (f->stream) (void)hipStreamDestroy(f->stream);
   f->stream = nullptr;
   f->d_X = nullptr; f->d_urp = f->d_ucol = f->d_lrp = f->d_lcol = f->d_lidx = nullptr; f->d_kval = nullptr; f->d_kval32 = nullptr; f->d_ke = nullptr; f->d_ke32 = nullptr;
-  f->d_kmap_ptr = f->d_kmap = nullptr; f->d_korder = nullptr; f->d_fe = nullptr; f->d_fmap_ptr = f->d_fmap = nullptr;
+  f->d_kmap_ptr = f->d_kmap = nullptr; f->d_korder = nullptr; f->d_kwave = nullptr; f->d_fe = nullptr; f->d_fmap_ptr = f->d_fmap = nullptr;
   f->d_u = f->d_ddu = f->d_fint = f->d_fext = f->d_rr = f->d_mdu = f->d_mdu0 = f->d_lam = nullptr;
   f->d_y = f->d_cval = nullptr; f->d_peq = nullptr; f->d_free = f->d_pcg = f->d_binv = f->d_red = f->d_scal = nullptr; f->d_err = nullptr; f->d_work = nullptr;
 }

@@ -279,6 +279,20 @@ int build_patterns(pm_fem* f) {
     std::iota(f->h_korder.begin(), f->h_korder.end(), 0);
     std::stable_sort(f->h_korder.begin(), f->h_korder.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
   }
+  // Wavefront ranges of the balanced K gather: whole CSR blocks, in order, packed greedily so that a range
+  // holds at most 64 contributions and 64 blocks (one lane per contribution, one owner lane per block); a
+  // block with more than 64 contributions has a range of its own, taken in rounds of 64.
+  {
+    std::vector<int32_t> kw(1, 0);
+    int64_t b = 0;
+    while (b < nnzu) {
+      const int64_t start = b++;
+      if (kptr[start + 1] - kptr[start] <= 64)
+        while (b < nnzu && b - start < 64 && kptr[b + 1] - kptr[start] <= 64) b++;
+      kw.push_back((int32_t)b);
+    }
+    f->h_kwave.swap(kw);
+  }
   // upload happens in device_alloc; keep host copies in temporaries stored on the handle
   f->h_kmap_ptr.swap(kptr);
   f->h_kmap.swap(kmap);

@@ -169,6 +169,7 @@ struct pm_fem {
   std::vector<int32_t> h_full2up;      // full block -> stored upper block, sign bit: read transposed
   std::vector<int32_t> h_kmap_ptr, h_kmap;   // K gather map (host copy until upload)
   std::vector<int32_t> h_korder;             // gather chunks (TPB CSR blocks each) in processing order
+  std::vector<int32_t> h_kwave;              // K gather by wavefronts: first CSR block of each wavefront's range
   std::vector<int32_t> h_fmap_ptr, h_fmap;   // R gather map
   // K gather by rows: per (node, cell) contribution in ascending cell order, the element's block row
   // (start block index, node count) and the CSR position of each of its blocks within the node's row
@@ -201,6 +202,8 @@ struct pm_fem {
   const char* refuse_msg = nullptr;    // the refusal's message (null: an out-of-range element-matrix launch)
   int32_t* d_kmap_ptr = nullptr;       // nnzb+1
   int32_t* d_korder = nullptr;         // gather chunk processing order
+  int32_t* d_kwave = nullptr;          // nkwave+1: whole CSR blocks of at most 64 contributions per wavefront
+  int64_t nkwave = 0;
   int32_t* d_kmap = nullptr;           // contributions: element block index (ascending cell order)
   double* d_fe = nullptr;              // element forces
   int64_t n_fe = 0;

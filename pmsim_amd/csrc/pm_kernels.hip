@@ -131,6 +131,8 @@ struct TanArgs {
 };
 template <typename KT>
 struct ScatArgs { int64_t nnzb; const int32_t* korder; const int32_t* kptr; const int32_t* kmap; const KT* ke; double* kval; float* kval32; };
+template <typename KT>
+struct ScatWArgs { int64_t nwave; const int32_t* kwave; const int32_t* kptr; const int32_t* kmap; const KT* ke; double* kval; float* kval32; };
 struct RgArgs { int64_t np; const int32_t* fptr; const int32_t* fmap; const double* fe; double* fint; };
 
 typedef __attribute__((address_space(3))) void lds_void_k;
@@ -682,6 +684,68 @@ __device__ __forceinline__ void scatter_body(int64_t blk_order, const ScatArgs<K
   }
 }
 
+// Balanced K gather: a wavefront per range of whole CSR blocks (at most 64 contributions and 64 blocks,
+// host-packed; a block with more contributions has its own range, taken in rounds of 64).  Every lane loads
+// one contribution (map entry and element block) and stages it in the wavefront's LDS; the lane that owns a
+// block then sums that block's staged contributions in ascending order from zero (the sums of scatter_body,
+// bit for bit), and the range's blocks are written contiguously.  No lane waits for another's contribution
+// list, so all of a range's loads are in flight together.  Wavefronts never wait for each other (no
+// barriers); LDS accesses of one wavefront complete in order.
+template <int BS, typename KT>
+__device__ __forceinline__ void scatter_wave_body(int64_t w, const ScatWArgs<KT>& A) {
+  constexpr int B2 = BS * BS;
+  __shared__ double stage[TPB * B2];
+  const int lane = threadIdx.x & 63;
+  if (w >= A.nwave) return;  // uniform over the wavefront
+  double* st = stage + (threadIdx.x >> 6) * 64 * B2;
+  const int32_t* __restrict__ kptr = A.kptr;
+  const int32_t* __restrict__ kmap = A.kmap;
+  const KT* __restrict__ ke = A.ke;
+  const int32_t b0 = A.kwave[w], nb = A.kwave[w + 1] - b0;  // nb <= 64
+  const int32_t c0 = kptr[b0], c1 = kptr[b0 + nb];
+  int32_t s = 0, e = 0;  // owner lane: its block's contributions
+  if (lane < nb) { s = kptr[b0 + lane]; e = kptr[b0 + lane + 1]; }
+  double acc[B2];
+#pragma unroll
+  for (int c = 0; c < B2; c++) acc[c] = 0.0;
+  for (int32_t r0 = c0; r0 < c1; r0 += 64) {  // one round unless the range is a single large block
+    const int32_t k = r0 + lane;
+    if (k < c1) {
+      const int32_t idx = kmap[k];
+      const KT* src = ke + (idx & 0x7fffffff);
+      KT raw[B2];
+#pragma unroll
+      for (int c = 0; c < B2; c++) raw[c] = src[c];
+      const bool tr = idx < 0;
+#pragma unroll
+      for (int r = 0; r < BS; r++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) st[lane * B2 + r * BS + c] = (double)(tr ? raw[c * BS + r] : raw[r * BS + c]);
+    }
+    asm volatile("" ::: "memory");
+    if (lane < nb) {
+      const int32_t lo = max(s, r0), hi = min(e, r0 + 64);
+      for (int32_t t = lo; t < hi; t++)  // ascending contribution order
+#pragma unroll
+        for (int c = 0; c < B2; c++) acc[c] += st[(t - r0) * B2 + c];
+    }
+    asm volatile("" ::: "memory");
+  }
+  if (lane < nb)
+#pragma unroll
+    for (int c = 0; c < B2; c++) st[lane * B2 + c] = acc[c];
+  asm volatile("" ::: "memory");
+  const int nv = nb * B2;
+  if (A.kval) {
+    double* dst = A.kval + (int64_t)b0 * B2;
+    for (int i = lane; i < nv; i += 64) dst[i] = st[i];
+  }
+  if (A.kval32) {
+    float* dst = A.kval32 + (int64_t)b0 * B2;
+    for (int i = lane; i < nv; i += 64) dst[i] = (float)st[i];
+  }
+}
+
 // deterministic gather of element forces into the global internal force vector (assembly.rs:38-46)
 template <int ND>
 __device__ __forceinline__ void rgather_body(int64_t blk, const RgArgs& A) {
@@ -721,6 +785,10 @@ __global__ __launch_bounds__(TW) void kb_tangent(const TanArgs<KT>* __restrict__
 }
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void k_scatter(ScatArgs<KT> A) { scatter_body<BS, KT>(xcd_block(), A); }
+template <int BS, typename KT>
+__global__ __launch_bounds__(TPB) void k_scatter_w(ScatWArgs<KT> A) {
+  scatter_wave_body<BS, KT>(xcd_block() * (TPB / 64) + (threadIdx.x >> 6), A);
+}
 template <int BS, typename KT>
 __global__ __launch_bounds__(TPB) void kb_scatter(const ScatArgs<KT>* __restrict__ jobs, const int32_t* __restrict__ map) {
   int j, w;
@@ -1020,18 +1088,28 @@ static RgArgs rg_args(const pm_fem* f) {
   return A;
 }
 
+template <typename KT>
+static ScatWArgs<KT> scatw_args(const pm_fem* f, const KT* ke) {
+  ScatWArgs<KT> A;
+  A.nwave = f->nkwave; A.kwave = f->d_kwave; A.kptr = f->d_kmap_ptr; A.kmap = f->d_kmap; A.ke = ke;
+  A.kval = f->cfg.tangent_precision == 0 ? f->d_kval : nullptr;
+  A.kval32 = f->cfg.tangent_precision == 0 ? nullptr : f->d_kval32;
+  return A;
+}
+
 void launch_scatter(pm_fem* f) {
-  const int64_t n = f->nnzu;
-  const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
+  const int64_t nw = f->nkwave;
+  if (nw <= 0) return;
+  const unsigned grid = (unsigned)((nw + TPB / 64 - 1) / (TPB / 64));
   const bool lowp = f->cfg.tangent_precision != 0;
   if (!lowp) {
-    const ScatArgs<double> A = scat_args(f, (const double*)f->d_ke);
-    if (f->bs == 2) hipLaunchKernelGGL((k_scatter<2, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
-    else hipLaunchKernelGGL((k_scatter<3, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    const ScatWArgs<double> A = scatw_args(f, (const double*)f->d_ke);
+    if (f->bs == 2) hipLaunchKernelGGL((k_scatter_w<2, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    else hipLaunchKernelGGL((k_scatter_w<3, double>), dim3(grid), dim3(TPB), 0, f->stream, A);
   } else {
-    const ScatArgs<float> A = scat_args(f, (const float*)f->d_ke32);
-    if (f->bs == 2) hipLaunchKernelGGL((k_scatter<2, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
-    else hipLaunchKernelGGL((k_scatter<3, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    const ScatWArgs<float> A = scatw_args(f, (const float*)f->d_ke32);
+    if (f->bs == 2) hipLaunchKernelGGL((k_scatter_w<2, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
+    else hipLaunchKernelGGL((k_scatter_w<3, float>), dim3(grid), dim3(TPB), 0, f->stream, A);
   }
 }
 
