@@ -483,8 +483,14 @@ def main():
     vbytes = 8 if a.precision == "f64" else 4
     nnzu = (f.nnzb + f.ndof // 3) // 2
     scatter_bytes = ke_values * vbytes + nnzu * 9 * vbytes + nel * 210 * 4 + (nnzu + 1) * 4
-    out["hbm"] = {"kernel": "k_scatter", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+    out["hbm"] = {"kernel": "k_scatter_w", "achieved": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": scatter_bytes / (ms["scatter"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": scatter_bytes}
+    # the update per IP: the state from the backup in and the new state out (sigma, z, lambda, epbar read and
+    # written, the flag written: 152 B), the stored geometry in (88 B), its share of the element's
+    # displacement increments in and of its force vector out (60 x 8 B / 27 each)
+    update_bytes = f.nip * (152 + 88 + 2 * 60 * 8 / 27)
+    out["hbm_update"] = {"kernel": "k_update", "achieved": update_bytes / (ms["update"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": update_bytes / (ms["update"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": update_bytes}
     f.close()
     del f
     if not a.no_serve:
