@@ -1,5 +1,5 @@
 // Host-side setup: element batches, node-block CSR pattern, deterministic scatter maps,
-// external forces, and the host direct solver used as fallback and judge.
+// external forces, and the host direct solver used as fallback and as the check of the iterative solves.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>

@@ -4,8 +4,10 @@
 // Same closed form as the node-block vector kernel of pm_kernels.hip (von_mises.rs:124-167,
 // element_solid.rs:168-185):
 //   K_ab = sum_q alpha/2 (Ga.Gb) 1 + alpha/2 Gb Ga^T + beta Ga Gb^T + gamma tsa tsb^T
-// with the same terms summed in the same order per entry (its element matrices agree with that kernel's to
-// rounding: the two compile to different contractions in places).  The organisation is what differs:
+// except that the elastic IPs (alpha = 2G, beta = lambda, gamma = 0) are summed first as one Gram
+// Y_ab = sum_q c Ga Gb^T, turned once into G (Y_ab^T + tr(Y_ab) 1) + lambda Y_ab; the plastic IPs then add
+// their terms in the node-block kernel's order (the element matrices agree with that kernel's to rounding).
+// The organisation also differs:
 //  * one workgroup per CU of up to twelve wavefronts; every wavefront works through elements one at a time,
 //    each taken from a work counter one element ahead (so wavefronts that get more issue cycles take more
 //    elements and all finish together; the counters are spread over many addresses), and never waits for

@@ -1,7 +1,7 @@
 """The persistent Hex20 tangent (27 IPs) against the node-block vector kernel with one workgroup per element
-group: the same closed form summed in the same order per entry, so the element matrices, the assembled K and
-K v agree to rounding (1e-14 relative to the largest entry; the compilers' contractions of the two kernels
-differ in the last bit), in FP64 and FP32 storage, on meshes whose element counts leave the persistent
+group: the same closed form (the persistent kernel sums its elastic IPs as one Gram turned into their terms
+once), so the element matrices, the assembled K and K v agree to rounding (1e-14 relative to the largest
+entry), in FP64 and FP32 storage, on meshes whose element counts leave the persistent
 wavefronts with zero, one, a few and about forty elements each (the next element's data is prefetched into
 LDS while the current one is accumulated, so more than one element per wavefront exercises the prefetch; a
 stale or half-written prefetch would show as an O(1) difference).  The persistent kernel is deterministic
