@@ -42,7 +42,7 @@ def _pair_pass(n, prec, seed=7):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("n", [(3, 2, 2), (5, 7, 3), (20, 20, 10)])
-def test_persistent_hex20_tangent_bitwise(n, prec):
+def test_persistent_hex20_tangent_vs_per_group(n, prec):
     mesh, u0, (fp, fw) = _pair_pass(n, prec)
     plastic = np.mean(fp.get_ip_state()[:, fp.ns + 2] == 0.0)
     assert 0.0 < plastic < 1.0
