@@ -56,7 +56,7 @@ def _profile(nx, ny, nz, precision, kernel, what):
     if (nx, ny, nz) != (64, 64, 32):
         return None
     tag = {("f64", "default"): "f64_vector", ("f64", "vector"): "f64_vector", ("f64", "mfma"): "f64_mfma",
-           ("f32", "default"): "f32_mfma", ("f32", "vector"): "f32_vector", ("f32", "mfma"): "f32_mfma",
+           ("f32", "default"): "f32_vector", ("f32", "vector"): "f32_vector", ("f32", "mfma"): "f32_mfma",
            ("bf16", "default"): "bf16_mfma", ("bf16", "mfma"): "bf16_mfma"}.get((precision, kernel))
     path = os.path.join(ROOT, "profiles", f"{what}_hex20_64x64x32_{tag}.json")
     if tag is None or not os.path.exists(path):
@@ -311,7 +311,10 @@ def run_precision(a):
            "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, {loading}",
                       "elements": int(mesh.ncell)}, "runs": {}}
     for prec in a.precisions.split(","):
-        cfg.set_tangent_precision(prec).set_scaled_rr(1e-9).set_pcg(1e-9, a.pcg_max).set_n_max_iterations(30).set_verbose(True, True)
+        # "f32-vector": the FP32 tangent from the vector kernel (FP64 products rounded when stored) instead of
+        # the matrix cores
+        cfg.set_tangent_kernel("vector" if prec.endswith("-vector") else "default")
+        cfg.set_tangent_precision(prec.split("-")[0]).set_scaled_rr(1e-9).set_pcg(1e-9, a.pcg_max).set_n_max_iterations(30).set_verbose(True, True)
         d = describe(mesh, base, cfg, ess, nat)
         f = Fem(d, cfg, keep_history=False)
         print(f"{prec}: handle ready ({f.ndof} DOFs, {f.nnzb} K blocks), {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -396,7 +399,7 @@ def main():
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision", "pcg"])
     ap.add_argument("--pcg-iters", type=int, default=1600, help="pcg: PCG iterations per timed solve")
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
-    ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first")
+    ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first (f32-vector: FP32 from the vector kernel)")
     ap.add_argument("--uz-stations", default="", help="precision: top uz after each increment (comma-separated) instead of --n-inc")
     ap.add_argument("--pcg-max", type=int, default=200000, help="precision: PCG iteration limit per linear solve")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")

@@ -47,8 +47,7 @@ typedef struct pm_config {
   int32_t pcg_max_iter;             /* default 20000 */
   int32_t keep_history;             /* store u and IP state of every output step on the host */
   double tol_rr_scaled;             /* > 0 adds ||R||inf <= tol_rr_scaled * ||F_int||inf as a convergence test */
-  int32_t tangent_kernel;           /* 0: default (vector kernel for FP64; matrix cores for BF16, and for FP32 where the persistent Hex20
-                                       matrix-core kernel fits, else the vector kernel), 1: node-block vector kernel (FP64 arithmetic;
+  int32_t tangent_kernel;           /* 0: default (vector kernel for FP64 / FP32, matrix cores for BF16), 1: node-block vector kernel (FP64 arithmetic;
                                        persistent form for Hex20 with 27 IPs, same terms, agrees to rounding), 2: matrix-core (MFMA) kernel in the tangent precision
                                        (FP32 / BF16: persistent form for Hex20 with 27 IPs and Poisson ratio >= 0),
                                        3: tune at pm_initialize (fastest measured; FP64 variants agree to rounding, not bitwise),

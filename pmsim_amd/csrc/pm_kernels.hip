@@ -959,17 +959,17 @@ void launch_update(pm_fem* f, Group& g, bool do_update) {
   PM_DISPATCH_KIND(g.kind, ND, NN, hipLaunchKernelGGL((k_update<ND, NN>), dim3((unsigned)grid), dim3(TPB), shm, f->stream, A));
 }
 
-// tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16 and for FP32
-// groups that the persistent Hex20 matrix-core kernel fits, the vector kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
+// tangent kernel of a group: the tuned or forced choice, else the default (matrix cores for BF16, the vector
+// kernel otherwise).  1: vector kernel (the persistent Hex20 form where it fits, tangent_hex20_fits), 2: matrix
 // cores (for FP32 and BF16 the persistent Hex20 form where it fits, tangent_hex20_mfma_fits), 4: vector kernel
 // with one workgroup per element group, 5: the per-group matrix-core kernel (never the persistent forms)
 static int tangent_kernel_of(const pm_fem* f, const Group& g) {
   const int prec = f->cfg.tangent_precision;
   if (prec == 2) return g.tune_kernel == 5 ? 5 : 2;  // BF16 runs on the matrix cores only
   if (g.tune_kernel) return g.tune_kernel;
-  // FP32: the persistent Hex20 matrix-core kernel where it fits (measured faster than the vector kernel),
-  // else the vector kernel (faster than the per-group FP32 matrix-core kernel); FP64: the vector kernel
-  if (prec == 1 && tangent_hex20_mfma_fits(f, g)) return 2;
+  // FP64 and FP32: the vector kernel.  For FP32 its FP64 products rounded once on store keep an inexact
+  // Newton solve closest to the FP64 one (the matrix-core form, FP32 accumulation, is faster per pass but
+  // no faster per solve, which the PCG dominates; `mfma` selects it)
   return 1;
 }
 // true when a group's tangent runs as the persistent Hex20 kernel
