@@ -92,8 +92,9 @@ def test_uniform_traction_3d(kind):
 
 
 def test_residual_apply_kk_and_element_access():
-    """After one plastic update of the C1 footing: R = F_int - l F_ext bit for bit (l = 1 and 0.5), y = K x equal
-    to the assembled CSR's product, and every cell's local K and f_int equal to the oracle's (1e-12)."""
+    """After one plastic update of the pressurised cylinder (SPO 7.5.1, Qn(t) loads): R = F_int - l F_ext bit for
+    bit (l = 1 and 0.5), y = K x equal to the assembled CSR's product, and every cell's local K and f_int equal
+    to the oracle's (1e-12)."""
     import scipy.sparse as sp
     mesh, base, cfg, ess, nat = cases.spo_751_pres_cylin()  # Qn(t) loads: a non-zero F_ext
     d = describe(mesh, base, cfg, ess, nat)
