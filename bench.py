@@ -311,9 +311,12 @@ def run_precision(a):
            "config": {"workload": f"Hex20 unit cube {a.nx}x{a.ny}x{a.nz}, 27 IP, von Mises, {loading}",
                       "elements": int(mesh.ncell)}, "runs": {}}
     for prec in a.precisions.split(","):
-        # "f32-vector": the FP32 tangent from the vector kernel (FP64 products rounded when stored) instead of
-        # the matrix cores
-        cfg.set_tangent_kernel("vector" if prec.endswith("-vector") else "default")
+        # a suffix picks the tangent kernel: "-vector" (FP64 products rounded when stored; the FP32 default) or
+        # "-mfma" (matrix cores; the BF16 default)
+        kern = prec.split("-")[1] if "-" in prec else "default"
+        if kern not in ("default", "vector", "mfma"):
+            raise SystemExit(f"unknown tangent kernel suffix in {prec!r}")
+        cfg.set_tangent_kernel(kern)
         cfg.set_tangent_precision(prec.split("-")[0]).set_scaled_rr(1e-9).set_pcg(1e-9, a.pcg_max).set_n_max_iterations(30).set_verbose(True, True)
         d = describe(mesh, base, cfg, ess, nat)
         f = Fem(d, cfg, keep_history=False)
@@ -399,7 +402,7 @@ def main():
     ap.add_argument("--workload", default="hex20", choices=["hex20", "mixed2d", "serve", "precision", "pcg"])
     ap.add_argument("--pcg-iters", type=int, default=1600, help="pcg: PCG iterations per timed solve")
     ap.add_argument("--n-inc", type=int, default=5, help="precision: increments of uz = -0.001")
-    ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first (f32-vector: FP32 from the vector kernel)")
+    ap.add_argument("--precisions", default="f64,f32,bf16", help="precision: tangent precisions, FP64 first (a suffix -vector or -mfma picks the kernel)")
     ap.add_argument("--uz-stations", default="", help="precision: top uz after each increment (comma-separated) instead of --n-inc")
     ap.add_argument("--pcg-max", type=int, default=200000, help="precision: PCG iteration limit per linear solve")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
