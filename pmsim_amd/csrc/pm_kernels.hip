@@ -137,6 +137,13 @@ struct RgArgs { int64_t np; const int32_t* fptr; const int32_t* fmap; const doub
 
 typedef __attribute__((address_space(3))) void lds_void_k;
 
+// the value held by the neighbouring lane of this lane's pair (lanes 2k and 2k + 1; DPP quad permutation
+// [1,0,3,2] on both words of the double)
+__device__ __forceinline__ double pair_lane(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0xB1, 0xf, 0xf, false), __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xf, 0xf, false));
+}
+
 // ------------------------------------------------------------------------------------------------
 // update kernel: gather of the displacement increments -> strain increment -> stress update from the
 // backup state -> element internal force f_e = sum_q c_q B_q^T sigma_q   (element_solid.rs:124-138, 190-224)
@@ -289,23 +296,55 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
     gq[ND * ND] = axisym ? (coef * sig[2]) / r : 0.0;
   }
   __syncthreads();
-  for (int w = tid; w < nloc * NN; w += blockDim.x) {
-    const int el = w / NN, m = w - el * NN;
+  // f_e: each thread takes two nodes of an element (m and m + NH, so every T read serves both) over one
+  // half of the IPs; the two halves sit in neighbouring lanes and are added once, lower half first
+  constexpr int NH = (NN + 1) / 2;
+  const int split = nip >= 8 ? 2 : 1;  // uniform
+  const int qh = split == 2 ? (nip + 1) / 2 : nip;
+  const int nwork = nloc * NH * split;  // even when split (the lanes of a pair are active together)
+  for (int w = tid; w < nwork; w += blockDim.x) {
+    const int h = split == 2 ? (w & 1) : 0;
+    const int pw = split == 2 ? (w >> 1) : w;
+    const int el = pw / NH, m0 = pw - el * NH;
+    const bool two = (NN % 2 == 0) || m0 + NH < NN;
+    const int m1 = two ? m0 + NH : m0;
     const int64_t e = e0 + el;
-    double fm[ND];
+    double f0[ND], f1[ND];
 #pragma unroll
-    for (int d = 0; d < ND; d++) fm[d] = 0.0;
-    const double* T = sG + el * nip * GW;
-    for (int q = 0; q < nip; q++, T += GW) {
-      const double* dN = sdN + q * DS + m * ND;
+    for (int d = 0; d < ND; d++) f0[d] = f1[d] = 0.0;
+    const int qa = h ? qh : 0, qb = h ? nip : qh;
+    const double* T = sG + (el * nip + qa) * GW;
+    for (int q = qa; q < qb; q++, T += GW) {
+      const double* dN0 = sdN + q * DS + m0 * ND;
+      const double* dN1 = sdN + q * DS + m1 * ND;
 #pragma unroll
       for (int i = 0; i < ND; i++)
 #pragma unroll
-        for (int k = 0; k < ND; k++) fm[i] += T[i * ND + k] * dN[k];
-      if (axisym) fm[0] += T[ND * ND] * tN[q * NN + m];
+        for (int k = 0; k < ND; k++) {
+          const double t = T[i * ND + k];
+          f0[i] += t * dN0[k];
+          f1[i] += t * dN1[k];
+        }
+      if (axisym) {
+        f0[0] += T[ND * ND] * tN[q * NN + m0];
+        f1[0] += T[ND * ND] * tN[q * NN + m1];
+      }
     }
+    if (split == 2) {
 #pragma unroll
-    for (int d = 0; d < ND; d++) fe[(e * NN + m) * ND + d] = fm[d];
+      for (int d = 0; d < ND; d++) {
+        const double g0 = pair_lane(f0[d]), g1 = pair_lane(f1[d]);
+        f0[d] += g0;  // the lower half's lane: its half plus the upper one
+        f1[d] += g1;
+      }
+    }
+    if (h == 0) {
+#pragma unroll
+      for (int d = 0; d < ND; d++) fe[(e * NN + m0) * ND + d] = f0[d];
+      if (two)
+#pragma unroll
+        for (int d = 0; d < ND; d++) fe[(e * NN + m1) * ND + d] = f1[d];
+    }
   }
 }
 
