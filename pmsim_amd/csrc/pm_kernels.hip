@@ -796,8 +796,21 @@ __device__ __forceinline__ void rgather_body(int64_t blk, const RgArgs& A) {
   if (t >= A.np * ND) return;
   const int64_t a = t / ND;
   const int d = (int)(t - a * ND);
+  // contributions four at a time: their map entries, then their values, requested together (the sum
+  // still runs in ascending order)
   double s = 0.0;
-  for (int32_t k = fptr[a]; k < fptr[a + 1]; k++) s += fe[(int64_t)fmap[k] * ND + d];
+  const int32_t k1 = fptr[a + 1];
+  for (int32_t k = fptr[a]; k < k1; k += 4) {
+    int32_t m[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) m[j] = fmap[min(k + j, k1 - 1)];
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = fe[(int64_t)m[j] * ND + d];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j < k1) s += v[j];
+  }
   fint[t] = s;
 }
 
