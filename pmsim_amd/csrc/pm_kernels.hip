@@ -154,6 +154,64 @@ __device__ __forceinline__ double pair_lane(double v) {
 // the force loop each IP leaves T = c sigma J^-T (and the hoop term c sigma_zz / r, axisymmetric) in place
 // of its geometry, so f_e[m] = sum_q T_q dN_mq: one thread per element node, ND*ND FMA per IP.
 // ------------------------------------------------------------------------------------------------
+// f_e[m] = sum_q T_q dN_mq over an element block's IPs (T = c sigma J^-T, and the hoop term, in the IP's
+// geometry slot).  PAIR (rules of eight or more IPs): each thread takes two nodes of an element (m and
+// m + NH, so every T read serves both) over one half of the IPs; the halves sit in neighbouring lanes and
+// are added once, lower half first.  Otherwise one thread per node over all IPs.
+template <int ND, int NN, bool PAIR>
+__device__ __forceinline__ void force_loop(const double* sdN, const double* sG, const double* __restrict__ tN,
+                                           double* __restrict__ fe, int64_t e0, int nloc, int nip, int axisym) {
+  constexpr int GW = ND * ND + 2, DS = NN * ND + 1;
+  constexpr int NH = PAIR ? (NN + 1) / 2 : NN;  // node groups per element
+  constexpr int SPLIT = PAIR ? 2 : 1;
+  const int qh = PAIR ? (nip + 1) / 2 : nip;
+  const int nwork = nloc * NH * SPLIT;  // even when PAIR (the lanes of a pair are active together)
+  for (int w = threadIdx.x; w < nwork; w += blockDim.x) {
+    const int h = PAIR ? (w & 1) : 0;
+    const int pw = PAIR ? (w >> 1) : w;
+    const int el = pw / NH, m0 = pw - el * NH;
+    const bool two = PAIR && ((NN % 2 == 0) || m0 + NH < NN);
+    const int m1 = two ? m0 + NH : m0;
+    const int64_t e = e0 + el;
+    double f0[ND], f1[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) f0[d] = f1[d] = 0.0;
+    const int qa = h ? qh : 0, qb = h ? nip : qh;
+    const double* T = sG + (el * nip + qa) * GW;
+    for (int q = qa; q < qb; q++, T += GW) {
+      const double* dN0 = sdN + q * DS + m0 * ND;
+      const double* dN1 = sdN + q * DS + m1 * ND;
+#pragma unroll
+      for (int i = 0; i < ND; i++)
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          const double t = T[i * ND + k];
+          f0[i] += t * dN0[k];
+          if constexpr (PAIR) f1[i] += t * dN1[k];
+        }
+      if (axisym) {
+        f0[0] += T[ND * ND] * tN[q * NN + m0];
+        if constexpr (PAIR) f1[0] += T[ND * ND] * tN[q * NN + m1];
+      }
+    }
+    if constexpr (PAIR) {
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        const double g0 = pair_lane(f0[d]), g1 = pair_lane(f1[d]);
+        f0[d] += g0;  // the lower half's lane: its half plus the upper one
+        f1[d] += g1;
+      }
+    }
+    if (h == 0) {
+#pragma unroll
+      for (int d = 0; d < ND; d++) fe[(e * NN + m0) * ND + d] = f0[d];
+      if (two)
+#pragma unroll
+        for (int d = 0; d < ND; d++) fe[(e * NN + m1) * ND + d] = f1[d];
+    }
+  }
+}
+
 template <int ND, int NN>
 __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
   const int64_t nel = A.nel;
@@ -296,56 +354,8 @@ __device__ __forceinline__ void update_body(int64_t blk, const UpdArgs& A) {
     gq[ND * ND] = axisym ? (coef * sig[2]) / r : 0.0;
   }
   __syncthreads();
-  // f_e: each thread takes two nodes of an element (m and m + NH, so every T read serves both) over one
-  // half of the IPs; the two halves sit in neighbouring lanes and are added once, lower half first
-  constexpr int NH = (NN + 1) / 2;
-  const int split = nip >= 8 ? 2 : 1;  // uniform
-  const int qh = split == 2 ? (nip + 1) / 2 : nip;
-  const int nwork = nloc * NH * split;  // even when split (the lanes of a pair are active together)
-  for (int w = tid; w < nwork; w += blockDim.x) {
-    const int h = split == 2 ? (w & 1) : 0;
-    const int pw = split == 2 ? (w >> 1) : w;
-    const int el = pw / NH, m0 = pw - el * NH;
-    const bool two = (NN % 2 == 0) || m0 + NH < NN;
-    const int m1 = two ? m0 + NH : m0;
-    const int64_t e = e0 + el;
-    double f0[ND], f1[ND];
-#pragma unroll
-    for (int d = 0; d < ND; d++) f0[d] = f1[d] = 0.0;
-    const int qa = h ? qh : 0, qb = h ? nip : qh;
-    const double* T = sG + (el * nip + qa) * GW;
-    for (int q = qa; q < qb; q++, T += GW) {
-      const double* dN0 = sdN + q * DS + m0 * ND;
-      const double* dN1 = sdN + q * DS + m1 * ND;
-#pragma unroll
-      for (int i = 0; i < ND; i++)
-#pragma unroll
-        for (int k = 0; k < ND; k++) {
-          const double t = T[i * ND + k];
-          f0[i] += t * dN0[k];
-          f1[i] += t * dN1[k];
-        }
-      if (axisym) {
-        f0[0] += T[ND * ND] * tN[q * NN + m0];
-        f1[0] += T[ND * ND] * tN[q * NN + m1];
-      }
-    }
-    if (split == 2) {
-#pragma unroll
-      for (int d = 0; d < ND; d++) {
-        const double g0 = pair_lane(f0[d]), g1 = pair_lane(f1[d]);
-        f0[d] += g0;  // the lower half's lane: its half plus the upper one
-        f1[d] += g1;
-      }
-    }
-    if (h == 0) {
-#pragma unroll
-      for (int d = 0; d < ND; d++) fe[(e * NN + m0) * ND + d] = f0[d];
-      if (two)
-#pragma unroll
-        for (int d = 0; d < ND; d++) fe[(e * NN + m1) * ND + d] = f1[d];
-    }
-  }
+  if (nip >= 8) force_loop<ND, NN, true>(sdN, sG, tN, fe, e0, nloc, nip, axisym);  // uniform
+  else force_loop<ND, NN, false>(sdN, sG, tN, fe, e0, nloc, nip, axisym);
 }
 
 // ------------------------------------------------------------------------------------------------
