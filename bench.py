@@ -109,7 +109,8 @@ def cpu_baseline(seconds=12.0, seconds_all=6.0, n=(32, 32, 16)):
     host core (the reported value) and, as `all_cores`, with its OpenMP element loops on every core this process
     may use.  The sample (32x32x16 Hex20: 16,384 elements, 442,368 IPs, 35 MB of IP state and its backup) is
     larger than one CCD's L3; the rate does not depend on the sample size (compute-bound).  `full_solve` times
-    the 16x8 Qua8 strip footing (20 increments) in the oracle beside the GPU solve of the same problem."""
+    the 16x8 Qua8 strip footing (20 increments) in the oracle beside the GPU solve of the same problem, and
+    `hex20_solve` the Hex20 cube's loading on a 4x4x2 mesh likewise."""
     from oracle import oracle as orc
     mesh, base, cfg, d = build_problem(*n)
     o = orc.Oracle(d)
@@ -130,7 +131,37 @@ def cpu_baseline(seconds=12.0, seconds_all=6.0, n=(32, 32, 16)):
         out["all_cores"] = dict(value=nip * reps2 / t2, unit="IP-updates/s", cores=nt,
                                 sample=f"{tag} x {reps2} passes, C++ oracle with OpenMP element loops on {nt} threads")
     out["full_solve"] = full_solve_baseline()
+    out["hex20_solve"] = hex20_solve_baseline()
     return out
+
+
+def hex20_solve_baseline(n=(4, 4, 2)):
+    """The Hex20 cube of the element-pass workload at a size the oracle solves in seconds (4x4x2, the same
+    material and loading: top uz to -0.01 in 10 increments, 27 IPs, plastic almost everywhere at the end): the
+    serial oracle (banded LU on the literal LMM system) against the GPU solve (FP64, PCG) of the same problem."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from oracle import oracle as orc
+    from pmsim_amd import describe
+    from pmsim_amd.solver import Fem
+    mesh, base, cfg, ess, nat = cases.hex20_cube(*n, n_inc=10, uz=-0.01)
+    cfg.set_n_max_iterations(60)
+    d = describe(mesh, base, cfg, ess, nat)
+    t0 = time.time()
+    uo, _, its = orc.Oracle(d).solve()
+    cpu_s = time.time() - t0
+    cfg.set_lin_sol("pcg")
+    d = describe(mesh, base, cfg, ess, nat)
+    Fem(d, cfg, keep_history=False).solve()  # warm-up (code objects, graphs)
+    g = Fem(d, cfg, keep_history=False)
+    t0 = time.time()
+    g.solve()
+    gpu_s = time.time() - t0
+    u = g.get_u()[0]
+    return {"workload": f"Hex20 cube {n[0]}x{n[1]}x{n[2]}, 27 IP, von Mises, top uz to -0.01 in 10 increments",
+            "cpu_s": cpu_s, "cpu_cores": 1, "gpu_s": gpu_s, "newton_cpu": int(np.abs(its).sum()),
+            "newton_gpu": g.stats()["n_iterations"],
+            "u_rel_diff": float(np.abs(u - uo[-1][: g.ndof]).max() / np.abs(uo[-1][: g.ndof]).max())}
 
 
 def full_solve_baseline():
